@@ -1,0 +1,212 @@
+/*
+ * orrery_hip.h -- C ABI of the MI355X force engine for Orrery.
+ *
+ * The engine replaces the device halves of the reference's CUDA solvers
+ * (src/solvers/cuda_direct_kernel.cu, cuda_tree_kernel.cu and the host halves
+ * in cuda_direct_solver.cpp / cuda_tree_solver.cpp) behind the same
+ * ForceSolver seam (include/orrery/solvers/force_solver.hpp:41-87). The C++
+ * classes in orrery_amd/host/ and the Python classes in orrery_amd/solvers.py
+ * are thin hosts over these entry points.
+ *
+ * Conventions (after the reference's launch boundary,
+ * include/orrery/solvers/cuda_kernels.hpp:35-111, and its only C ABI,
+ * wasm/orrery_wasm.h:48-203):
+ *   - every entry point returns an int status, ORR_OK (0) on success; the
+ *     message of the last failure on the calling thread is orr_last_error();
+ *   - arguments are plain pointers and sizes bundled in POD structs; no C++
+ *     or torch type crosses the boundary;
+ *   - pointers named "device" are HIP device addresses of this context's
+ *     device; every evaluation is synchronous: it returns once the results
+ *     are in place (cuda_kernels.hpp:108-110);
+ *   - particle arrays are structure-of-arrays FP64 (the reference's Real in
+ *     its double build), accelerations are written, never accumulated
+ *     (include/orrery/integrators/acceleration_field.hpp:48-49).
+ */
+#ifndef ORRERY_HIP_H
+#define ORRERY_HIP_H
+
+#include <stdint.h>
+
+/* Exported from liborrery_hip.so; everything else in the library is hidden. */
+#if defined(__GNUC__) || defined(__clang__)
+#define ORR_API __attribute__((visibility("default")))
+#else
+#define ORR_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. Mirrors the exception kinds of the reference's CUDA host
+ * (backend::CudaError for runtime failures, cuda_memory.hpp:129-153;
+ * std::length_error for counts beyond 2^32, cuda_tree_solver.cpp:322-325). */
+#define ORR_OK 0
+#define ORR_ERR_INVALID_ARGUMENT 1
+#define ORR_ERR_NO_DEVICE 2
+#define ORR_ERR_RUNTIME 3
+#define ORR_ERR_LENGTH 4
+#define ORR_ERR_OUT_OF_MEMORY 5
+
+typedef struct orr_ctx orr_ctx;
+typedef struct orr_system orr_system;
+
+/* ---- device discovery: replaces backend/cuda_device.hpp:61-209 ---------- */
+
+typedef struct {
+    char name[256];
+    char arch[64];               /* e.g. "gfx950" */
+    int32_t compute_units;
+    int32_t wavefront_size;
+    int32_t max_threads_per_block;
+    int32_t clock_khz;
+    uint64_t global_memory_bytes;
+    uint64_t lds_per_block_bytes;
+    uint64_t l2_bytes;
+} orr_device_desc;
+
+/* Number of HIP devices (0 when none; never an error for "no device"). */
+ORR_API int orr_device_count(int32_t* count);
+/* CudaDeviceDescription equivalent (cuda_device.hpp:61-100). */
+ORR_API int orr_device_describe(int32_t device, orr_device_desc* out);
+/* Message of the last failure on this thread (wasm/orrery_wasm.h:86-94). */
+ORR_API const char* orr_last_error(void);
+
+/* ---- context: one device, one stream, growing scratch buffers ----------- */
+
+/* Replaces CudaDirectSolver::try_create / CudaTreeSolver::try_create device
+ * set-up (cuda_direct_solver.cpp:140-164): ORR_ERR_NO_DEVICE when there is
+ * none, so hosts can return null and fall back (sim/assembly.cpp:253-271). */
+ORR_API int orr_ctx_create(int32_t device, orr_ctx** out);
+ORR_API int orr_ctx_destroy(orr_ctx* ctx);
+
+/* Device-memory plumbing for hosts that own no allocator
+ * (backend/cuda_memory.hpp:77-348: CudaArray and its copies). */
+ORR_API int orr_malloc(orr_ctx* ctx, uint64_t bytes, void** device_out);
+ORR_API int orr_free(orr_ctx* ctx, void* device);
+ORR_API int orr_upload(orr_ctx* ctx, void* device_dst, const void* host_src, uint64_t bytes);
+ORR_API int orr_download(orr_ctx* ctx, void* host_dst, const void* device_src, uint64_t bytes);
+ORR_API int orr_memset(orr_ctx* ctx, void* device_dst, int32_t value, uint64_t bytes);
+
+/* ---- direct summation --------------------------------------------------- */
+
+/* Replaces CudaDirectArguments + launch_cuda_direct (cuda_kernels.hpp:65-111;
+ * cuda_direct_kernel.cu:87-214). a_i = G sum_{j != i} m_j d_ij / (r^2 + eps^2)^(3/2),
+ * the law of core/softening.hpp:80-100. FP32 pairwise terms on tile-relative
+ * coordinates, per-tile partial sums Kahan-added, FP64 result. */
+typedef struct {
+    const double* x;      /* device, count */
+    const double* y;
+    const double* z;
+    const double* m;
+    double* ax;           /* device, count; written */
+    double* ay;
+    double* az;
+    uint64_t count;
+    double softening_squared;
+    double gravitational_constant;
+    int32_t spatial_order;  /* 1: Morton-order the tiles first (default), 0: index order */
+} orr_direct_args;
+ORR_API int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* args);
+
+/* ---- Barnes-Hut --------------------------------------------------------- */
+
+/* Replaces CudaTreeArguments + launch_cuda_tree (cuda_kernels.hpp:137-229) and
+ * moves the reference's host ordering, sort and build (cuda_tree_solver.cpp:306-446)
+ * onto the device. TreeParameters of solvers/octree.hpp:147-199; theta is
+ * clamped to [0, 1] and leaf_capacity to >= 1 (octree.cpp:189-194). */
+typedef struct {
+    const double* x;      /* device, count */
+    const double* y;
+    const double* z;
+    const double* m;
+    double* ax;           /* device, count; written */
+    double* ay;
+    double* az;
+    uint64_t count;
+    double softening_squared;
+    double gravitational_constant;
+    double opening_angle;
+    uint64_t leaf_capacity;
+    int32_t quadrupole;
+} orr_tree_args;
+
+/* Interaction counts (solvers/interaction_count.hpp:37-76) and a host-readable
+ * tree summary (Octree::leaf_count / depth, octree.hpp:229-236). */
+typedef struct {
+    uint64_t particle_particle;
+    uint64_t particle_cell;
+    uint64_t node_visits;
+    uint64_t node_count;
+    uint64_t leaf_count;
+    uint32_t depth;
+    uint32_t reserved;
+} orr_tree_stats;
+ORR_API int orr_tree_eval(orr_ctx* ctx, const orr_tree_args* args, orr_tree_stats* stats);
+
+/* Morton ordering on the device: MortonOrdering::build (solvers/morton.cpp:180-197).
+ * Writes the bounding cube {origin x, y, z, size} to host cube_out and the
+ * ascending (code, index) keys to device arrays of `count` entries. */
+ORR_API int orr_morton_order(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                     uint64_t count, uint64_t* codes_out, uint32_t* index_out,
+                     double cube_out[4]);
+
+/* Tree nodes of the last orr_tree_eval in DFS order (TreeNode, octree.hpp:106-139),
+ * copied to host arrays of node_count entries; any pointer may be NULL. */
+ORR_API int orr_tree_nodes(orr_ctx* ctx, double* mass, double* com_x, double* com_y, double* com_z,
+                   double* acceptance_radius_squared, uint64_t* next,
+                   uint64_t* first_particle, uint64_t* particle_count);
+
+/* ---- phase timings (hipEvents on the context stream) -------------------- */
+
+/* Phases: 0 direct prepare, 1 direct kernel, 2 direct reduce, 3 bbox, 4 encode,
+ * 5 sort, 6 gather, 7 build, 8 moments, 9 walk, 10 integrate, 11 energy.
+ * Enabled per context; the reference's EvaluationTimings / CudaEvaluationTimings
+ * (barnes_hut_solver.hpp:92-105; cuda_direct_solver.hpp:85-...) carry the same
+ * information. Milliseconds summed since the last reset, with launch counts. */
+#define ORR_PHASES 16
+typedef struct {
+    double milliseconds[ORR_PHASES];
+    uint64_t launches[ORR_PHASES];
+} orr_timings;
+ORR_API int orr_ctx_set_timing(orr_ctx* ctx, int32_t enabled);
+ORR_API int orr_ctx_timings(orr_ctx* ctx, orr_timings* out);
+ORR_API int orr_ctx_reset_timings(orr_ctx* ctx);
+
+/* ---- device-resident systems: state stays in HBM across steps ----------- */
+
+/* Integrators: 0 velocity Verlet (integrators/velocity_verlet.cpp:20-72),
+ * 1 Yoshida-4 (integrators/yoshida4.cpp:24-59). Solvers: 0 direct, 1 Barnes-Hut.
+ * FP64 kick and drift on the device; the field is the FP32-term engine above. */
+typedef struct {
+    int32_t integrator;
+    int32_t solver;
+    double softening_squared;
+    double gravitational_constant;
+    double opening_angle;
+    uint64_t leaf_capacity;
+    int32_t quadrupole;
+    int32_t reserved;
+} orr_step_config;
+
+ORR_API int orr_system_create(orr_ctx* ctx, uint64_t count, orr_system** out);
+ORR_API int orr_system_destroy(orr_system* sys);
+/* Host arrays of `count` FP64 values each. */
+ORR_API int orr_system_upload(orr_system* sys, const double* x, const double* y, const double* z,
+                      const double* vx, const double* vy, const double* vz, const double* m);
+ORR_API int orr_system_download(orr_system* sys, double* x, double* y, double* z, double* vx,
+                        double* vy, double* vz, double* ax, double* ay, double* az);
+/* refresh_accelerations (integrators/integrator.hpp:51-53). */
+ORR_API int orr_system_refresh(orr_system* sys, const orr_step_config* config);
+ORR_API int orr_system_step(orr_system* sys, const orr_step_config* config, double timestep,
+                    uint64_t steps);
+/* measure_diagnostics (core/diagnostics.cpp:179-200) in FP64 on the device:
+ * out = {kinetic, potential, px, py, pz, lx, ly, lz}. */
+ORR_API int orr_system_diagnostics(orr_system* sys, double softening_squared,
+                           double gravitational_constant, double out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,425 @@
+// extern "C" surface of the engine (include/orrery_hip.h).
+//
+// Each entry point validates its arguments, runs the C++ implementation and
+// maps any orr::Error to its status with a thread-local message: the wasm C
+// ABI's convention (wasm/orrery_wasm.h:86-94) applied to the device engine.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "engine.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template<typename F> int guarded(F&& body) noexcept {
+    try {
+        body();
+        return ORR_OK;
+    } catch (const orr::Error& e) {
+        g_last_error = e.what();
+        return e.status();
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return ORR_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return ORR_ERR_RUNTIME;
+    } catch (...) {
+        g_last_error = "unknown failure";
+        return ORR_ERR_RUNTIME;
+    }
+}
+
+void require(bool ok, const char* what) {
+    if (!ok) orr::fail(ORR_ERR_INVALID_ARGUMENT, what);
+}
+
+void use_device(orr_ctx* ctx) { ORR_HIP(hipSetDevice(ctx->device)); }
+
+}  // namespace
+
+struct orr_system {
+    orr_ctx* ctx = nullptr;
+    std::uint64_t count = 0;
+    orr::Buffer state;  // x y z vx vy vz ax ay az m, FP64, count each
+    double* array(int k) const { return state.as<double>() + static_cast<std::size_t>(k) * count; }
+};
+
+namespace {
+
+void field_eval(orr_system* s, const orr_step_config& c) {
+    if (c.solver == 1) {
+        orr_tree_args t{};
+        t.x = s->array(0);
+        t.y = s->array(1);
+        t.z = s->array(2);
+        t.m = s->array(9);
+        t.ax = s->array(6);
+        t.ay = s->array(7);
+        t.az = s->array(8);
+        t.count = s->count;
+        t.softening_squared = c.softening_squared;
+        t.gravitational_constant = c.gravitational_constant;
+        t.opening_angle = c.opening_angle;
+        t.leaf_capacity = c.leaf_capacity;
+        t.quadrupole = c.quadrupole;
+        orr::tree_evaluate(s->ctx, t, nullptr);
+    } else {
+        orr_direct_args d{};
+        d.x = s->array(0);
+        d.y = s->array(1);
+        d.z = s->array(2);
+        d.m = s->array(9);
+        d.ax = s->array(6);
+        d.ay = s->array(7);
+        d.az = s->array(8);
+        d.count = s->count;
+        d.softening_squared = c.softening_squared;
+        d.gravitational_constant = c.gravitational_constant;
+        d.spatial_order = 1;
+        const std::uint32_t* order = nullptr;
+        if (s->count > 64) {
+            orr::morton_enqueue(s->ctx, d.x, d.y, d.z, s->count);
+            order = s->ctx->order_index.as<std::uint32_t>();
+        }
+        orr::direct_enqueue(s->ctx, d, order);
+    }
+}
+
+// velocity_verlet.cpp:59-72: kick half, drift, evaluate, kick half.
+void vv_substep(orr_system* s, const orr_step_config& c, double dt) {
+    const double half = dt / 2;
+    const std::uint64_t n = s->count;
+    orr::kick(s->ctx, n, s->array(3), s->array(4), s->array(5), s->array(6), s->array(7),
+              s->array(8), half);
+    orr::drift(s->ctx, n, s->array(0), s->array(1), s->array(2), s->array(3), s->array(4),
+               s->array(5), dt);
+    field_eval(s, c);
+    orr::kick(s->ctx, n, s->array(3), s->array(4), s->array(5), s->array(6), s->array(7),
+              s->array(8), half);
+}
+
+// yoshida4.cpp:24-41 weights, evaluated in FP64 exactly as the reference's constexprs.
+constexpr double kCubeRootOfTwo = 1.2599210498948731647672106;
+constexpr double kOuterWeight = 1.0 / (2 - kCubeRootOfTwo);
+constexpr double kInnerWeight = -kCubeRootOfTwo / (2 - kCubeRootOfTwo);
+
+void check_config(const orr_step_config* c) {
+    require(c != nullptr, "step config is null");
+    require(c->integrator == 0 || c->integrator == 1, "integrator must be 0 (VV) or 1 (Yoshida-4)");
+    require(c->solver == 0 || c->solver == 1, "solver must be 0 (direct) or 1 (Barnes-Hut)");
+    require(c->softening_squared >= 0, "softening squared must be >= 0");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* orr_last_error(void) { return g_last_error.c_str(); }
+
+int orr_device_count(int32_t* count) {
+    return guarded([&] {
+        require(count != nullptr, "count is null");
+        int n = 0;
+        const hipError_t e = hipGetDeviceCount(&n);
+        if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) n = 0;
+        else ORR_HIP(e);
+        *count = n;
+    });
+}
+
+int orr_device_describe(int32_t device, orr_device_desc* out) {
+    return guarded([&] {
+        require(out != nullptr, "out is null");
+        hipDeviceProp_t p{};
+        ORR_HIP(hipGetDeviceProperties(&p, device));
+        std::memset(out, 0, sizeof(*out));
+        std::strncpy(out->name, p.name, sizeof(out->name) - 1);
+        std::strncpy(out->arch, p.gcnArchName, sizeof(out->arch) - 1);
+        out->compute_units = p.multiProcessorCount;
+        out->wavefront_size = p.warpSize;
+        out->max_threads_per_block = p.maxThreadsPerBlock;
+        out->clock_khz = p.clockRate;
+        out->global_memory_bytes = p.totalGlobalMem;
+        out->lds_per_block_bytes = p.sharedMemPerBlock;
+        out->l2_bytes = static_cast<uint64_t>(p.l2CacheSize);
+    });
+}
+
+int orr_ctx_create(int32_t device, orr_ctx** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is null");
+        *out = nullptr;
+        int n = 0;
+        const hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n == 0) orr::fail(ORR_ERR_NO_DEVICE, "no HIP device");
+        require(device >= 0 && device < n, "device index out of range");
+        ORR_HIP(hipSetDevice(device));
+        auto ctx = std::make_unique<orr_ctx>();
+        ctx->device = device;
+        hipDeviceProp_t p{};
+        ORR_HIP(hipGetDeviceProperties(&p, device));
+        ctx->compute_units = p.multiProcessorCount;
+        // Default priority on purpose: streams of mixed priority are not used.
+        ORR_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        *out = ctx.release();
+    });
+}
+
+int orr_ctx_destroy(orr_ctx* ctx) {
+    return guarded([&] {
+        if (ctx == nullptr) return;
+        use_device(ctx);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (orr::Buffer* b : {&ctx->direct_packed, &ctx->direct_centres, &ctx->direct_partials,
+                               &ctx->order_codes, &ctx->order_index, &ctx->order_codes_alt,
+                               &ctx->order_index_alt, &ctx->order_hist, &ctx->order_bbox,
+                               &ctx->order_flags, &ctx->tree_nodes, &ctx->tree_aux,
+                               &ctx->tree_work, &ctx->tree_counts, &ctx->tree_sorted,
+                               &ctx->tree_leafref}) {
+            b->release();
+        }
+        for (auto& p : ctx->pending) {
+            (void)hipEventDestroy(p.start);
+            (void)hipEventDestroy(p.stop);
+        }
+        for (hipEvent_t e : ctx->free_events) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int orr_malloc(orr_ctx* ctx, uint64_t bytes, void** out) {
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "null argument");
+        use_device(ctx);
+        *out = nullptr;
+        ORR_HIP(hipMalloc(out, bytes == 0 ? 1 : bytes));
+    });
+}
+
+int orr_free(orr_ctx* ctx, void* p) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        use_device(ctx);
+        if (p != nullptr) ORR_HIP(hipFree(p));
+    });
+}
+
+int orr_upload(orr_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] {
+        require(ctx != nullptr && (bytes == 0 || (dst != nullptr && src != nullptr)), "null argument");
+        use_device(ctx);
+        if (bytes == 0) return;
+        ORR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        ctx->sync();
+    });
+}
+
+int orr_download(orr_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] {
+        require(ctx != nullptr && (bytes == 0 || (dst != nullptr && src != nullptr)), "null argument");
+        use_device(ctx);
+        if (bytes == 0) return;
+        ORR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+    });
+}
+
+int orr_memset(orr_ctx* ctx, void* dst, int32_t value, uint64_t bytes) {
+    return guarded([&] {
+        require(ctx != nullptr && (bytes == 0 || dst != nullptr), "null argument");
+        use_device(ctx);
+        if (bytes == 0) return;
+        ORR_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+        ctx->sync();
+    });
+}
+
+int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* a) {
+    return guarded([&] {
+        require(ctx != nullptr && a != nullptr, "null argument");
+        require(a->softening_squared >= 0, "softening squared must be >= 0");
+        if (a->count == 0) return;
+        require(a->x && a->y && a->z && a->m && a->ax && a->ay && a->az, "null particle array");
+        use_device(ctx);
+        const std::uint32_t* order = nullptr;
+        if (a->spatial_order != 0 && a->count > 64) {
+            orr::morton_enqueue(ctx, a->x, a->y, a->z, a->count);
+            order = ctx->order_index.as<std::uint32_t>();
+        }
+        orr::direct_enqueue(ctx, *a, order);
+        ctx->sync();
+    });
+}
+
+int orr_tree_eval(orr_ctx* ctx, const orr_tree_args* a, orr_tree_stats* stats) {
+    return guarded([&] {
+        require(ctx != nullptr && a != nullptr, "null argument");
+        require(a->softening_squared >= 0, "softening squared must be >= 0");
+        if (a->count > 0) {
+            require(a->x && a->y && a->z && a->m && a->ax && a->ay && a->az, "null particle array");
+        }
+        use_device(ctx);
+        orr::tree_evaluate(ctx, *a, stats);
+        ctx->sync();
+    });
+}
+
+int orr_morton_order(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                     uint64_t n, uint64_t* codes_out, uint32_t* index_out, double cube_out[4]) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        use_device(ctx);
+        if (n == 0) {
+            if (cube_out) cube_out[0] = cube_out[1] = cube_out[2] = cube_out[3] = 0.0;
+            return;
+        }
+        require(x && y && z && codes_out && index_out, "null array");
+        orr::morton_enqueue(ctx, x, y, z, n);
+        ORR_HIP(hipMemcpyAsync(codes_out, ctx->order_codes.ptr, n * sizeof(uint64_t),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+        ORR_HIP(hipMemcpyAsync(index_out, ctx->order_index.ptr, n * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+        double cube[4];
+        orr::morton_cube(ctx, cube);
+        if (cube_out) std::memcpy(cube_out, cube, sizeof(cube));
+    });
+}
+
+int orr_tree_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double* cz, double* acc,
+                   uint64_t* next, uint64_t* first, uint64_t* count) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        use_device(ctx);
+        orr::tree_copy_nodes(ctx, mass, cx, cy, cz, acc, next, first, count);
+    });
+}
+
+int orr_ctx_set_timing(orr_ctx* ctx, int32_t enabled) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        use_device(ctx);
+        ctx->sync();
+        ctx->timing = enabled != 0;
+    });
+}
+
+int orr_ctx_timings(orr_ctx* ctx, orr_timings* out) {
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "null argument");
+        use_device(ctx);
+        ctx->sync();
+        *out = ctx->totals;
+    });
+}
+
+int orr_ctx_reset_timings(orr_ctx* ctx) {
+    return guarded([&] {
+        require(ctx != nullptr, "null context");
+        use_device(ctx);
+        ctx->sync();
+        ctx->totals = orr_timings{};
+    });
+}
+
+int orr_system_create(orr_ctx* ctx, uint64_t count, orr_system** out) {
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "null argument");
+        *out = nullptr;
+        require(count <= (uint64_t{1} << 31), "more than 2^31 particles");
+        use_device(ctx);
+        auto s = std::make_unique<orr_system>();
+        s->ctx = ctx;
+        s->count = count;
+        s->state.reserve(sizeof(double) * 10 * (count == 0 ? 1 : count));
+        ORR_HIP(hipMemsetAsync(s->state.ptr, 0, s->state.bytes, ctx->stream));
+        ctx->sync();
+        *out = s.release();
+    });
+}
+
+int orr_system_destroy(orr_system* s) {
+    return guarded([&] {
+        if (s == nullptr) return;
+        use_device(s->ctx);
+        s->ctx->sync();
+        s->state.release();
+        delete s;
+    });
+}
+
+int orr_system_upload(orr_system* s, const double* x, const double* y, const double* z,
+                      const double* vx, const double* vy, const double* vz, const double* m) {
+    return guarded([&] {
+        require(s != nullptr, "null system");
+        use_device(s->ctx);
+        const double* src[10] = {x, y, z, vx, vy, vz, nullptr, nullptr, nullptr, m};
+        for (int k = 0; k < 10; ++k) {
+            if (src[k] == nullptr || s->count == 0) continue;
+            ORR_HIP(hipMemcpyAsync(s->array(k), src[k], s->count * sizeof(double),
+                                   hipMemcpyHostToDevice, s->ctx->stream));
+        }
+        s->ctx->sync();
+    });
+}
+
+int orr_system_download(orr_system* s, double* x, double* y, double* z, double* vx, double* vy,
+                        double* vz, double* ax, double* ay, double* az) {
+    return guarded([&] {
+        require(s != nullptr, "null system");
+        use_device(s->ctx);
+        double* dst[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+        for (int k = 0; k < 9; ++k) {
+            if (dst[k] == nullptr || s->count == 0) continue;
+            ORR_HIP(hipMemcpyAsync(dst[k], s->array(k), s->count * sizeof(double),
+                                   hipMemcpyDeviceToHost, s->ctx->stream));
+        }
+        s->ctx->sync();
+    });
+}
+
+int orr_system_refresh(orr_system* s, const orr_step_config* c) {
+    return guarded([&] {
+        require(s != nullptr, "null system");
+        check_config(c);
+        use_device(s->ctx);
+        if (s->count > 0) field_eval(s, *c);
+        s->ctx->sync();
+    });
+}
+
+int orr_system_step(orr_system* s, const orr_step_config* c, double dt, uint64_t steps) {
+    return guarded([&] {
+        require(s != nullptr, "null system");
+        check_config(c);
+        use_device(s->ctx);
+        if (s->count == 0) return;
+        for (uint64_t k = 0; k < steps; ++k) {
+            if (c->integrator == 1) {
+                vv_substep(s, *c, kOuterWeight * dt);
+                vv_substep(s, *c, kInnerWeight * dt);
+                vv_substep(s, *c, kOuterWeight * dt);
+            } else {
+                vv_substep(s, *c, dt);
+            }
+        }
+        s->ctx->sync();
+    });
+}
+
+int orr_system_diagnostics(orr_system* s, double e2, double G, double out[8]) {
+    return guarded([&] {
+        require(s != nullptr && out != nullptr, "null argument");
+        use_device(s->ctx);
+        orr::diagnostics(s->ctx, s->count, s->array(0), s->array(1), s->array(2), s->array(3),
+                         s->array(4), s->array(5), s->array(9), e2, G, out);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+// Internal interface of the MI355X force engine (not part of the C ABI).
+//
+// Errors inside the library are C++ exceptions carrying an ORR_* status; the
+// extern "C" layer (capi.hip) catches them and returns the status, so no
+// exception crosses the ABI (the reference's rule for its kernel translation
+// units, include/orrery/solvers/cuda_kernels.hpp:96-111).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/orrery_hip.h"
+
+namespace orr {
+
+class Error : public std::runtime_error {
+public:
+    Error(int status, const std::string& what) : std::runtime_error(what), status_(status) {}
+    int status() const noexcept { return status_; }
+
+private:
+    int status_;
+};
+
+[[noreturn]] void fail(int status, const std::string& what);
+void check_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define ORR_HIP(expr) ::orr::check_hip((expr), #expr, __FILE__, __LINE__)
+#define ORR_LAUNCHED(what) ::orr::check_hip(hipGetLastError(), what, __FILE__, __LINE__)
+
+// A device allocation that only grows (the reference's solver pimpl rule,
+// cuda_direct_solver.cpp:78-137).
+struct Buffer {
+    void* ptr = nullptr;
+    std::size_t bytes = 0;
+    template<typename T> T* as() const { return static_cast<T*>(ptr); }
+    void reserve(std::size_t wanted);
+    void release() noexcept;
+};
+
+enum Phase : int {
+    kDirectPrepare = 0,
+    kDirectKernel = 1,
+    kDirectReduce = 2,
+    kBbox = 3,
+    kEncode = 4,
+    kSort = 5,
+    kGather = 6,
+    kBuild = 7,
+    kMoments = 8,
+    kWalk = 9,
+    kIntegrate = 10,
+    kEnergy = 11,
+};
+
+}  // namespace orr
+
+// The context owns its stream, its scratch and its timing events.
+struct orr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int compute_units = 0;
+
+    // Scratch, grown on demand.
+    orr::Buffer direct_packed, direct_centres, direct_partials;
+    orr::Buffer order_codes, order_index, order_codes_alt, order_index_alt, order_hist,
+        order_bbox, order_flags;
+    orr::Buffer tree_nodes, tree_aux, tree_work, tree_counts, tree_sorted, tree_leafref;
+
+    // Host mirror of the last tree summary.
+    orr_tree_stats last_tree{};
+
+    // Phase timing with hipEvents recorded on `stream`.
+    bool timing = false;
+    struct Pending {
+        int phase;
+        hipEvent_t start, stop;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> free_events;
+    orr_timings totals{};
+
+    hipEvent_t take_event();
+    void begin(int phase, hipEvent_t& start);
+    void end(int phase, hipEvent_t start);
+    // Synchronises the stream and folds finished event pairs into `totals`.
+    void sync();
+};
+
+namespace orr {
+
+// RAII phase marker: records events around the launches it brackets.
+class PhaseTimer {
+public:
+    PhaseTimer(orr_ctx* ctx, int phase) : ctx_(ctx), phase_(phase) { ctx_->begin(phase_, start_); }
+    ~PhaseTimer() { ctx_->end(phase_, start_); }
+    PhaseTimer(const PhaseTimer&) = delete;
+    PhaseTimer& operator=(const PhaseTimer&) = delete;
+
+private:
+    orr_ctx* ctx_;
+    int phase_;
+    hipEvent_t start_ = nullptr;
+};
+
+// ---- entry points implemented by the kernel translation units -------------
+
+// direct.hip: enqueue the direct sum (no synchronisation).
+void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order);
+
+// morton.hip: bounding cube, codes and stable radix sort from identity order.
+// Leaves ascending codes and permutation in ctx->order_codes / order_index
+// (uint64 / uint32). Returns nothing on the host; cube lands in ctx->order_bbox.
+void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                    std::uint64_t n);
+// Copies the cube {origin, size} computed by the last morton_enqueue (syncs).
+void morton_cube(orr_ctx* ctx, double cube[4]);
+
+// tree.hip: order, build, moments and walk; fills stats (syncs).
+void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats);
+void tree_copy_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double* cz,
+                     double* acc, std::uint64_t* next, std::uint64_t* first,
+                     std::uint64_t* count);
+
+// integrate.hip: FP64 kick / drift / diagnostics on device-resident state.
+void kick(orr_ctx* ctx, std::uint64_t n, double* vx, double* vy, double* vz, const double* ax,
+          const double* ay, const double* az, double scale);
+void drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, const double* vx,
+           const double* vy, const double* vz, double scale);
+void diagnostics(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
+                 const double* z, const double* vx, const double* vy, const double* vz,
+                 const double* m, double softening_squared, double G, double out[8]);
+
+}  // namespace orr

@@ -1,0 +1,165 @@
+// FP64 integrator update and diagnostics on device-resident state.
+//
+// kick / drift follow src/integrators/velocity_verlet.cpp:20-55 (v += s a,
+// x += s v, one multiply and one add each, never fused: the file disables
+// contraction so the update rounds as the reference's portable build does).
+// The diagnostics follow src/core/diagnostics.cpp:85-200: kinetic energy,
+// momentum and angular momentum as Neumaier-compensated sums, potential
+// energy as rows i of plain sums over j > i, the rows then compensated.
+// Device reductions are deterministic: fixed chunks, each summed in index
+// order by one lane, then the chunk results summed in order by one lane.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "engine.hpp"
+
+#pragma clang fp contract(off)
+
+namespace orr {
+namespace {
+
+__global__ __launch_bounds__(256) void kick_kernel(std::uint64_t n, double* __restrict__ vx,
+                                                   double* __restrict__ vy,
+                                                   double* __restrict__ vz,
+                                                   const double* __restrict__ ax,
+                                                   const double* __restrict__ ay,
+                                                   const double* __restrict__ az, double s) {
+    const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vx[i] = vx[i] + s * ax[i];
+    vy[i] = vy[i] + s * ay[i];
+    vz[i] = vz[i] + s * az[i];
+}
+
+// Neumaier step (core/diagnostics.cpp:36-59).
+struct Compensated {
+    double total = 0.0, correction = 0.0;
+    __device__ void add(double v) {
+        const double u = total + v;
+        if (fabs(total) >= fabs(v)) {
+            correction += (total - u) + v;
+        } else {
+            correction += (v - u) + total;
+        }
+        total = u;
+    }
+    __device__ double value() const { return total + correction; }
+};
+
+constexpr int kTerms = 8;  // ke, px, py, pz, lx, ly, lz, pe
+
+// Per-particle terms, term-major [kTerms][n].
+__global__ __launch_bounds__(256) void diagnostic_terms(std::uint64_t n,
+                                                        const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ z,
+                                                        const double* __restrict__ vx,
+                                                        const double* __restrict__ vy,
+                                                        const double* __restrict__ vz,
+                                                        const double* __restrict__ m, double e2,
+                                                        double G, double* __restrict__ terms) {
+    const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double mi = m[i];
+    const double ux = vx[i], uy = vy[i], uz = vz[i];
+    const double xi = x[i], yi = y[i], zi = z[i];
+    terms[i] = mi * ((ux * ux) + (uy * uy) + (uz * uz));
+    terms[n + i] = ux * mi;
+    terms[2 * n + i] = uy * mi;
+    terms[3 * n + i] = uz * mi;
+    terms[4 * n + i] = ((yi * uz) - (zi * uy)) * mi;
+    terms[5 * n + i] = ((zi * ux) - (xi * uz)) * mi;
+    terms[6 * n + i] = ((xi * uy) - (yi * ux)) * mi;
+    // Row i of the potential: plain FP64 sum over j > i in index order.
+    double inter = 0.0;
+    for (std::uint64_t j = i + 1; j < n; ++j) {
+        const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+        inter += m[j] * (1.0 / sqrt(((dx * dx) + (dy * dy) + (dz * dz)) + e2));
+    }
+    terms[7 * n + i] = -G * mi * inter;
+}
+
+// One lane per (term, chunk): compensated sum of the chunk in index order.
+__global__ __launch_bounds__(64) void chunk_sums(std::uint64_t n, const double* __restrict__ terms,
+                                                 std::uint32_t chunks,
+                                                 double* __restrict__ out /* [kTerms][chunks][2] */) {
+    const std::uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= kTerms * chunks) return;
+    const std::uint32_t term = id / chunks, chunk = id % chunks;
+    const std::uint64_t per = (n + chunks - 1) / chunks;
+    const std::uint64_t b = chunk * per;
+    const std::uint64_t e = b + per < n ? b + per : n;
+    Compensated s;
+    for (std::uint64_t i = b; i < e; ++i) s.add(terms[term * n + i]);
+    out[2 * id] = s.total;
+    out[2 * id + 1] = s.correction;
+}
+
+__global__ void final_sums(std::uint32_t chunks, const double* __restrict__ parts,
+                           double* __restrict__ result) {
+    const int term = threadIdx.x;
+    if (term >= kTerms) return;
+    Compensated s;
+    for (std::uint32_t c = 0; c < chunks; ++c) {
+        s.add(parts[2 * (term * chunks + c)]);
+        s.add(parts[2 * (term * chunks + c) + 1]);
+    }
+    result[term] = s.value();
+}
+
+}  // namespace
+
+void kick(orr_ctx* ctx, std::uint64_t n, double* vx, double* vy, double* vz, const double* ax,
+          const double* ay, const double* az, double s) {
+    if (n == 0) return;
+    PhaseTimer t(ctx, kIntegrate);
+    hipLaunchKernelGGL(kick_kernel, dim3(static_cast<std::uint32_t>((n + 255) / 256)), dim3(256),
+                       0, ctx->stream, n, vx, vy, vz, ax, ay, az, s);
+    ORR_LAUNCHED("kick");
+}
+
+void drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, const double* vx,
+           const double* vy, const double* vz, double s) {
+    // Same arithmetic as a kick with the roles of the arrays exchanged.
+    kick(ctx, n, x, y, z, vx, vy, vz, s);
+}
+
+void diagnostics(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
+                 const double* z, const double* vx, const double* vy, const double* vz,
+                 const double* m, double e2, double G, double out[8]) {
+    for (int k = 0; k < 8; ++k) out[k] = 0.0;
+    if (n == 0) return;
+    const std::uint32_t chunks = static_cast<std::uint32_t>(n < 1024 ? n : 1024);
+    const std::size_t bytes = sizeof(double) * (kTerms * n + 2 * kTerms * chunks + kTerms);
+    Buffer scratch;
+    scratch.reserve(bytes);
+    double* terms = scratch.as<double>();
+    double* parts = terms + kTerms * n;
+    double* result = parts + 2 * kTerms * chunks;
+    try {
+        PhaseTimer t(ctx, kEnergy);
+        hipLaunchKernelGGL(diagnostic_terms, dim3(static_cast<std::uint32_t>((n + 255) / 256)),
+                           dim3(256), 0, ctx->stream, n, x, y, z, vx, vy, vz, m, e2, G, terms);
+        ORR_LAUNCHED("diagnostic_terms");
+        hipLaunchKernelGGL(chunk_sums, dim3((kTerms * chunks + 63) / 64), dim3(64), 0,
+                           ctx->stream, n, terms, chunks, parts);
+        ORR_LAUNCHED("chunk_sums");
+        hipLaunchKernelGGL(final_sums, dim3(1), dim3(64), 0, ctx->stream, chunks, parts, result);
+        ORR_LAUNCHED("final_sums");
+    } catch (...) {
+        scratch.release();
+        throw;
+    }
+    double host[kTerms];
+    ORR_HIP(hipMemcpyAsync(host, result, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    scratch.release();
+    // {kinetic, potential, p, L}: kinetic is half the summed m v^2 (diagnostics.cpp:197).
+    out[0] = host[0] / 2;
+    out[1] = host[7];
+    for (int k = 0; k < 6; ++k) out[2 + k] = host[1 + k];
+}
+
+}  // namespace orr

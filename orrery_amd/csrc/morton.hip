@@ -1,0 +1,293 @@
+// Morton ordering on gfx950: bounding cube, 63-bit keys, stable LSD radix sort.
+//
+// Bit-exact with the reference's MortonOrdering (src/solvers/morton.cpp:60-197):
+//   * the cube is the exact min/max box, size = largest extent (1 if not > 0),
+//     centred: origin = (lo + hi) / 2 - size / 2 (morton.cpp:60-104);
+//   * scale = 2^21 / size, cell = (c - origin) * scale with !(cell > 0) -> 0 and
+//     cell >= 2^21 -> 2^21 - 1, then truncation (:128-164) -- a subtract and a
+//     multiply, nothing to contract, and the file disables contraction anyway;
+//   * the 21-bit dilation masks of :106-122 and x-major interleave of :124-126;
+//   * a stable LSD radix sort starting from identity order reproduces the
+//     (code, index) lexicographic total order (morton.hpp:155-175).
+//
+// Sort: 8 passes of 8-bit digits. Each pass is histogram -> exclusive scan ->
+// scatter. The scatter ranks keys stably inside 256-key tiles with wave64
+// ballots (8 ballots give each lane the mask of lanes sharing its digit) and an
+// LDS table of per-wave digit counts, so no LDS atomics decide an order.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "engine.hpp"
+
+#pragma clang fp contract(off)
+
+namespace orr {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTilesPerBlock = 16;                  // keys per sort block = 4096
+constexpr int kKeysPerBlock = kBlock * kTilesPerBlock;
+constexpr unsigned kGrid = 1u << 21;
+
+// ---- bounding cube -----------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void bbox_partial(const double* __restrict__ x,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ z,
+                                                       std::uint64_t n,
+                                                       double* __restrict__ partial) {
+    __shared__ double s[6][kBlock];
+    const double inf = __builtin_huge_val();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    for (std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<std::uint64_t>(gridDim.x) * kBlock) {
+        const double v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = v[c] < lo[c] ? v[c] : lo[c];
+            hi[c] = v[c] > hi[c] ? v[c] : hi[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        s[c][threadIdx.x] = lo[c];
+        s[3 + c][threadIdx.x] = hi[c];
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double a = s[c][threadIdx.x + w];
+                if (a < s[c][threadIdx.x]) s[c][threadIdx.x] = a;
+                const double b = s[3 + c][threadIdx.x + w];
+                if (b > s[3 + c][threadIdx.x]) s[3 + c][threadIdx.x] = b;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Single thread: fold block extrema, then the cube arithmetic of morton.cpp:87-103.
+// cube = {origin x, y, z, size}.
+__global__ void bbox_final(const double* __restrict__ partial, std::uint32_t blocks,
+                           double* __restrict__ cube) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = partial[c];
+        hi[c] = partial[3 + c];
+    }
+    for (std::uint32_t b = 1; b < blocks; ++b) {
+        for (int c = 0; c < 3; ++c) {
+            const double a = partial[b * 6 + c];
+            const double h = partial[b * 6 + 3 + c];
+            lo[c] = a < lo[c] ? a : lo[c];
+            hi[c] = h > hi[c] ? h : hi[c];
+        }
+    }
+    const double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    // std::max({ex, ey, ez}) keeps the first of equals; the value is what matters.
+    double size = ex;
+    if (size < ey) size = ey;
+    if (size < ez) size = ez;
+    if (!(size > 0)) size = 1;
+    for (int c = 0; c < 3; ++c) {
+        const double centre = (lo[c] + hi[c]) / 2;
+        cube[c] = centre - size / 2;
+    }
+    cube[3] = size;
+}
+
+// ---- encode -----------------------------------------------------------------
+
+__device__ inline std::uint64_t spread_bits(std::uint32_t v) {
+    std::uint64_t s = v & 0x1fffffu;
+    s = (s | (s << 32)) & 0x1f00000000ffffull;
+    s = (s | (s << 16)) & 0x1f0000ff0000ffull;
+    s = (s | (s << 8)) & 0x100f00f00f00f00full;
+    s = (s | (s << 4)) & 0x10c30c30c30c30c3ull;
+    s = (s | (s << 2)) & 0x1249249249249249ull;
+    return s;
+}
+
+__device__ inline std::uint32_t to_grid(double coordinate, double origin, double scale) {
+    const double cell = (coordinate - origin) * scale;
+    if (!(cell > 0)) return 0;
+    if (cell >= static_cast<double>(kGrid)) return kGrid - 1;
+    return static_cast<std::uint32_t>(cell);
+}
+
+__global__ __launch_bounds__(kBlock) void encode(const double* __restrict__ x,
+                                                 const double* __restrict__ y,
+                                                 const double* __restrict__ z, std::uint64_t n,
+                                                 const double* __restrict__ cube,
+                                                 std::uint64_t* __restrict__ codes,
+                                                 std::uint32_t* __restrict__ index) {
+    const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double size = cube[3];
+    std::uint64_t code = 0;
+    if (size > 0) {
+        const double scale = static_cast<double>(kGrid) / size;
+        code = (spread_bits(to_grid(x[i], cube[0], scale)) << 2) |
+               (spread_bits(to_grid(y[i], cube[1], scale)) << 1) |
+               spread_bits(to_grid(z[i], cube[2], scale));
+    }
+    codes[i] = code;
+    index[i] = static_cast<std::uint32_t>(i);
+}
+
+// ---- radix sort -------------------------------------------------------------
+
+// hist is digit-major: hist[d * blocks + b] = count of digit d in block b.
+__global__ __launch_bounds__(kBlock) void radix_histogram(const std::uint64_t* __restrict__ keys,
+                                                          std::uint64_t n, int shift,
+                                                          std::uint32_t* __restrict__ hist) {
+    __shared__ std::uint32_t counts[256];
+    counts[threadIdx.x] = 0;
+    __syncthreads();
+    const std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * kKeysPerBlock;
+    for (int t = 0; t < kTilesPerBlock; ++t) {
+        const std::uint64_t i = base + static_cast<std::uint64_t>(t) * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&counts[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[static_cast<std::uint64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = counts[threadIdx.x];
+}
+
+// Exclusive scan of `count` entries by one block of 1024 threads.
+__global__ __launch_bounds__(1024) void exclusive_scan(std::uint32_t* __restrict__ data,
+                                                       std::uint32_t count) {
+    __shared__ std::uint32_t sums[1024];
+    const std::uint32_t per = (count + 1023) / 1024;
+    const std::uint32_t b = threadIdx.x * per;
+    const std::uint32_t e = min(count, b + per);
+    std::uint32_t local = 0;
+    for (std::uint32_t i = b; i < e; ++i) local += data[i];
+    sums[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+        const std::uint32_t v = threadIdx.x >= static_cast<unsigned>(off) ? sums[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sums[threadIdx.x] += v;
+        __syncthreads();
+    }
+    std::uint32_t run = sums[threadIdx.x] - local;
+    for (std::uint32_t i = b; i < e; ++i) {
+        const std::uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __restrict__ keys_in,
+                                                        const std::uint32_t* __restrict__ vals_in,
+                                                        std::uint64_t n, int shift,
+                                                        const std::uint32_t* __restrict__ offsets,
+                                                        std::uint64_t* __restrict__ keys_out,
+                                                        std::uint32_t* __restrict__ vals_out) {
+    __shared__ std::uint32_t running[256];        // base offset of each digit for the next tile
+    __shared__ std::uint32_t wave_counts[4][256];  // digit counts of each wave in this tile
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    running[threadIdx.x] = offsets[static_cast<std::uint64_t>(threadIdx.x) * gridDim.x + blockIdx.x];
+    for (int w = 0; w < 4; ++w) wave_counts[w][threadIdx.x] = 0;
+    __syncthreads();
+    const std::uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * kKeysPerBlock;
+    for (int t = 0; t < kTilesPerBlock; ++t) {
+        const std::uint64_t i = base + static_cast<std::uint64_t>(t) * kBlock + threadIdx.x;
+        const bool valid = i < n;
+        const std::uint64_t key = valid ? keys_in[i] : 0;
+        const std::uint32_t val = valid ? vals_in[i] : 0;
+        const std::uint32_t d = static_cast<std::uint32_t>((key >> shift) & 255u);
+        // Lanes of this wave holding a valid key with the same digit.
+        std::uint64_t same = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = ((d >> b) & 1u) != 0;
+            const std::uint64_t m = __ballot(bit);
+            same &= bit ? m : ~m;
+        }
+        const std::uint32_t rank = static_cast<std::uint32_t>(__popcll(same & lanes_below));
+        if (valid && rank == 0) wave_counts[wave][d] = static_cast<std::uint32_t>(__popcll(same));
+        __syncthreads();
+        if (valid) {
+            std::uint32_t pos = running[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wave_counts[w][d];
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        // Advance each digit's base past this tile and clear the wave table.
+        const std::uint32_t total = wave_counts[0][threadIdx.x] + wave_counts[1][threadIdx.x] +
+                                    wave_counts[2][threadIdx.x] + wave_counts[3][threadIdx.x];
+        running[threadIdx.x] += total;
+        for (int w = 0; w < 4; ++w) wave_counts[w][threadIdx.x] = 0;
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                    std::uint64_t n) {
+    if (n == 0) return;
+    if (n > (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "morton: more than 2^31 particles");
+    const std::uint32_t bbox_blocks = static_cast<std::uint32_t>(std::min<std::uint64_t>(1024, (n + kBlock - 1) / kBlock));
+    const std::uint32_t sort_blocks = static_cast<std::uint32_t>((n + kKeysPerBlock - 1) / kKeysPerBlock);
+    ctx->order_bbox.reserve(sizeof(double) * (6 * bbox_blocks + 4));
+    ctx->order_codes.reserve(sizeof(std::uint64_t) * n);
+    ctx->order_codes_alt.reserve(sizeof(std::uint64_t) * n);
+    ctx->order_index.reserve(sizeof(std::uint32_t) * n);
+    ctx->order_index_alt.reserve(sizeof(std::uint32_t) * n);
+    ctx->order_hist.reserve(sizeof(std::uint32_t) * 256 * sort_blocks);
+    double* partial = ctx->order_bbox.as<double>() + 4;
+    double* cube = ctx->order_bbox.as<double>();
+    {
+        PhaseTimer t(ctx, kBbox);
+        hipLaunchKernelGGL(bbox_partial, dim3(bbox_blocks), dim3(kBlock), 0, ctx->stream, x, y, z, n,
+                           partial);
+        ORR_LAUNCHED("bbox_partial");
+        hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, ctx->stream, partial, bbox_blocks, cube);
+        ORR_LAUNCHED("bbox_final");
+    }
+    std::uint64_t* keys[2] = {ctx->order_codes.as<std::uint64_t>(), ctx->order_codes_alt.as<std::uint64_t>()};
+    std::uint32_t* vals[2] = {ctx->order_index.as<std::uint32_t>(), ctx->order_index_alt.as<std::uint32_t>()};
+    {
+        PhaseTimer t(ctx, kEncode);
+        hipLaunchKernelGGL(encode, dim3(static_cast<std::uint32_t>((n + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, ctx->stream, x, y, z, n, cube, keys[0], vals[0]);
+        ORR_LAUNCHED("encode");
+    }
+    {
+        PhaseTimer t(ctx, kSort);
+        std::uint32_t* hist = ctx->order_hist.as<std::uint32_t>();
+        int cur = 0;
+        for (int pass = 0; pass < 8; ++pass) {
+            const int shift = 8 * pass;
+            hipLaunchKernelGGL(radix_histogram, dim3(sort_blocks), dim3(kBlock), 0, ctx->stream,
+                               keys[cur], n, shift, hist);
+            ORR_LAUNCHED("radix_histogram");
+            hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, ctx->stream, hist,
+                               256u * sort_blocks);
+            ORR_LAUNCHED("exclusive_scan");
+            hipLaunchKernelGGL(radix_scatter, dim3(sort_blocks), dim3(kBlock), 0, ctx->stream,
+                               keys[cur], vals[cur], n, shift, hist, keys[1 - cur], vals[1 - cur]);
+            ORR_LAUNCHED("radix_scatter");
+            cur = 1 - cur;
+        }
+        // Eight passes end back in the primary buffers (cur == 0).
+    }
+}
+
+void morton_cube(orr_ctx* ctx, double cube[4]) {
+    ORR_HIP(hipMemcpyAsync(cube, ctx->order_bbox.ptr, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ctx->sync();
+}
+
+}  // namespace orr

@@ -1,0 +1,472 @@
+"""Host classes of the MI355X engine, mirroring the reference's solver seam.
+
+``HipDirectSolver`` and ``HipTreeSolver`` implement the ForceSolver contract of
+include/orrery/solvers/force_solver.hpp:41-87 and
+include/orrery/integrators/acceleration_field.hpp:37-70 over the C ABI:
+
+* ``evaluate(positions, masses, accelerations)`` takes three equal-length SoA
+  component arrays for positions (the Vec3Span of core/vec3_span.hpp:33-74), the
+  masses, and three arrays it *writes* (never accumulates);
+* ``name()``, ``softening()``, ``interaction_count()``, ``reset_interaction_count()``;
+* ``try_create(...)`` returns None when there is no usable device, as
+  CudaDirectSolver::try_create does (cuda_direct_solver.cpp:140-164).
+
+Host arrays go through device staging (the reference CUDA solvers' H2D, kernel,
+D2H). ``evaluate_device`` takes device pointers instead, for state that already
+lives in HBM.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as C
+
+
+@dataclass(frozen=True)
+class Softening:
+    """Plummer softening held as its square (core/softening.hpp:50-78)."""
+
+    length_: float = 0.0
+
+    def squared(self) -> float:
+        return float(self.length_) * float(self.length_)
+
+    def length(self) -> float:
+        return math.sqrt(self.squared())
+
+
+@dataclass
+class InteractionCount:
+    """solvers/interaction_count.hpp:37-76."""
+
+    evaluations: int = 0
+    particle_particle: int = 0
+    particle_cell: int = 0
+
+
+@dataclass(frozen=True)
+class TreeParameters:
+    """solvers/octree.hpp:147-199; corrected as octree.cpp:189-194 does."""
+
+    opening_angle: float = 0.5
+    leaf_capacity: int = 32
+    quadrupole: bool = False
+
+    def corrected(self) -> "TreeParameters":
+        return TreeParameters(min(max(float(self.opening_angle), 0.0), 1.0),
+                              max(int(self.leaf_capacity), 1), bool(self.quadrupole))
+
+
+def device_count() -> int:
+    n = ctypes.c_int32()
+    C.check(C.load().orr_device_count(ctypes.byref(n)))
+    return int(n.value)
+
+
+def describe(device: int = 0) -> dict:
+    d = C.OrrDeviceDesc()
+    C.check(C.load().orr_device_describe(device, ctypes.byref(d)))
+    return dict(name=d.name.decode(), arch=d.arch.decode(), compute_units=d.compute_units,
+                wavefront_size=d.wavefront_size, clock_khz=d.clock_khz,
+                global_memory_bytes=d.global_memory_bytes, l2_bytes=d.l2_bytes)
+
+
+class Context:
+    """One device, one stream, the engine's scratch (orr_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = C.load()
+        h = ctypes.c_void_p()
+        C.check(self._lib.orr_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.check(self._lib.orr_ctx_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- device memory ---------------------------------------------------
+    def alloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        C.check(self._lib.orr_malloc(self.handle, int(nbytes), ctypes.byref(p)))
+        return int(p.value or 0)
+
+    def free(self, ptr: int) -> None:
+        C.check(self._lib.orr_free(self.handle, ctypes.c_void_p(ptr)))
+
+    def upload(self, dst: int, src: np.ndarray) -> None:
+        src = np.ascontiguousarray(src)
+        C.check(self._lib.orr_upload(self.handle, ctypes.c_void_p(dst),
+                                     src.ctypes.data_as(ctypes.c_void_p), src.nbytes))
+
+    def download(self, dst: np.ndarray, src: int) -> None:
+        assert dst.flags.c_contiguous
+        C.check(self._lib.orr_download(self.handle, dst.ctypes.data_as(ctypes.c_void_p),
+                                       ctypes.c_void_p(src), dst.nbytes))
+
+    # -- timings -----------------------------------------------------------
+    def set_timing(self, enabled: bool) -> None:
+        C.check(self._lib.orr_ctx_set_timing(self.handle, 1 if enabled else 0))
+
+    def timings(self) -> dict:
+        t = C.OrrTimings()
+        C.check(self._lib.orr_ctx_timings(self.handle, ctypes.byref(t)))
+        return {C.PHASE_NAMES[k]: (float(t.milliseconds[k]), int(t.launches[k]))
+                for k in range(C.ORR_PHASES) if t.launches[k]}
+
+    def reset_timings(self) -> None:
+        C.check(self._lib.orr_ctx_reset_timings(self.handle))
+
+
+class DeviceArray:
+    """A 1-D device allocation of a numpy dtype (backend/cuda_memory.hpp's CudaArray)."""
+
+    def __init__(self, ctx: Context, count: int, dtype=np.float64):
+        self.ctx = ctx
+        self.count = int(count)
+        self.dtype = np.dtype(dtype)
+        self.ptr = ctx.alloc(max(1, self.count) * self.dtype.itemsize)
+
+    @classmethod
+    def from_host(cls, ctx: Context, host: np.ndarray) -> "DeviceArray":
+        a = cls(ctx, host.shape[0], host.dtype)
+        if a.count:
+            ctx.upload(a.ptr, host)
+        return a
+
+    def to_host(self, out: np.ndarray | None = None) -> np.ndarray:
+        if out is None:
+            out = np.empty(self.count, dtype=self.dtype)
+        if self.count:
+            self.ctx.download(out, self.ptr)
+        return out
+
+    def upload(self, host: np.ndarray) -> None:
+        assert host.shape[0] == self.count and host.dtype == self.dtype
+        if self.count:
+            self.ctx.upload(self.ptr, host)
+
+    def release(self) -> None:
+        if self.ptr:
+            self.ctx.free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class _Staging:
+    """Grow-only device staging for host-array evaluations (cuda_direct_solver.cpp:78-137)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.capacity = 0
+        self.arrays: list[DeviceArray] = []
+
+    def reserve(self, n: int) -> list[DeviceArray]:
+        if n > self.capacity:
+            for a in self.arrays:
+                a.release()
+            cap = max(n, 1)
+            self.arrays = [DeviceArray(self.ctx, cap) for _ in range(7)]
+            self.capacity = cap
+        return self.arrays
+
+
+def _check_spans(positions, masses, accelerations):
+    x, y, z = positions
+    ax, ay, az = accelerations
+    n = len(masses)
+    for a in (x, y, z, ax, ay, az):
+        if len(a) != n:
+            raise ValueError("positions, masses and accelerations must have equal length")
+    return n
+
+
+class HipDirectSolver:
+    """Direct O(N^2) summation on the GPU; name ``hip-direct``.
+
+    Drop-in for CudaDirectSolver (include/orrery/solvers/cuda_direct_solver.hpp).
+    """
+
+    def __init__(self, softening: Softening | float = Softening(), context: Context | None = None,
+                 spatial_order: bool = True, gravitational_constant: float = 1.0):
+        self._softening = softening if isinstance(softening, Softening) else Softening(softening)
+        self.ctx = context or Context(0)
+        self.spatial_order = spatial_order
+        self.G = float(gravitational_constant)
+        self._count = InteractionCount()
+        self._staging = _Staging(self.ctx)
+
+    @classmethod
+    def try_create(cls, softening: Softening | float = Softening(), **kw):
+        try:
+            if device_count() == 0:
+                return None
+            return cls(softening, **kw)
+        except C.NoDeviceError:
+            return None
+
+    def name(self) -> str:
+        return "hip-direct"
+
+    def softening(self) -> Softening:
+        return self._softening
+
+    def interaction_count(self) -> InteractionCount:
+        return InteractionCount(**vars(self._count))
+
+    def reset_interaction_count(self) -> None:
+        self._count = InteractionCount()
+
+    def timings(self) -> dict:
+        return self.ctx.timings()
+
+    def _args(self, ptrs, n) -> C.OrrDirectArgs:
+        a = C.OrrDirectArgs()
+        a.x, a.y, a.z, a.m, a.ax, a.ay, a.az = ptrs
+        a.count = n
+        a.softening_squared = self._softening.squared()
+        a.gravitational_constant = self.G
+        a.spatial_order = 1 if self.spatial_order else 0
+        return a
+
+    def evaluate_device(self, x: int, y: int, z: int, m: int, ax: int, ay: int, az: int, n: int):
+        """Evaluate on device pointers (state already resident in HBM)."""
+        args = self._args((x, y, z, m, ax, ay, az), n)
+        C.check(C.load().orr_direct_eval(self.ctx.handle, ctypes.byref(args)))
+        self._tally(n)
+
+    def evaluate(self, positions, masses, accelerations) -> None:
+        n = _check_spans(positions, masses, accelerations)
+        if n == 0:
+            self._tally(0)
+            return
+        d = self._staging.reserve(n)
+        for dev, host in zip(d[:4], (*positions, masses)):
+            self.ctx.upload(dev.ptr, np.ascontiguousarray(host, dtype=np.float64))
+        args = self._args(tuple(a.ptr for a in d), n)
+        C.check(C.load().orr_direct_eval(self.ctx.handle, ctypes.byref(args)))
+        for dev, host in zip(d[4:], accelerations):
+            out = np.empty(n)
+            self.ctx.download(out, dev.ptr)
+            host[:] = out
+        self._tally(n)
+
+    def _tally(self, n: int) -> None:
+        # direct_solver.cpp:75-79: one evaluation, N(N-1) ordered pairs.
+        self._count.evaluations += 1
+        if n > 1:
+            self._count.particle_particle += n * (n - 1)
+
+
+class HipTreeSolver:
+    """Barnes-Hut with on-device ordering, build and walk; name ``hip-tree``.
+
+    Drop-in for CudaTreeSolver (include/orrery/solvers/cuda_tree_solver.hpp).
+    """
+
+    def __init__(self, parameters: TreeParameters = TreeParameters(),
+                 softening: Softening | float = Softening(), context: Context | None = None,
+                 gravitational_constant: float = 1.0):
+        self._parameters = parameters.corrected()
+        self._softening = softening if isinstance(softening, Softening) else Softening(softening)
+        self.ctx = context or Context(0)
+        self.G = float(gravitational_constant)
+        self._count = InteractionCount()
+        self._visits = 0
+        self._stats = C.OrrTreeStats()
+        self._staging = _Staging(self.ctx)
+
+    @classmethod
+    def try_create(cls, parameters: TreeParameters = TreeParameters(),
+                   softening: Softening | float = Softening(), **kw):
+        try:
+            if device_count() == 0:
+                return None
+            return cls(parameters, softening, **kw)
+        except C.NoDeviceError:
+            return None
+
+    def name(self) -> str:
+        return "hip-tree"
+
+    def softening(self) -> Softening:
+        return self._softening
+
+    def parameters(self) -> TreeParameters:
+        return self._parameters
+
+    def interaction_count(self) -> InteractionCount:
+        return InteractionCount(**vars(self._count))
+
+    def reset_interaction_count(self) -> None:
+        self._count = InteractionCount()
+        self._visits = 0
+
+    def node_visits(self) -> int:
+        return self._visits
+
+    def tree_summary(self) -> dict:
+        s = self._stats
+        return dict(node_count=int(s.node_count), leaf_count=int(s.leaf_count), depth=int(s.depth))
+
+    def timings(self) -> dict:
+        return self.ctx.timings()
+
+    def _args(self, ptrs, n) -> C.OrrTreeArgs:
+        a = C.OrrTreeArgs()
+        a.x, a.y, a.z, a.m, a.ax, a.ay, a.az = ptrs
+        a.count = n
+        a.softening_squared = self._softening.squared()
+        a.gravitational_constant = self.G
+        a.opening_angle = self._parameters.opening_angle
+        a.leaf_capacity = self._parameters.leaf_capacity
+        a.quadrupole = 1 if self._parameters.quadrupole else 0
+        return a
+
+    def evaluate_device(self, x: int, y: int, z: int, m: int, ax: int, ay: int, az: int, n: int):
+        args = self._args((x, y, z, m, ax, ay, az), n)
+        C.check(C.load().orr_tree_eval(self.ctx.handle, ctypes.byref(args), ctypes.byref(self._stats)))
+        self._tally()
+
+    def evaluate(self, positions, masses, accelerations) -> None:
+        n = _check_spans(positions, masses, accelerations)
+        if n == 0:
+            self._stats = C.OrrTreeStats()
+            self._count.evaluations += 1
+            return
+        d = self._staging.reserve(n)
+        for dev, host in zip(d[:4], (*positions, masses)):
+            self.ctx.upload(dev.ptr, np.ascontiguousarray(host, dtype=np.float64))
+        args = self._args(tuple(a.ptr for a in d), n)
+        C.check(C.load().orr_tree_eval(self.ctx.handle, ctypes.byref(args), ctypes.byref(self._stats)))
+        for dev, host in zip(d[4:], accelerations):
+            out = np.empty(n)
+            self.ctx.download(out, dev.ptr)
+            host[:] = out
+        self._tally()
+
+    def _tally(self) -> None:
+        self._count.evaluations += 1
+        self._count.particle_particle += int(self._stats.particle_particle)
+        self._count.particle_cell += int(self._stats.particle_cell)
+        self._visits += int(self._stats.node_visits)
+
+    def nodes(self) -> dict:
+        """Nodes of the last evaluation in DFS order (TreeNode fields)."""
+        M = int(self._stats.node_count)
+        mass, cx, cy, cz, acc = (np.zeros(M) for _ in range(5))
+        nxt, first, count = (np.zeros(M, dtype=np.uint64) for _ in range(3))
+        ptrs = [a.ctypes.data_as(ctypes.c_void_p) for a in (mass, cx, cy, cz, acc, nxt, first, count)]
+        C.check(C.load().orr_tree_nodes(self.ctx.handle, *ptrs))
+        return dict(mass=mass, com=np.stack([cx, cy, cz], axis=1), acc_r2=acc, next=nxt,
+                    first=first, count=count)
+
+
+def morton_order(ctx: Context, x: np.ndarray, y: np.ndarray, z: np.ndarray):
+    """Device Morton ordering of host positions: (codes, index, origin, size)."""
+    n = len(x)
+    dx, dy, dz = (DeviceArray.from_host(ctx, np.ascontiguousarray(a, dtype=np.float64)) for a in (x, y, z))
+    codes = DeviceArray(ctx, n, np.uint64)
+    index = DeviceArray(ctx, n, np.uint32)
+    cube = np.zeros(4)
+    C.check(C.load().orr_morton_order(ctx.handle, dx.ptr, dy.ptr, dz.ptr, n, codes.ptr, index.ptr,
+                                      cube.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    out = codes.to_host(), index.to_host(), cube[:3].copy(), float(cube[3])
+    for a in (dx, dy, dz, codes, index):
+        a.release()
+    return out
+
+
+INTEGRATORS = {"velocity-verlet": 0, "vv": 0, "yoshida4": 1}
+SOLVERS = {"direct": 0, "hip-direct": 0, "barnes-hut": 1, "hip-tree": 1}
+
+
+@dataclass
+class StepConfig:
+    integrator: str = "velocity-verlet"
+    solver: str = "direct"
+    softening: float = 0.0
+    gravitational_constant: float = 1.0
+    opening_angle: float = 0.5
+    leaf_capacity: int = 32
+    quadrupole: bool = False
+
+    def native(self) -> C.OrrStepConfig:
+        c = C.OrrStepConfig()
+        c.integrator = INTEGRATORS[self.integrator]
+        c.solver = SOLVERS[self.solver]
+        c.softening_squared = float(self.softening) * float(self.softening)
+        c.gravitational_constant = self.gravitational_constant
+        c.opening_angle = self.opening_angle
+        c.leaf_capacity = self.leaf_capacity
+        c.quadrupole = 1 if self.quadrupole else 0
+        return c
+
+
+class DeviceSystem:
+    """Particle state resident in HBM across steps (FP64 x, v, a, m)."""
+
+    def __init__(self, ctx: Context, count: int):
+        self.ctx = ctx
+        self.count = int(count)
+        h = ctypes.c_void_p()
+        C.check(C.load().orr_system_create(ctx.handle, self.count, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.check(C.load().orr_system_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _p(a):
+        return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+    def upload(self, x, y, z, vx, vy, vz, m) -> None:
+        arrs = [self._p(a) for a in (x, y, z, vx, vy, vz, m)]
+        ptrs = [None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in arrs]
+        C.check(C.load().orr_system_upload(self.handle, *ptrs))
+
+    def download(self):
+        out = [np.zeros(self.count) for _ in range(9)]
+        C.check(C.load().orr_system_download(self.handle, *(a.ctypes.data_as(ctypes.c_void_p) for a in out)))
+        return dict(x=out[0], y=out[1], z=out[2], vx=out[3], vy=out[4], vz=out[5],
+                    ax=out[6], ay=out[7], az=out[8])
+
+    def refresh(self, config: StepConfig) -> None:
+        c = config.native()
+        C.check(C.load().orr_system_refresh(self.handle, ctypes.byref(c)))
+
+    def step(self, config: StepConfig, timestep: float, steps: int = 1) -> None:
+        c = config.native()
+        C.check(C.load().orr_system_step(self.handle, ctypes.byref(c), float(timestep), int(steps)))
+
+    def diagnostics(self, softening: float, gravitational_constant: float = 1.0) -> dict:
+        out = np.zeros(8)
+        C.check(C.load().orr_system_diagnostics(self.handle, float(softening) * float(softening),
+                                                float(gravitational_constant),
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return dict(kinetic=out[0], potential=out[1], total=out[0] + out[1],
+                    momentum=out[2:5].copy(), angular_momentum=out[5:8].copy())

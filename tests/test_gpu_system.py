@@ -1,0 +1,105 @@
+"""Device-resident stepping (FP64 kick/drift on the GPU, FP32-term field).
+
+Golden vectors held by the reference (web/tests/fixtures):
+* kepler-double.otj -- two bodies, direct, no softening, velocity Verlet,
+  dt = 0.02221441469, 200 steps, frames every 20 (examples/kepler.orrery);
+* agreement.otj / agreement.csv -- 64-particle uniform sphere, direct,
+  eps = 0.05, velocity Verlet dt = 1e-3, 200 steps; E0 and E200 in the CSV.
+The device field uses FP32 pairwise terms, so trajectories are compared to
+what those terms can hold, and conserved quantities in FP64.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import orrery_amd as E
+from tests.golden import otj
+
+pytestmark = pytest.mark.gpu
+
+
+def kepler_state():
+    # initial_conditions/kepler.cpp: periapsis 0.5, speed sqrt(G M (1 + e) / r_p).
+    sep = 1.0 * (1 - 0.5)
+    speed = np.sqrt(1.0 * 2.0 * (1 + 0.5) / sep)
+    x = np.array([-0.5 * sep, 0.5 * sep])
+    vy = np.array([-0.5 * speed, 0.5 * speed])
+    z = np.zeros(2)
+    return x, z.copy(), z.copy(), z.copy(), vy, z.copy(), np.array([1.0, 1.0])
+
+
+def test_kepler_fixture_frames(ctx):
+    t = otj.read("tests/golden/kepler-double.otj")
+    x, y, z, vx, vy, vz, m = kepler_state()
+    assert np.array_equal(t["frames"][0]["positions"][0], x)
+    s = E.DeviceSystem(ctx, 2)
+    s.upload(x, y, z, vx, vy, vz, m)
+    cfg = E.StepConfig(integrator="velocity-verlet", solver="direct", softening=0.0)
+    s.refresh(cfg)
+    for frame in t["frames"][1:]:
+        s.step(cfg, t["timestep"], 20)
+        st = s.download()
+        got = np.stack([st["x"], st["y"], st["z"]])
+        want = np.stack(frame["positions"])
+        # 200 steps of a 2-body orbit from FP32 terms: positions to 2e-5.
+        assert np.max(np.abs(got - want)) < 2e-5, (frame["step"], np.max(np.abs(got - want)))
+    s.close()
+
+
+def test_agreement_fixture_energy(ctx):
+    t = otj.read("tests/golden/agreement.otj")
+    rows = otj.read_csv("tests/golden/agreement.csv")
+    x, y, z = t["frames"][0]["positions"]
+    n = t["n"]
+    zero = np.zeros(n)
+    s = E.DeviceSystem(ctx, n)
+    s.upload(x, y, z, zero, zero, zero, t["masses"])
+    eps = 0.05
+    d0 = s.diagnostics(eps)
+    assert abs(d0["total"] - rows[0]["total_energy"]) <= 1e-15 * abs(rows[0]["total_energy"])
+    cfg = E.StepConfig(integrator="velocity-verlet", solver="direct", softening=eps)
+    s.refresh(cfg)
+    s.step(cfg, t["timestep"], 200)
+    d1 = s.diagnostics(eps)
+    e200 = rows[1]["total_energy"]
+    assert abs(d1["total"] - e200) <= 1e-8 * abs(e200), (d1["total"], e200)
+    assert abs(d1["kinetic"] - rows[1]["kinetic_energy"]) <= 1e-6 * rows[1]["kinetic_energy"]
+    st = s.download()
+    want = np.stack(t["frames"][1]["positions"])
+    got = np.stack([st["x"], st["y"], st["z"]])
+    assert np.max(np.abs(got - want)) < 1e-6
+    s.close()
+
+
+def test_device_diagnostics_match_oracle(ctx):
+    p = oracle.plummer(3000, 9)
+    s = E.DeviceSystem(ctx, p.n)
+    s.upload(*p.arrays())
+    d = s.diagnostics(0.01)
+    r = oracle.diagnostics(p, 1e-4)
+    assert abs(d["kinetic"] - r["kinetic"]) <= 1e-14 * abs(r["kinetic"])
+    assert abs(d["potential"] - r["potential"]) <= 1e-14 * abs(r["potential"])
+    assert np.allclose(d["momentum"], r["momentum"], rtol=0, atol=1e-16)
+    assert np.allclose(d["angular_momentum"], r["angular_momentum"], rtol=1e-12, atol=1e-16)
+    s.close()
+
+
+@pytest.mark.parametrize("integrator", ["velocity-verlet", "yoshida4"])
+def test_steps_match_cpu_reference_path(ctx, integrator):
+    # Same seed, same steps: device state against the oracle's FP64 integrator.
+    p = oracle.plummer(2048, 11)
+    eps, dt, steps = 0.01, 1e-3, 10
+    f = oracle.make_field(integrator="vv" if integrator == "velocity-verlet" else "yoshida4",
+                          solver="direct", eps2=eps * eps)
+    ref = oracle.Particles(*(a.copy() for a in p.arrays()))
+    acc = oracle.refresh(ref, f)
+    oracle.steps(ref, acc, f, dt, steps)
+    s = E.DeviceSystem(ctx, p.n)
+    s.upload(*p.arrays())
+    cfg = E.StepConfig(integrator=integrator, solver="direct", softening=eps)
+    s.refresh(cfg)
+    s.step(cfg, dt, steps)
+    st = s.download()
+    assert np.max(np.abs(st["x"] - ref.x)) < 1e-9
+    assert np.max(np.abs(st["vx"] - ref.vx)) < 1e-7
+    s.close()
