@@ -109,6 +109,21 @@ typedef struct {
 } orr_direct_args;
 ORR_API int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* args);
 
+/* Launch configuration of the direct kernel, chosen per context by the
+ * autotuner (no reference counterpart: the CUDA solver fixes its block size,
+ * cuda_direct_solver.cpp:46-63). targets_per_lane in {1, 2, 4}; refine = 1
+ * applies a Newton step to the hardware reciprocal square root (needed for
+ * <= 1e-6 relative force error); block_target is the number of thread blocks
+ * the source range is split to reach. Zero fields keep the current value. */
+typedef struct {
+    int32_t targets_per_lane;
+    int32_t refine;
+    uint32_t block_target;
+    uint32_t reserved;
+} orr_direct_config;
+ORR_API int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* config);
+ORR_API int orr_ctx_get_direct_config(orr_ctx* ctx, orr_direct_config* config);
+
 /* ---- Barnes-Hut --------------------------------------------------------- */
 
 /* Replaces CudaTreeArguments + launch_cuda_tree (cuda_kernels.hpp:137-229) and

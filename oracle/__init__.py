@@ -80,6 +80,10 @@ def load():
         lib = ctypes.CDLL(path)
         lib.orc_spread_bits.restype = ctypes.c_uint64
         lib.orc_spread_bits.argtypes = [ctypes.c_uint32]
+        lib.orc_morton_interleave.restype = ctypes.c_uint64
+        lib.orc_morton_interleave.argtypes = [ctypes.c_uint32] * 3
+        lib.orc_morton_octant.restype = ctypes.c_uint
+        lib.orc_morton_octant.argtypes = [ctypes.c_uint64, ctypes.c_uint]
         lib.orc_tree_build.restype = ctypes.c_void_p
         lib.orc_tree_node_count.restype = ctypes.c_uint64
         lib.orc_tree_node_count.argtypes = [ctypes.c_void_p]

@@ -881,6 +881,12 @@ int orc_bounding_cube(uint64_t n, const double* x, const double* y, const double
 
 uint64_t orc_spread_bits(uint32_t value) { return spread(value); }
 
+uint64_t orc_morton_interleave(uint32_t x, uint32_t y, uint32_t z) {
+    return (spread(x) << 2) | (spread(y) << 1) | spread(z);
+}
+
+unsigned orc_morton_octant(uint64_t code, unsigned level) { return octant(code, level); }
+
 int orc_morton_codes(uint64_t n, const double* x, const double* y, const double* z,
                      const double origin[3], double size, uint64_t* codes) {
     Cube c{{origin[0], origin[1], origin[2]}, size};

@@ -92,6 +92,9 @@ int orc_bounding_cube(uint64_t n, const double* x, const double* y, const double
                       double origin[3], double* size);
 /* solvers/morton.cpp:106-122 */
 uint64_t orc_spread_bits(uint32_t value);
+/* solvers/morton.cpp:124-126 and :166-178. */
+uint64_t orc_morton_interleave(uint32_t x, uint32_t y, uint32_t z);
+unsigned orc_morton_octant(uint64_t code, unsigned level);
 /* solvers/morton.cpp:124-164: codes in particle order. */
 int orc_morton_codes(uint64_t n, const double* x, const double* y, const double* z,
                      const double origin[3], double size, uint64_t* codes);

@@ -41,6 +41,11 @@ class OrrDirectArgs(ctypes.Structure):
                 ("gravitational_constant", ctypes.c_double), ("spatial_order", ctypes.c_int32)]
 
 
+class OrrDirectConfig(ctypes.Structure):
+    _fields_ = [("targets_per_lane", ctypes.c_int32), ("refine", ctypes.c_int32),
+                ("block_target", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
 class OrrTreeArgs(ctypes.Structure):
     _fields_ = [("x", _vp), ("y", _vp), ("z", _vp), ("m", _vp), ("ax", _vp), ("ay", _vp),
                 ("az", _vp), ("count", ctypes.c_uint64), ("softening_squared", ctypes.c_double),
@@ -80,6 +85,8 @@ SIGNATURES = {
     "orr_download": [_vp, _vp, _vp, ctypes.c_uint64],
     "orr_memset": [_vp, _vp, ctypes.c_int32, ctypes.c_uint64],
     "orr_direct_eval": [_vp, ctypes.POINTER(OrrDirectArgs)],
+    "orr_ctx_set_direct_config": [_vp, ctypes.POINTER(OrrDirectConfig)],
+    "orr_ctx_get_direct_config": [_vp, ctypes.POINTER(OrrDirectConfig)],
     "orr_tree_eval": [_vp, ctypes.POINTER(OrrTreeArgs), ctypes.POINTER(OrrTreeStats)],
     "orr_morton_order": [_vp, _vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _d],
     "orr_tree_nodes": [_vp] + [_vp] * 8,

@@ -258,6 +258,29 @@ int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* a) {
     });
 }
 
+int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* c) {
+    return guarded([&] {
+        require(ctx != nullptr && c != nullptr, "null argument");
+        require(c->targets_per_lane == 0 || c->targets_per_lane == 1 || c->targets_per_lane == 2 ||
+                    c->targets_per_lane == 4,
+                "targets_per_lane must be 1, 2 or 4");
+        require(c->refine == 0 || c->refine == 1 || c->refine == -1, "refine must be 0 or 1 (-1 keeps)");
+        if (c->targets_per_lane != 0) ctx->direct_targets_per_lane = c->targets_per_lane;
+        if (c->refine >= 0) ctx->direct_refine = c->refine;
+        if (c->block_target != 0) ctx->direct_block_target = c->block_target;
+    });
+}
+
+int orr_ctx_get_direct_config(orr_ctx* ctx, orr_direct_config* c) {
+    return guarded([&] {
+        require(ctx != nullptr && c != nullptr, "null argument");
+        c->targets_per_lane = ctx->direct_targets_per_lane;
+        c->refine = ctx->direct_refine;
+        c->block_target = ctx->direct_block_target;
+        c->reserved = 0;
+    });
+}
+
 int orr_tree_eval(orr_ctx* ctx, const orr_tree_args* a, orr_tree_stats* stats) {
     return guarded([&] {
         require(ctx != nullptr && a != nullptr, "null argument");

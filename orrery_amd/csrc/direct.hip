@@ -4,26 +4,31 @@
 // which gives one thread one target and stages j-tiles of positions through
 // shared memory with two barriers per tile. Here:
 //
-//   * Sources are packed once per evaluation into 64-particle tiles of float4
-//     {dx, dy, dz, m}, the offsets taken in FP64 from the tile's FP64 centre
-//     (the midpoint of its bounding box) and only then rounded to FP32. In
-//     Morton order a tile is spatially compact, so an offset carries the
-//     precision of the tile's size rather than of the system's.
+//   * Sources are packed once per evaluation into 64-particle tiles: a float4
+//     {ox, oy, oz, m} per particle, the offset o taken in FP64 from the tile's
+//     FP64 centre (midpoint of its bounding box) and rounded to FP32, plus the
+//     rounding residual o_lo = (x - c) - o as a second float4. In Morton order
+//     a tile is spatially compact, so o carries the precision of the tile's
+//     size rather than of the system's.
 //   * A wave reads a source tile through the scalar unit: the address is
-//     wave-uniform, so each {dx, dy, dz, m} arrives in SGPRs and is broadcast
+//     wave-uniform, so each {ox, oy, oz, m} arrives in SGPRs and is broadcast
 //     to all 64 lanes as an instruction operand. No LDS, no barriers.
 //   * Each lane owns R targets. Per target and source tile the FP64
-//     centre-to-centre shift is rounded once to FP32, so a pairwise
-//     separation is (o_j - (o_i - shift)) with every term small.
+//     centre-to-centre shift is rounded once to FP32; far tiles use the
+//     separation (o_j - (o_i - shift)), 13 VALU operations per interaction.
+//   * Near tiles -- boxes whose gap is at most a tenth of their summed
+//     half-widths, the only place where a separation can be much smaller than
+//     the offsets (about 2% of tile pairs at N = 262,144) -- use a
+//     double-float separation (o_j - t) + (o_lo_j - t_lo), exact to FP32
+//     rounding of the separation itself, at 19 operations per interaction.
+//     Every pairwise term therefore has O(FP32 epsilon) relative error.
 //   * The 64 terms of one source tile are summed plainly in FP32 and the tile
 //     partial is Kahan-added into the running sum (compensation at tile
-//     granularity, three VALU operations per target per tile).
+//     granularity).
 //   * The source range is split over gridDim.y; a fixed-order FP64 reduction
 //     kernel folds the splits, multiplies by G and scatters to particle order.
 //
-// Pairwise arithmetic per interaction: 3 sub, 3 FMA (r^2 + eps^2), 1 rsq,
-// 3 mul, 3 FMA: 13 VALU issues for the reference's 20-flop interaction
-// (docs/performance.md:103-109 counts 20).
+// The reference counts 20 flop per interaction (docs/performance.md:103-109).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +42,7 @@ namespace {
 
 constexpr int kTile = 64;    // sources per tile = one wave of the prepare kernel
 constexpr int kBlock = 256;  // four waves
+constexpr float kNearGap = 0.1f;
 
 __device__ inline double wave_min(double v) {
 #pragma unroll
@@ -50,13 +56,16 @@ __device__ inline double wave_max(double v) {
 }
 
 // One wave per tile: gather (optionally through `order`), centre, pack.
+// centres[t] = {cx, cy, cz, largest half-extent}; extents[t] = {hx, hy, hz, w}.
 __global__ __launch_bounds__(64) void direct_prepare(const double* __restrict__ x,
                                                      const double* __restrict__ y,
                                                      const double* __restrict__ z,
                                                      const double* __restrict__ m,
                                                      const std::uint32_t* __restrict__ order,
                                                      std::uint64_t n, float4* __restrict__ packed,
-                                                     double4* __restrict__ centres) {
+                                                     float4* __restrict__ packed_lo,
+                                                     double4* __restrict__ centres,
+                                                     float4* __restrict__ extents) {
     const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * kTile + threadIdx.x;
     const bool valid = k < n;
     const std::uint64_t src = valid ? (order != nullptr ? order[k] : k) : 0;
@@ -65,21 +74,64 @@ __global__ __launch_bounds__(64) void direct_prepare(const double* __restrict__ 
     const double pz = valid ? z[src] : 0.0;
     const double pm = valid ? m[src] : 0.0;
     const double inf = __builtin_huge_val();
-    const double cx = 0.5 * (wave_min(valid ? px : inf) + wave_max(valid ? px : -inf));
-    const double cy = 0.5 * (wave_min(valid ? py : inf) + wave_max(valid ? py : -inf));
-    const double cz = 0.5 * (wave_min(valid ? pz : inf) + wave_max(valid ? pz : -inf));
-    packed[k] = valid ? make_float4(static_cast<float>(px - cx), static_cast<float>(py - cy),
-                                    static_cast<float>(pz - cz), static_cast<float>(pm))
-                      : make_float4(0.f, 0.f, 0.f, 0.f);
-    if (threadIdx.x == 0) centres[blockIdx.x] = make_double4(cx, cy, cz, 0.0);
+    const double lx = wave_min(valid ? px : inf), hx = wave_max(valid ? px : -inf);
+    const double ly = wave_min(valid ? py : inf), hy = wave_max(valid ? py : -inf);
+    const double lz = wave_min(valid ? pz : inf), hz = wave_max(valid ? pz : -inf);
+    const double cx = 0.5 * (lx + hx), cy = 0.5 * (ly + hy), cz = 0.5 * (lz + hz);
+    const double w = 0.5 * fmax(hx - lx, fmax(hy - ly, hz - lz));
+    if (valid) {
+        const double dx = px - cx, dy = py - cy, dz = pz - cz;
+        const float ox = static_cast<float>(dx), oy = static_cast<float>(dy), oz = static_cast<float>(dz);
+        packed[k] = make_float4(ox, oy, oz, static_cast<float>(pm));
+        packed_lo[k] = make_float4(static_cast<float>(dx - ox), static_cast<float>(dy - oy),
+                                   static_cast<float>(dz - oz), 0.f);
+    } else {
+        packed[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        packed_lo[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (threadIdx.x == 0) {
+        centres[blockIdx.x] = make_double4(cx, cy, cz, w);
+        extents[blockIdx.x] = make_float4(static_cast<float>(0.5 * (hx - lx)), static_cast<float>(0.5 * (hy - ly)),
+                                          static_cast<float>(0.5 * (hz - lz)), static_cast<float>(w));
+    }
 }
 
-// kSafe masks r^2 == 0 (zero softening): the self term and coincident pads
-// then contribute m * finite * 0 = 0 instead of NaN, as the reference's CUDA
-// kernel masks self and tail terms (cuda_direct_kernel.cu:134-159).
-template<int R, bool kSafe>
+// One pairwise term accumulated into (px, py, pz). kSafe masks r^2 == 0 (zero
+// softening): the self term and coincident pads then contribute m * finite * 0
+// = 0 instead of NaN, as the reference's CUDA kernel masks self and tail terms
+// (cuda_direct_kernel.cu:134-159). kRefine applies one Newton-Raphson step to
+// the hardware reciprocal square root y0 before cubing: with e = 1 - x y0^2
+// (one rounding, via FMA), y^3 = y0^3 (1 + 3e/2) to second order in e.
+template<bool kSafe, bool kRefine>
+__device__ __forceinline__ void term(float dx, float dy, float dz, float m, float eps2, float& px,
+                                     float& py, float& pz) {
+    float r2 = __builtin_fmaf(dx, dx, eps2);
+    r2 = __builtin_fmaf(dy, dy, r2);
+    r2 = __builtin_fmaf(dz, dz, r2);
+    if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    float f = (m * inv) * (inv * inv);
+    if constexpr (kRefine) {
+        const float e = __builtin_fmaf(-(r2 * inv), inv, 1.f);
+        f = f * __builtin_fmaf(1.5f, e, 1.f);
+    }
+    px = __builtin_fmaf(f, dx, px);
+    py = __builtin_fmaf(f, dy, py);
+    pz = __builtin_fmaf(f, dz, pz);
+}
+
+__device__ __forceinline__ void kahan(float p, float& s, float& c) {
+    const float y = p - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+template<int R, bool kSafe, bool kRefine>
 __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict__ packed,
+                                                       const float4* __restrict__ packed_lo,
                                                        const double4* __restrict__ centres,
+                                                       const float4* __restrict__ extents,
                                                        std::uint32_t npad, std::uint32_t tiles,
                                                        std::uint32_t tiles_per_split, float eps2,
                                                        float* __restrict__ partial) {
@@ -88,70 +140,99 @@ __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict_
     const std::uint32_t j1 = min(tiles, j0 + tiles_per_split);
     const std::uint32_t base = blockIdx.x * (kBlock * R) + threadIdx.x;
 
-    float ox[R], oy[R], oz[R];
+    float ox[R], oy[R], oz[R], lx[R], ly[R], lz[R];
     double cx[R], cy[R], cz[R];
+    float4 ce[R];
     float sx[R], sy[R], sz[R], kx[R], ky[R], kz[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const std::uint32_t k = min(base + r * kBlock, npad - 1);
         const float4 o = packed[k];
+        const float4 l = packed_lo[k];
         const double4 c = centres[k / kTile];
         ox[r] = o.x;
         oy[r] = o.y;
         oz[r] = o.z;
+        lx[r] = l.x;
+        ly[r] = l.y;
+        lz[r] = l.z;
         cx[r] = c.x;
         cy[r] = c.y;
         cz[r] = c.z;
+        ce[r] = extents[k / kTile];
         sx[r] = sy[r] = sz[r] = 0.f;
         kx[r] = ky[r] = kz[r] = 0.f;
     }
 
     for (std::uint32_t J = j0; J < j1; ++J) {
         const double4 C = centres[J];
-        float ux[R], uy[R], uz[R], px[R], py[R], pz[R];
+        const float4 E = extents[J];
+        const float4* __restrict__ src = packed + static_cast<std::size_t>(J) * kTile;
+        // Near: the gap between the two tiles' boxes is at most kNearGap of their
+        // summed half-widths. Outside it every separation exceeds that gap, so the
+        // FP32 offsets round it to O(epsilon) relative error. All lanes of a wave
+        // share one target tile per r, so the branch is uniform.
+        bool near = false;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            // Target offset re-expressed relative to the source tile's centre.
-            ux[r] = ox[r] - static_cast<float>(C.x - cx[r]);
-            uy[r] = oy[r] - static_cast<float>(C.y - cy[r]);
-            uz[r] = oz[r] - static_cast<float>(C.z - cz[r]);
-            px[r] = py[r] = pz[r] = 0.f;
+            const float gx = fmaxf(0.f, fabsf(static_cast<float>(C.x - cx[r])) - ce[r].x - E.x);
+            const float gy = fmaxf(0.f, fabsf(static_cast<float>(C.y - cy[r])) - ce[r].y - E.y);
+            const float gz = fmaxf(0.f, fabsf(static_cast<float>(C.z - cz[r])) - ce[r].z - E.z);
+            const float reach = kNearGap * (ce[r].w + E.w);
+            near = near || (gx * gx + gy * gy + gz * gz <= reach * reach);
         }
-        const float4* __restrict__ src = packed + static_cast<std::size_t>(J) * kTile;
-#pragma unroll 16
-        for (int j = 0; j < kTile; ++j) {
-            const float4 q = src[j];
+        float px[R], py[R], pz[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) px[r] = py[r] = pz[r] = 0.f;
+        if (!near) {
+            float ux[R], uy[R], uz[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const float dx = q.x - ux[r];
-                const float dy = q.y - uy[r];
-                const float dz = q.z - uz[r];
-                float r2 = __builtin_fmaf(dx, dx, eps2);
-                r2 = __builtin_fmaf(dy, dy, r2);
-                r2 = __builtin_fmaf(dz, dz, r2);
-                if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
-                const float inv = __builtin_amdgcn_rsqf(r2);
-                const float f = (q.w * inv) * (inv * inv);
-                px[r] = __builtin_fmaf(f, dx, px[r]);
-                py[r] = __builtin_fmaf(f, dy, py[r]);
-                pz[r] = __builtin_fmaf(f, dz, pz[r]);
+                // Target offset re-expressed relative to the source tile's centre.
+                ux[r] = ox[r] - static_cast<float>(C.x - cx[r]);
+                uy[r] = oy[r] - static_cast<float>(C.y - cy[r]);
+                uz[r] = oz[r] - static_cast<float>(C.z - cz[r]);
+            }
+#pragma unroll 16
+            for (int j = 0; j < kTile; ++j) {
+                const float4 q = src[j];
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    term<kSafe, kRefine>(q.x - ux[r], q.y - uy[r], q.z - uz[r], q.w, eps2, px[r],
+                                         py[r], pz[r]);
+            }
+        } else {
+            // Double-float target position relative to the source tile's centre.
+            float th[R][3], tl[R][3];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double t[3] = {(static_cast<double>(ox[r]) + lx[r]) - (C.x - cx[r]),
+                                     (static_cast<double>(oy[r]) + ly[r]) - (C.y - cy[r]),
+                                     (static_cast<double>(oz[r]) + lz[r]) - (C.z - cz[r])};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    th[r][c] = static_cast<float>(t[c]);
+                    tl[r][c] = static_cast<float>(t[c] - th[r][c]);
+                }
+            }
+            const float4* __restrict__ src_lo = packed_lo + static_cast<std::size_t>(J) * kTile;
+#pragma unroll 8
+            for (int j = 0; j < kTile; ++j) {
+                const float4 q = src[j];
+                const float4 ql = src_lo[j];
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    term<kSafe, kRefine>((q.x - th[r][0]) + (ql.x - tl[r][0]),
+                                         (q.y - th[r][1]) + (ql.y - tl[r][1]),
+                                         (q.z - th[r][2]) + (ql.z - tl[r][2]), q.w, eps2, px[r],
+                                         py[r], pz[r]);
             }
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            // Kahan: sum += partial, compensation carries the lost low part.
-            float y = px[r] - kx[r];
-            float t = sx[r] + y;
-            kx[r] = (t - sx[r]) - y;
-            sx[r] = t;
-            y = py[r] - ky[r];
-            t = sy[r] + y;
-            ky[r] = (t - sy[r]) - y;
-            sy[r] = t;
-            y = pz[r] - kz[r];
-            t = sz[r] + y;
-            kz[r] = (t - sz[r]) - y;
-            sz[r] = t;
+            kahan(px[r], sx[r], kx[r]);
+            kahan(py[r], sy[r], ky[r]);
+            kahan(pz[r], sz[r], kz[r]);
         }
     }
 
@@ -193,7 +274,25 @@ __global__ __launch_bounds__(256) void direct_reduce(const float* __restrict__ p
     az[dst] = G * a2;
 }
 
-constexpr int kTargetsPerLane = 2;
+template<int R, bool kSafe, bool kRefine>
+void launch_tiles(orr_ctx* ctx, std::uint32_t blocks_i, std::uint32_t splits, const float4* packed,
+                  const float4* packed_lo, const double4* centres, const float4* extents,
+                  std::uint32_t npad,
+                  std::uint32_t tiles, std::uint32_t per_split, float eps2, float* partials) {
+    hipLaunchKernelGGL((direct_tiles<R, kSafe, kRefine>), dim3(blocks_i, splits), dim3(kBlock), 0,
+                       ctx->stream, packed, packed_lo, centres, extents, npad, tiles, per_split,
+                       eps2, partials);
+}
+
+using TilesLauncher = void (*)(orr_ctx*, std::uint32_t, std::uint32_t, const float4*,
+                               const float4*, const double4*, const float4*, std::uint32_t,
+                               std::uint32_t,
+                               std::uint32_t, float, float*);
+
+template<int R> TilesLauncher pick(bool safe, bool refine) {
+    if (safe) return refine ? launch_tiles<R, true, true> : launch_tiles<R, true, false>;
+    return refine ? launch_tiles<R, false, true> : launch_tiles<R, false, false>;
+}
 
 }  // namespace
 
@@ -203,38 +302,40 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
     if (n > (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "direct: more than 2^31 particles");
     const std::uint32_t tiles = static_cast<std::uint32_t>((n + kTile - 1) / kTile);
     const std::uint32_t npad = tiles * kTile;
-    constexpr int R = kTargetsPerLane;
+    const int R = ctx->direct_targets_per_lane;
     const std::uint32_t blocks_i = (npad + kBlock * R - 1) / (kBlock * R);
     // Enough blocks to fill 256 CUs several times over; each split keeps >= 1 tile.
-    const std::uint32_t want = 4096;
+    const std::uint32_t want = ctx->direct_block_target;
     std::uint32_t splits = std::max<std::uint32_t>(1, (want + blocks_i - 1) / blocks_i);
     splits = std::min(splits, tiles);
     const std::uint32_t per_split = (tiles + splits - 1) / splits;
     splits = (tiles + per_split - 1) / per_split;
 
-    ctx->direct_packed.reserve(sizeof(float4) * npad);
-    ctx->direct_centres.reserve(sizeof(double4) * tiles);
+    ctx->direct_packed.reserve(2 * sizeof(float4) * npad);
+    ctx->direct_centres.reserve((sizeof(double4) + sizeof(float4)) * tiles);
     ctx->direct_partials.reserve(sizeof(float) * 6 * static_cast<std::size_t>(npad) * splits);
     float4* packed = ctx->direct_packed.as<float4>();
+    float4* packed_lo = packed + npad;
     double4* centres = ctx->direct_centres.as<double4>();
+    float4* extents = reinterpret_cast<float4*>(centres + tiles);
     float* partials = ctx->direct_partials.as<float>();
 
     {
         PhaseTimer t(ctx, kDirectPrepare);
         hipLaunchKernelGGL(direct_prepare, dim3(tiles), dim3(kTile), 0, ctx->stream, a.x, a.y, a.z,
-                           a.m, order, n, packed, centres);
+                           a.m, order, n, packed, packed_lo, centres, extents);
         ORR_LAUNCHED("direct_prepare");
     }
     const float eps2 = static_cast<float>(a.softening_squared);
     {
         PhaseTimer t(ctx, kDirectKernel);
-        if (a.softening_squared > 0) {
-            hipLaunchKernelGGL((direct_tiles<R, false>), dim3(blocks_i, splits), dim3(kBlock), 0,
-                               ctx->stream, packed, centres, npad, tiles, per_split, eps2, partials);
-        } else {
-            hipLaunchKernelGGL((direct_tiles<R, true>), dim3(blocks_i, splits), dim3(kBlock), 0,
-                               ctx->stream, packed, centres, npad, tiles, per_split, eps2, partials);
-        }
+        const bool safe = !(a.softening_squared > 0);
+        const bool refine = ctx->direct_refine != 0;
+        TilesLauncher launch = R == 1 ? pick<1>(safe, refine)
+                             : R == 4 ? pick<4>(safe, refine)
+                                      : pick<2>(safe, refine);
+        launch(ctx, blocks_i, splits, packed, packed_lo, centres, extents, npad, tiles, per_split,
+               eps2, partials);
         ORR_LAUNCHED("direct_tiles");
     }
     {

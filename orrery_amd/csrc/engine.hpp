@@ -72,6 +72,11 @@ struct orr_ctx {
         order_bbox, order_flags;
     orr::Buffer tree_nodes, tree_aux, tree_work, tree_counts, tree_sorted, tree_leafref;
 
+    // Direct-kernel configuration (orr_ctx_set_direct_config).
+    int direct_targets_per_lane = 1;
+    int direct_refine = 0;
+    std::uint32_t direct_block_target = 8192;
+
     // Host mirror of the last tree summary.
     orr_tree_stats last_tree{};
 

@@ -70,23 +70,44 @@ __global__ __launch_bounds__(kBlock) void bbox_partial(const double* __restrict_
     if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// Single thread: fold block extrema, then the cube arithmetic of morton.cpp:87-103.
+// One block folds the block extrema (min and max are exact, so the order does
+// not matter), then one thread does the cube arithmetic of morton.cpp:87-103.
 // cube = {origin x, y, z, size}.
-__global__ void bbox_final(const double* __restrict__ partial, std::uint32_t blocks,
-                           double* __restrict__ cube) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double lo[3], hi[3];
-    for (int c = 0; c < 3; ++c) {
-        lo[c] = partial[c];
-        hi[c] = partial[3 + c];
-    }
-    for (std::uint32_t b = 1; b < blocks; ++b) {
+__global__ __launch_bounds__(kBlock) void bbox_final(const double* __restrict__ partial,
+                                                     std::uint32_t blocks,
+                                                     double* __restrict__ cube) {
+    __shared__ double s[6][kBlock];
+    const double inf = __builtin_huge_val();
+    double l[3] = {inf, inf, inf}, h[3] = {-inf, -inf, -inf};
+    for (std::uint32_t b = threadIdx.x; b < blocks; b += kBlock) {
         for (int c = 0; c < 3; ++c) {
             const double a = partial[b * 6 + c];
-            const double h = partial[b * 6 + 3 + c];
-            lo[c] = a < lo[c] ? a : lo[c];
-            hi[c] = h > hi[c] ? h : hi[c];
+            const double e = partial[b * 6 + 3 + c];
+            l[c] = a < l[c] ? a : l[c];
+            h[c] = e > h[c] ? e : h[c];
         }
+    }
+    for (int c = 0; c < 3; ++c) {
+        s[c][threadIdx.x] = l[c];
+        s[3 + c][threadIdx.x] = h[c];
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            for (int c = 0; c < 3; ++c) {
+                const double a = s[c][threadIdx.x + w];
+                if (a < s[c][threadIdx.x]) s[c][threadIdx.x] = a;
+                const double e = s[3 + c][threadIdx.x + w];
+                if (e > s[3 + c][threadIdx.x]) s[3 + c][threadIdx.x] = e;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = s[c][0];
+        hi[c] = s[3 + c][0];
     }
     const double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
     // std::max({ex, ey, ez}) keeps the first of equals; the value is what matters.
@@ -252,7 +273,8 @@ void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double
         hipLaunchKernelGGL(bbox_partial, dim3(bbox_blocks), dim3(kBlock), 0, ctx->stream, x, y, z, n,
                            partial);
         ORR_LAUNCHED("bbox_partial");
-        hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, ctx->stream, partial, bbox_blocks, cube);
+        hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBlock), 0, ctx->stream, partial, bbox_blocks,
+                           cube);
         ORR_LAUNCHED("bbox_final");
     }
     std::uint64_t* keys[2] = {ctx->order_codes.as<std::uint64_t>(), ctx->order_codes_alt.as<std::uint64_t>()};

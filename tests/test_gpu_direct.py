@@ -8,9 +8,11 @@ particle |a - a_ref| / |a_ref|, worst and root mean square
 (cuda_direct_solver_test.cpp:86-111).
 
 Bounds: the reference's FP32 device bounds are worst 2e-4 / RMS 2e-5
-(cuda_direct_solver_test.cpp:122-128); this engine is held to RMS <= 1e-6 and
-worst <= 2e-5 relative on Plummer inputs (FP32 pairwise terms on tile-relative
-coordinates, compensated accumulation).
+(cuda_direct_solver_test.cpp:122-128). With Morton-ordered tiles (the default)
+this engine is held to worst <= 1e-6 and RMS <= 1e-6 relative: FP32 pairwise
+terms on tile-relative coordinates, double-float separations between near
+tiles, tile-level Kahan accumulation. Index-ordered tiles (spatial_order off)
+give up the compact tiles and are held to the reference's FP32 bounds.
 """
 import numpy as np
 import pytest
@@ -21,7 +23,7 @@ import orrery_amd as E
 pytestmark = pytest.mark.gpu
 
 RMS_BOUND = 1e-6
-WORST_BOUND = 2e-5
+WORST_BOUND = 1e-6
 
 
 def evaluate(ctx, pos, m, eps=0.0, spatial=True):
@@ -117,12 +119,26 @@ def test_plummer_4096_against_fp64_reference(ctx, spatial):
     rx, ry, rz = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps)
     worst, rms = oracle.error_summary(oracle.relative_errors(ax, ay, az, rx, ry, rz))
     print(f"spatial={spatial} worst={worst:.3e} rms={rms:.3e}")
+    if spatial:
+        assert rms <= RMS_BOUND and worst <= WORST_BOUND, (rms, worst)
+    else:
+        assert rms <= 2e-5 and worst <= 2e-4, (rms, worst)
+
+
+def test_config_plummer_65536_every_particle(ctx):
+    # The N = 65,536 direct configuration: a = 1, r < 10 a, eps = 0.005, seed 1;
+    # the error of every particle against the FP64 compensated sum.
+    n, eps = 65536, 0.005
+    p = oracle.plummer(n, 1)
+    (ax, ay, az), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
+    rx, ry, rz = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps)
+    worst, rms = oracle.error_summary(oracle.relative_errors(ax, ay, az, rx, ry, rz))
+    print(f"N=65536 all particles worst={worst:.3e} rms={rms:.3e}")
     assert rms <= RMS_BOUND and worst <= WORST_BOUND, (rms, worst)
 
 
-@pytest.mark.parametrize("seed", [1, 2, 3])
-def test_config_plummer_65536_all_targets(ctx, seed):
-    # The N = 65,536 direct configuration: a = 1, r < 10 a, eps = 0.005, seeds 1-3.
+@pytest.mark.parametrize("seed", [2, 3])
+def test_config_plummer_65536_sampled(ctx, seed):
     n, eps = 65536, 0.005
     p = oracle.plummer(n, seed)
     (ax, ay, az), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
