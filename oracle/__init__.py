@@ -279,6 +279,16 @@ def barnes_hut(x, y, z, m, eps2, theta, leaf_capacity, G=1.0, quadrupole=False, 
     return ax, ay, az, (int(counts[0]), int(counts[1]))
 
 
+def barnes_hut_fp32_counts(x, y, z, m, theta, leaf_capacity, threads=0):
+    """(pp, pc) of the device walk's FP32 acceptance policy on the reference tree."""
+    counts = np.zeros(2, dtype=np.uint64)
+    rc = load().orc_barnes_hut_fp32_counts(_u64(len(m)), _ptr(x), _ptr(y), _ptr(z), _ptr(m),
+                                           ctypes.c_double(theta), _u64(leaf_capacity),
+                                           ctypes.c_uint(threads), _ptr(counts, _u64p))
+    assert rc == 0
+    return int(counts[0]), int(counts[1])
+
+
 def make_field(integrator="vv", solver="direct", eps2=0.0, G=1.0, theta=0.5, leaf_capacity=32,
                threads=0, kernel=1):
     f = OrcField()

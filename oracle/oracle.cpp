@@ -957,6 +957,68 @@ int orc_barnes_hut(uint64_t n, const double* x, const double* y, const double* z
     return 0;
 }
 
+int orc_barnes_hut_fp32_counts(uint64_t n, const double* x, const double* y, const double* z,
+                               const double* m, double theta, uint64_t cap, unsigned threads,
+                               uint64_t counts[2]) {
+    counts[0] = counts[1] = 0;
+    if (n == 0) return 0;
+    Cube cube;
+    const std::vector<Key> keys = morton_order(n, x, y, z, cube);
+    std::vector<double> sx(n), sy(n), sz(n), sm(n);
+    std::vector<std::uint64_t> codes(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        const std::size_t src = keys[i].index;
+        sx[i] = x[src];
+        sy[i] = y[src];
+        sz[i] = z[src];
+        sm[i] = m[src];
+        codes[i] = keys[i].code;
+    }
+    const double origin[3] = {cube.origin.x, cube.origin.y, cube.origin.z};
+    orc_tree* tree = orc_tree_build(n, sx.data(), sy.data(), sz.data(), sm.data(), codes.data(),
+                                    origin, cube.size, theta, std::max<uint64_t>(cap, 1), 0);
+    const V3<double> c0 = cube.centre();
+    const std::size_t M = tree->nodes.size();
+    std::vector<float> nx(M), ny(M), nz(M), acc(M);
+    for (std::size_t k = 0; k < M; ++k) {
+        nx[k] = static_cast<float>(tree->nodes[k].com.x - c0.x);
+        ny[k] = static_cast<float>(tree->nodes[k].com.y - c0.y);
+        nz[k] = static_cast<float>(tree->nodes[k].com.z - c0.z);
+        acc[k] = static_cast<float>(tree->nodes[k].acc_r2);
+    }
+    std::atomic<std::uint64_t> pp{0}, pc{0};
+    Executor ex(resolve_threads(threads));
+    ex.run(n, [&](std::size_t b, std::size_t e) {
+        std::uint64_t lpp = 0, lpc = 0;
+        for (std::size_t i = b; i < e; ++i) {
+            const float xi = static_cast<float>(sx[i] - c0.x);
+            const float yi = static_cast<float>(sy[i] - c0.y);
+            const float zi = static_cast<float>(sz[i] - c0.z);
+            std::size_t node = 0;
+            while (node < M) {
+                const Node& c = tree->nodes[node];
+                const float dx = nx[node] - xi, dy = ny[node] - yi, dz = nz[node] - zi;
+                const float r2 = std::fmaf(dz, dz, std::fmaf(dy, dy, dx * dx));
+                if (r2 >= acc[node]) {
+                    ++lpc;
+                    node = c.next;
+                } else if (c.count > 0) {
+                    lpp += c.count - ((i >= c.first && i < c.first + c.count) ? 1 : 0);
+                    node = c.next;
+                } else {
+                    ++node;
+                }
+            }
+        }
+        pp.fetch_add(lpp);
+        pc.fetch_add(lpc);
+    });
+    orc_tree_free(tree);
+    counts[0] = pp.load();
+    counts[1] = pc.load();
+    return 0;
+}
+
 int orc_refresh(uint64_t n, const double* x, const double* y, const double* z, const double* m,
                 const orc_field* f, double* ax, double* ay, double* az) {
     if (f == nullptr) return 1;

@@ -129,6 +129,16 @@ int orc_barnes_hut(uint64_t n, const double* x, const double* y, const double* z
                    uint64_t leaf_capacity, int quadrupole, int kernel, unsigned threads,
                    double* ax, double* ay, double* az, uint64_t counts[2]);
 
+/* Interaction counts of the device walk's precision policy, for exact count
+ * parity: the reference tree (FP64 moments and radii), then per target the
+ * FP32 test r2 >= (float)acc_r2 with r2 = fma(dz, dz, fma(dy, dy, dx*dx)) on
+ * FP32 positions relative to the root centre (origin + size/2). The decisions
+ * follow tree_walk.cpp:42-81; only the arithmetic of the test is FP32.
+ * counts[0] = particle_particle, counts[1] = particle_cell. */
+int orc_barnes_hut_fp32_counts(uint64_t n, const double* x, const double* y, const double* z,
+                               const double* m, double opening_angle, uint64_t leaf_capacity,
+                               unsigned threads, uint64_t counts[2]);
+
 /* ---- integrators and diagnostics ---------------------------------------- */
 
 /* integrators/velocity_verlet.cpp:20-72 (integrator 0) and yoshida4.cpp:24-59

@@ -77,8 +77,11 @@ struct orr_ctx {
     int direct_refine = 0;
     std::uint32_t direct_block_target = 8192;
 
-    // Host mirror of the last tree summary.
+    // Host mirror of the last tree summary and the shape of its device layout.
     orr_tree_stats last_tree{};
+    std::uint64_t tree_last_n = 0;
+    std::size_t tree_last_bound = 0;
+    double tree_last_theta = 0.5;
 
     // Phase timing with hipEvents recorded on `stream`.
     bool timing = false;
@@ -125,6 +128,12 @@ void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double
                     std::uint64_t n);
 // Copies the cube {origin, size} computed by the last morton_enqueue (syncs).
 void morton_cube(orr_ctx* ctx, double cube[4]);
+
+// scan.hip: exclusive prefix sum of n uint32 (in == out allowed). scratch holds
+// scan_scratch_words(n) words; *total_out (device, may be null) gets the sum.
+std::size_t scan_scratch_words(std::uint64_t n);
+void exclusive_scan_u32(orr_ctx* ctx, const std::uint32_t* in, std::uint32_t* out, std::uint64_t n,
+                        std::uint32_t* scratch, std::uint32_t* total_out);
 
 // tree.hip: order, build, moments and walk; fills stats (syncs).
 void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats);

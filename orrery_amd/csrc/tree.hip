@@ -1,17 +1,670 @@
-// Barnes-Hut on gfx950 (in progress).
+// Barnes-Hut on gfx950: octree build from sorted Morton keys, FP64 monopole
+// moments, and a wave-coherent theta-criterion walk.
+//
+// The tree is the reference's (src/solvers/octree.cpp:196-468): cells split in
+// octant order, empty cells skipped, a leaf when a cell holds <= leaf_capacity
+// particles or sits at level 21, nodes in depth-first order with `next` the
+// index just past the subtree. The reference builds it by recursion; here every
+// node is derived independently from the sorted keys:
+//
+//   d(k)   the first level at which key k differs from key k-1 (d(0) = 0, 22
+//          for a duplicate key): a cell of level l starts at particle k exactly
+//          when d(k) <= l.
+//   chain  the cells that start at k form a chain of levels d(k), d(k)+1, ...
+//          Its first cell is a node iff its parent (level d(k)-1) holds more
+//          than leaf_capacity particles (found by scanning +-leaf_capacity keys
+//          around k); every deeper cell of the chain is a node while the one
+//          above it holds more than leaf_capacity, i.e. down to level
+//          min(21, max(d(k), min d over the next leaf_capacity keys)), a leaf.
+//   DFS    depth-first order sorts nodes by (first particle, level), so an
+//          exclusive scan of chain lengths numbers the nodes, and the node
+//          after a subtree is the first node starting at the cell's end:
+//          next = first_node[end].
+//
+// Moments follow octree.cpp:91-185 operation for operation in FP64 (leaf: sums
+// over particles in index order; internal: over children in DFS order), with
+// contraction disabled, so masses, centres of mass and acceptance radii
+// (octree.cpp:374-387) are the reference's bit for bit.
+//
+// The walk (tree_walk.cpp:20-83 per target) runs one wave over 64 consecutive
+// sorted targets. The wave visits node min(next node of each lane) -- each lane
+// still follows exactly its own walk, as the reference's coherent CUDA walk
+// does (ADR-0029) -- with the node read through the scalar unit. Accepted cells
+// use FP32 positions relative to the root centre; leaf particles are stored as
+// FP32 offsets from their leaf's geometric centre and a lane interacting with a
+// leaf re-expresses its own offset against that centre in FP64, so close pairs
+// keep the precision of the leaf size. Per-lane partial sums are Kahan-folded
+// after each leaf.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
 
 #include "engine.hpp"
 
-namespace orr {
+#pragma clang fp contract(off)
 
-void tree_evaluate(orr_ctx*, const orr_tree_args&, orr_tree_stats*) {
-    fail(ORR_ERR_RUNTIME, "tree: not built yet");
+namespace orr {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr unsigned kLevels = 21;
+constexpr std::uint8_t kNoBoundary = 22;
+
+struct WalkNode {
+    float4 a;  // centre of mass relative to the root centre (FP32), mass
+    uint4 b;   // acceptance radius^2 (FP32 bits), next, first particle, particle count
+};
+
+// Device counters, uint32: [0] nodes, [1] leaves, [2] depth.
+// Device statistics, uint64: [0] particle-particle, [1] particle-cell, [2] lane visits,
+// [3] wave steps.
+
+__device__ inline std::uint32_t grid_stride_start() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline std::uint32_t grid_stride_step() { return gridDim.x * blockDim.x; }
+
+__global__ void clear_counters(std::uint32_t* __restrict__ c32, unsigned long long* __restrict__ c64) {
+    if (threadIdx.x < 4) c32[threadIdx.x] = 0;
+    if (threadIdx.x < 4) c64[threadIdx.x] = 0;
 }
 
-void tree_copy_nodes(orr_ctx*, double*, double*, double*, double*, double*, std::uint64_t*,
-                     std::uint64_t*, std::uint64_t*) {
-    fail(ORR_ERR_RUNTIME, "tree: not built yet");
+// d(k) from adjacent sorted codes.
+__global__ __launch_bounds__(kBlock) void tree_levels(const std::uint64_t* __restrict__ codes,
+                                                      std::uint32_t n, std::uint8_t* __restrict__ d) {
+    const std::uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    if (k == 0) {
+        d[0] = 0;
+        return;
+    }
+    const std::uint64_t x = codes[k] ^ codes[k - 1];
+    if (x == 0) {
+        d[k] = kNoBoundary;
+        return;
+    }
+    const unsigned h = 63u - static_cast<unsigned>(__clzll(x));  // highest differing bit
+    d[k] = static_cast<std::uint8_t>(kLevels - h / 3u);
+}
+
+// Chain length of the nodes that start at particle k; leaf level in `leaf_level`.
+__global__ __launch_bounds__(kBlock) void tree_chain(const std::uint8_t* __restrict__ d,
+                                                     std::uint32_t n, std::uint32_t cap,
+                                                     std::uint32_t* __restrict__ chain,
+                                                     std::uint8_t* __restrict__ leaf_level) {
+    const std::uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const unsigned dk = d[k];
+    std::uint32_t len = 0;
+    unsigned lleaf = dk;
+    if (dk <= kLevels) {
+        bool head = true;
+        if (k > 0) {
+            // Parent cell of level dk-1 spans [p, q) around k; it is a leaf (so
+            // no node starts at k) iff q - p <= cap.
+            const unsigned P = dk - 1;
+            const std::uint32_t lo = k + 1 > cap ? k + 1 - cap : 0;
+            std::int64_t p = -1;
+            for (std::int64_t j = static_cast<std::int64_t>(k) - 1; j >= static_cast<std::int64_t>(lo); --j) {
+                if (d[j] <= P) {
+                    p = j;
+                    break;
+                }
+            }
+            if (p >= 0) {
+                const std::uint64_t limit = static_cast<std::uint64_t>(p) + cap;  // q must be <= limit
+                std::uint64_t q = n;
+                const std::uint64_t stop = limit < n - 1 ? limit : n - 1;
+                for (std::uint64_t j = k + 1; j <= stop; ++j) {
+                    if (d[j] <= P) {
+                        q = j;
+                        break;
+                    }
+                }
+                head = !(q <= limit);
+            }
+        }
+        if (head) {
+            if (n - k <= cap) {
+                lleaf = dk;
+            } else {
+                unsigned w = kNoBoundary;
+                for (std::uint32_t j = k + 1; j <= k + cap; ++j) w = min(w, static_cast<unsigned>(d[j]));
+                lleaf = min(kLevels, max(dk, w));
+            }
+            len = lleaf - dk + 1;
+        }
+    }
+    chain[k] = len;
+    leaf_level[k] = static_cast<std::uint8_t>(lleaf);
+}
+
+// First index j >= lo whose code prefix of `level` exceeds that of `key`.
+__device__ std::uint32_t prefix_end(const std::uint64_t* __restrict__ codes, std::uint32_t n,
+                                    std::uint32_t lo, std::uint64_t key, unsigned level) {
+    if (level == 0) return n;
+    const unsigned s = 3u * (kLevels - level);
+    const std::uint64_t want = key >> s;
+    if (lo >= n || (codes[lo] >> s) != want) return lo;
+    // Gallop, then bisect: codes[lo] shares the prefix.
+    std::uint32_t good = lo;
+    std::uint32_t step = 1;
+    std::uint32_t bad = n;
+    while (true) {
+        const std::uint64_t probe = static_cast<std::uint64_t>(good) + step;
+        if (probe >= n) break;
+        if ((codes[probe] >> s) != want) {
+            bad = static_cast<std::uint32_t>(probe);
+            break;
+        }
+        good = static_cast<std::uint32_t>(probe);
+        step <<= 1;
+    }
+    while (bad - good > 1) {
+        const std::uint32_t mid = good + (bad - good) / 2;
+        if ((codes[mid] >> s) != want) bad = mid; else good = mid;
+    }
+    return bad;
+}
+
+// Geometric centre of every node of the chain (octree.cpp:41-47 from the root
+// centre of morton.hpp:93-95), its range, level and escape index.
+__global__ __launch_bounds__(kBlock) void tree_nodes(
+    const std::uint64_t* __restrict__ codes, const std::uint8_t* __restrict__ d,
+    const std::uint8_t* __restrict__ leaf_level, const std::uint32_t* __restrict__ chain,
+    const std::uint32_t* __restrict__ first_node, std::uint32_t n, std::uint32_t cap,
+    const double* __restrict__ cube, std::uint32_t* __restrict__ counters,
+    std::uint32_t* __restrict__ node_begin, std::uint32_t* __restrict__ node_count,
+    std::uint32_t* __restrict__ node_next, std::uint8_t* __restrict__ node_level,
+    double4* __restrict__ node_geo, std::uint32_t* __restrict__ leaf_of) {
+    const std::uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n || chain[k] == 0) return;
+    const std::uint32_t M = counters[0];
+    const unsigned dk = d[k], lleaf = leaf_level[k];
+    const std::uint32_t base = first_node[k];
+    const std::uint64_t key = codes[k];
+    // Ends from the leaf upwards: each is a lower bound for the next search.
+    std::uint32_t ends[kLevels + 1];
+    std::uint32_t lo = k + 1;
+    for (int l = static_cast<int>(lleaf); l >= static_cast<int>(dk); --l) {
+        lo = prefix_end(codes, n, lo, key, static_cast<unsigned>(l));
+        ends[l] = lo;
+    }
+    double size = cube[3];
+    double c[3] = {cube[0] + size / 2, cube[1] + size / 2, cube[2] + size / 2};
+    for (unsigned l = 0; l <= lleaf; ++l) {
+        if (l > 0) {
+            const unsigned oct = static_cast<unsigned>((key >> (3u * (kLevels - l))) & 7u);
+            const double q = size / 4;
+            c[0] = c[0] + ((oct & 4u) != 0 ? q : -q);
+            c[1] = c[1] + ((oct & 2u) != 0 ? q : -q);
+            c[2] = c[2] + ((oct & 1u) != 0 ? q : -q);
+            size = size / 2;
+        }
+        if (l < dk) continue;
+        const std::uint32_t node = base + (l - dk);
+        const std::uint32_t end = ends[l];
+        node_begin[node] = k;
+        node_count[node] = end - k;
+        node_level[node] = static_cast<std::uint8_t>(l);
+        node_next[node] = end >= n ? M : first_node[end];
+        node_geo[node] = make_double4(c[0], c[1], c[2], size);
+        if (l == lleaf) {
+            for (std::uint32_t j = k; j < end; ++j) leaf_of[j] = node;
+            atomicAdd(&counters[1], 1u);
+            atomicMax(&counters[2], l);
+        }
+    }
+    (void)cap;
+}
+
+// Sorted FP64 particles for the moments; FP32 walk data: position relative to
+// the root centre, and offset from the particle's leaf centre.
+__global__ __launch_bounds__(kBlock) void tree_gather(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, const std::uint32_t* __restrict__ index, std::uint32_t n,
+    const double* __restrict__ cube, const std::uint32_t* __restrict__ leaf_of,
+    const double4* __restrict__ node_geo, double4* __restrict__ sorted,
+    float4* __restrict__ root_rel, float4* __restrict__ leaf_rel) {
+    const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const std::uint32_t src = index[i];
+    const double px = x[src], py = y[src], pz = z[src], pm = m[src];
+    sorted[i] = make_double4(px, py, pz, pm);
+    const double half = cube[3] / 2;
+    const double c0x = cube[0] + half, c0y = cube[1] + half, c0z = cube[2] + half;
+    root_rel[i] = make_float4(static_cast<float>(px - c0x), static_cast<float>(py - c0y),
+                              static_cast<float>(pz - c0z), static_cast<float>(pm));
+    const double4 g = node_geo[leaf_of[i]];
+    leaf_rel[i] = make_float4(static_cast<float>(px - g.x), static_cast<float>(py - g.y),
+                              static_cast<float>(pz - g.z), static_cast<float>(pm));
+}
+
+__device__ inline void finish_node(std::uint32_t node, double mass, double cx, double cy, double cz,
+                                   const double4 geo, double theta, const double* __restrict__ cube,
+                                   bool leaf, std::uint32_t begin, std::uint32_t count,
+                                   std::uint32_t next, double4* __restrict__ moment,
+                                   WalkNode* __restrict__ walk) {
+    moment[node] = make_double4(cx, cy, cz, mass);
+    // octree.cpp:370-371 and :374-387.
+    const double ox = cx - geo.x, oy = cy - geo.y, oz = cz - geo.z;
+    const double offset = sqrt((ox * ox) + (oy * oy) + (oz * oz));
+    double acc;
+    if (!(theta > 0)) {
+        acc = __builtin_huge_val();
+    } else {
+        const double r = (geo.w / theta) + offset;
+        acc = r * r;
+    }
+    const double half = cube[3] / 2;
+    WalkNode w;
+    w.a = make_float4(static_cast<float>(cx - (cube[0] + half)), static_cast<float>(cy - (cube[1] + half)),
+                      static_cast<float>(cz - (cube[2] + half)), static_cast<float>(mass));
+    float accf = static_cast<float>(acc);
+    std::uint32_t bits;
+    std::memcpy(&bits, &accf, sizeof(bits));
+    w.b = make_uint4(bits, next, leaf ? begin : 0u, leaf ? count : 0u);
+    walk[node] = w;
+}
+
+// octree.cpp:91-135 (leaf): index-order sums, fallback to the first particle.
+__global__ __launch_bounds__(kBlock) void leaf_moments(
+    const std::uint32_t* __restrict__ counters, const std::uint8_t* __restrict__ node_level,
+    const std::uint8_t* __restrict__ leaf_level_of_first, const std::uint32_t* __restrict__ node_begin,
+    const std::uint32_t* __restrict__ node_count, const std::uint32_t* __restrict__ node_next,
+    const double4* __restrict__ node_geo, const double4* __restrict__ sorted, double theta,
+    const double* __restrict__ cube, double4* __restrict__ moment, WalkNode* __restrict__ walk) {
+    const std::uint32_t M = counters[0];
+    for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
+        const std::uint32_t b = node_begin[node];
+        if (node_level[node] != leaf_level_of_first[b]) continue;  // internal
+        const std::uint32_t e = b + node_count[node];
+        double mass = 0, wx = 0, wy = 0, wz = 0;
+        for (std::uint32_t i = b; i < e; ++i) {
+            const double4 p = sorted[i];
+            mass += p.w;
+            wx += p.x * p.w;
+            wy += p.y * p.w;
+            wz += p.z * p.w;
+        }
+        double cx, cy, cz;
+        if (mass > 0) {
+            cx = wx / mass;
+            cy = wy / mass;
+            cz = wz / mass;
+        } else {
+            cx = sorted[b].x;
+            cy = sorted[b].y;
+            cz = sorted[b].z;
+        }
+        finish_node(node, mass, cx, cy, cz, node_geo[node], theta, cube, true, b, e - b,
+                    node_next[node], moment, walk);
+    }
+}
+
+// octree.cpp:142-185 (internal): children in DFS order via the next chain.
+__global__ __launch_bounds__(kBlock) void internal_moments(
+    unsigned level, const std::uint32_t* __restrict__ counters,
+    const std::uint8_t* __restrict__ node_level, const std::uint8_t* __restrict__ leaf_level_of_first,
+    const std::uint32_t* __restrict__ node_begin, const std::uint32_t* __restrict__ node_next,
+    const double4* __restrict__ node_geo, double theta, const double* __restrict__ cube,
+    double4* __restrict__ moment, WalkNode* __restrict__ walk) {
+    const std::uint32_t M = counters[0];
+    for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
+        if (node_level[node] != level) continue;
+        const std::uint32_t b = node_begin[node];
+        if (leaf_level_of_first[b] == level) continue;  // a leaf
+        const std::uint32_t end = node_next[node];
+        double mass = 0, wx = 0, wy = 0, wz = 0, mx = 0, my = 0, mz = 0;
+        std::uint32_t children = 0;
+        for (std::uint32_t c = node + 1; c < end; c = max(node_next[c], c + 1)) {
+            const double4 cm = moment[c];
+            mass += cm.w;
+            wx += cm.x * cm.w;
+            wy += cm.y * cm.w;
+            wz += cm.z * cm.w;
+            mx += cm.x;
+            my += cm.y;
+            mz += cm.z;
+            ++children;
+        }
+        double cx, cy, cz;
+        if (mass > 0) {
+            cx = wx / mass;
+            cy = wy / mass;
+            cz = wz / mass;
+        } else {
+            const double k = static_cast<double>(children == 0 ? 1 : children);
+            cx = mx / k;
+            cy = my / k;
+            cz = mz / k;
+        }
+        finish_node(node, mass, cx, cy, cz, node_geo[node], theta, cube, false, b, 0,
+                    node_next[node], moment, walk);
+    }
+}
+
+__device__ inline std::uint32_t wave_min_u32(std::uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), off, 64)));
+    return v;
+}
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void kahan(float p, float& s, float& c) {
+    const float y = p - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+template<bool kSafe>
+__global__ __launch_bounds__(kBlock) void tree_walk(
+    const WalkNode* __restrict__ nodes, const double4* __restrict__ node_geo,
+    const float4* __restrict__ root_rel, const float4* __restrict__ leaf_rel,
+    const std::uint32_t* __restrict__ leaf_of, const std::uint32_t* __restrict__ index,
+    const std::uint32_t* __restrict__ counters, std::uint32_t n, float eps2, double G,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    unsigned long long* __restrict__ stats) {
+    const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    const std::uint32_t M = counters[0];
+    const std::uint32_t ic = valid ? i : n - 1;
+    const float4 xi = root_rel[ic];
+    const float4 oi = leaf_rel[ic];
+    const std::uint32_t own = leaf_of[ic];
+    const double4 ca = node_geo[own];
+    std::uint32_t mine = valid ? 0u : M;
+    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    unsigned long long pp = 0, pc = 0, visits = 0, steps = 0;
+    std::uint32_t cur = __builtin_amdgcn_readfirstlane(wave_min_u32(mine));
+    while (cur < M) {
+        ++steps;
+        const WalkNode nd = nodes[cur];
+        // An escape index always lies past its node; the guard only makes a
+        // corrupted tree end the walk instead of looping.
+        const std::uint32_t next = nd.b.y > cur ? nd.b.y : M;
+        bool leaf_pp = false;
+        if (mine == cur) {
+            ++visits;
+            const float dx = nd.a.x - xi.x, dy = nd.a.y - xi.y, dz = nd.a.z - xi.z;
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            float acc;
+            std::memcpy(&acc, &nd.b.x, sizeof(acc));
+            if (r2 >= acc) {
+                // tree_walk.hpp:129-139 monopole.
+                float r2e = r2 + eps2;
+                if constexpr (kSafe) r2e = r2e > 0.f ? r2e : 1.f;
+                const float inv = __builtin_amdgcn_rsqf(r2e);
+                const float f = (nd.a.w * inv) * (inv * inv);
+                px = __builtin_fmaf(f, dx, px);
+                py = __builtin_fmaf(f, dy, py);
+                pz = __builtin_fmaf(f, dz, pz);
+                ++pc;
+                mine = next;
+            } else if (nd.b.w > 0) {
+                leaf_pp = true;
+                pp += nd.b.w - (own == cur ? 1u : 0u);
+                mine = next;
+            } else {
+                mine = cur + 1;
+            }
+        }
+        if (__ballot(leaf_pp) != 0) {
+            const double4 cb = node_geo[cur];
+            const float tx = oi.x - static_cast<float>(cb.x - ca.x);
+            const float ty = oi.y - static_cast<float>(cb.y - ca.y);
+            const float tz = oi.z - static_cast<float>(cb.z - ca.z);
+            const std::uint32_t b = nd.b.z, e = nd.b.z + nd.b.w;
+            if (leaf_pp) {
+                for (std::uint32_t j = b; j < e; ++j) {
+                    const float4 q = leaf_rel[j];
+                    const float dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+                    float r2 = __builtin_fmaf(dx, dx, eps2);
+                    r2 = __builtin_fmaf(dy, dy, r2);
+                    r2 = __builtin_fmaf(dz, dz, r2);
+                    if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float f = (q.w * inv) * (inv * inv);
+                    px = __builtin_fmaf(f, dx, px);
+                    py = __builtin_fmaf(f, dy, py);
+                    pz = __builtin_fmaf(f, dz, pz);
+                }
+                kahan(px, sx, kx);
+                kahan(py, sy, ky);
+                kahan(pz, sz, kz);
+                px = py = pz = 0.f;
+            }
+        }
+        const std::uint32_t step = cur + 1;
+        cur = (__ballot(mine == step) != 0) ? step : wave_min_u32(mine);
+        cur = __builtin_amdgcn_readfirstlane(cur);
+    }
+    kahan(px, sx, kx);
+    kahan(py, sy, ky);
+    kahan(pz, sz, kz);
+    if (valid) {
+        const std::uint32_t dst = index[i];
+        ax[dst] = G * (static_cast<double>(sx) - static_cast<double>(kx));
+        ay[dst] = G * (static_cast<double>(sy) - static_cast<double>(ky));
+        az[dst] = G * (static_cast<double>(sz) - static_cast<double>(kz));
+    }
+    pp = wave_sum_u64(pp);
+    pc = wave_sum_u64(pc);
+    visits = wave_sum_u64(visits);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], pp);
+        atomicAdd(&stats[1], pc);
+        atomicAdd(&stats[2], visits);
+        atomicAdd(&stats[3], steps);
+    }
+}
+
+template<typename T> T* carve(char*& p, std::size_t count) {
+    T* out = reinterpret_cast<T*>(p);
+    p += (sizeof(T) * count + 255) & ~static_cast<std::size_t>(255);
+    return out;
+}
+
+struct Layout {
+    // Per particle.
+    std::uint8_t* d;
+    std::uint8_t* leaf_level;
+    std::uint32_t* chain;
+    std::uint32_t* first_node;
+    std::uint32_t* leaf_of;
+    double4* sorted;
+    float4* root_rel;
+    float4* leaf_rel;
+    std::uint32_t* scan_scratch;
+    // Per node.
+    std::uint32_t* begin;
+    std::uint32_t* count;
+    std::uint32_t* next;
+    std::uint8_t* level;
+    double4* geo;
+    double4* moment;
+    WalkNode* walk;
+    // Counters.
+    std::uint32_t* c32;
+    unsigned long long* c64;
+};
+
+std::size_t node_bound(std::uint64_t n, std::uint64_t cap) {
+    // Leaves partition the particles (<= n); internal nodes of one level are
+    // disjoint cells of > cap particles (<= n / (cap + 1) per level, 21 levels).
+    const std::uint64_t by_levels = n + 21 * (n / (cap + 1)) + 22;
+    const std::uint64_t by_chains = 22 * n;
+    return static_cast<std::size_t>(std::min(by_levels, by_chains));
+}
+
+Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool allocate) {
+    // Every carved array is rounded up to 256 bytes: 19 arrays of padding.
+    const std::size_t per_particle = 2 * n + 3 * 4 * n + 32 * n + 2 * 16 * n +
+                                     4 * scan_scratch_words(n);
+    const std::size_t per_node = nodes * (4 * 3 + 1 + 32 + 32 + sizeof(WalkNode));
+    const std::size_t bytes = per_particle + per_node + 19 * 256 + 64;
+    if (allocate) ctx->tree_work.reserve(bytes);
+    char* p = static_cast<char*>(ctx->tree_work.ptr);
+    Layout L{};
+    L.d = carve<std::uint8_t>(p, n);
+    L.leaf_level = carve<std::uint8_t>(p, n);
+    L.chain = carve<std::uint32_t>(p, n);
+    L.first_node = carve<std::uint32_t>(p, n);
+    L.leaf_of = carve<std::uint32_t>(p, n);
+    L.sorted = carve<double4>(p, n);
+    L.root_rel = carve<float4>(p, n);
+    L.leaf_rel = carve<float4>(p, n);
+    L.scan_scratch = carve<std::uint32_t>(p, scan_scratch_words(n));
+    L.begin = carve<std::uint32_t>(p, nodes);
+    L.count = carve<std::uint32_t>(p, nodes);
+    L.next = carve<std::uint32_t>(p, nodes);
+    L.level = carve<std::uint8_t>(p, nodes);
+    L.geo = carve<double4>(p, nodes);
+    L.moment = carve<double4>(p, nodes);
+    L.walk = carve<WalkNode>(p, nodes);
+    L.c32 = carve<std::uint32_t>(p, 4);
+    L.c64 = carve<unsigned long long>(p, 4);
+    return L;
+}
+
+}  // namespace
+
+void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) {
+    const std::uint64_t n = a.count;
+    if (stats != nullptr) std::memset(stats, 0, sizeof(*stats));
+    ctx->last_tree = orr_tree_stats{};
+    ctx->tree_last_n = n;
+    if (n == 0) return;
+    if (n >= (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "tree: 2^31 or more particles");
+    // octree.cpp:189-194 parameter correction.
+    const double theta = std::clamp(a.opening_angle, 0.0, 1.0);
+    const std::uint64_t cap64 = std::max<std::uint64_t>(a.leaf_capacity, 1);
+    const std::uint32_t cap = static_cast<std::uint32_t>(std::min<std::uint64_t>(cap64, n));
+    const std::size_t bound = node_bound(n, cap);
+    ctx->tree_last_bound = bound;
+    ctx->tree_last_theta = theta;
+    const Layout L = layout(ctx, n, bound, true);
+    const std::uint32_t n32 = static_cast<std::uint32_t>(n);
+    const std::uint32_t blocks = (n32 + kBlock - 1) / kBlock;
+    const std::uint32_t node_blocks = static_cast<std::uint32_t>(std::min<std::size_t>(4096, (bound + kBlock - 1) / kBlock));
+
+    morton_enqueue(ctx, a.x, a.y, a.z, n);
+    const std::uint64_t* codes = ctx->order_codes.as<std::uint64_t>();
+    const std::uint32_t* index = ctx->order_index.as<std::uint32_t>();
+    const double* cube = ctx->order_bbox.as<double>();
+    {
+        PhaseTimer t(ctx, kBuild);
+        hipLaunchKernelGGL(clear_counters, dim3(1), dim3(64), 0, ctx->stream, L.c32, L.c64);
+        ORR_LAUNCHED("clear_counters");
+        hipLaunchKernelGGL(tree_levels, dim3(blocks), dim3(kBlock), 0, ctx->stream, codes, n32, L.d);
+        ORR_LAUNCHED("tree_levels");
+        hipLaunchKernelGGL(tree_chain, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.d, n32, cap,
+                           L.chain, L.leaf_level);
+        ORR_LAUNCHED("tree_chain");
+        exclusive_scan_u32(ctx, L.chain, L.first_node, n, L.scan_scratch, &L.c32[0]);
+        hipLaunchKernelGGL(tree_nodes, dim3(blocks), dim3(kBlock), 0, ctx->stream, codes, L.d,
+                           L.leaf_level, L.chain, L.first_node, n32, cap, cube, L.c32, L.begin,
+                           L.count, L.next, L.level, L.geo, L.leaf_of);
+        ORR_LAUNCHED("tree_nodes");
+    }
+    {
+        PhaseTimer t(ctx, kGather);
+        hipLaunchKernelGGL(tree_gather, dim3(blocks), dim3(kBlock), 0, ctx->stream, a.x, a.y, a.z,
+                           a.m, index, n32, cube, L.leaf_of, L.geo, L.sorted, L.root_rel,
+                           L.leaf_rel);
+        ORR_LAUNCHED("tree_gather");
+    }
+    {
+        PhaseTimer t(ctx, kMoments);
+        hipLaunchKernelGGL(leaf_moments, dim3(node_blocks), dim3(kBlock), 0, ctx->stream, L.c32,
+                           L.level, L.leaf_level, L.begin, L.count, L.next, L.geo, L.sorted, theta,
+                           cube, L.moment, L.walk);
+        ORR_LAUNCHED("leaf_moments");
+        for (int level = static_cast<int>(kLevels) - 1; level >= 0; --level) {
+            hipLaunchKernelGGL(internal_moments, dim3(node_blocks), dim3(kBlock), 0, ctx->stream,
+                               static_cast<unsigned>(level), L.c32, L.level, L.leaf_level, L.begin,
+                               L.next, L.geo, theta, cube, L.moment, L.walk);
+            ORR_LAUNCHED("internal_moments");
+        }
+    }
+    {
+        PhaseTimer t(ctx, kWalk);
+        const float eps2 = static_cast<float>(a.softening_squared);
+        if (a.softening_squared > 0) {
+            hipLaunchKernelGGL(tree_walk<false>, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.walk,
+                               L.geo, L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, n32, eps2,
+                               a.gravitational_constant, a.ax, a.ay, a.az, L.c64);
+        } else {
+            hipLaunchKernelGGL(tree_walk<true>, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.walk,
+                               L.geo, L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, n32, eps2,
+                               a.gravitational_constant, a.ax, a.ay, a.az, L.c64);
+        }
+        ORR_LAUNCHED("tree_walk");
+    }
+    if (stats != nullptr) {
+        std::uint32_t c32[4];
+        unsigned long long c64[4];
+        ORR_HIP(hipMemcpyAsync(c32, L.c32, sizeof(c32), hipMemcpyDeviceToHost, ctx->stream));
+        ORR_HIP(hipMemcpyAsync(c64, L.c64, sizeof(c64), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        stats->node_count = c32[0];
+        stats->leaf_count = c32[1];
+        stats->depth = c32[2];
+        stats->particle_particle = c64[0];
+        stats->particle_cell = c64[1];
+        stats->node_visits = c64[2];
+        ctx->last_tree = *stats;
+    }
+}
+
+void tree_copy_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double* cz, double* acc,
+                     std::uint64_t* next, std::uint64_t* first, std::uint64_t* count) {
+    const std::uint64_t n = ctx->tree_last_n;
+    if (n == 0 || ctx->tree_work.ptr == nullptr) return;
+    const Layout L = layout(ctx, n, ctx->tree_last_bound, false);
+    std::uint32_t M = 0;
+    ORR_HIP(hipMemcpyAsync(&M, L.c32, sizeof(M), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    std::vector<double4> mom(M), geo(M);
+    std::vector<WalkNode> walk(M);
+    std::vector<std::uint32_t> nx(M), b(M), c(M);
+    if (M > 0) {
+        ORR_HIP(hipMemcpyAsync(mom.data(), L.moment, M * sizeof(double4), hipMemcpyDeviceToHost, ctx->stream));
+        ORR_HIP(hipMemcpyAsync(geo.data(), L.geo, M * sizeof(double4), hipMemcpyDeviceToHost, ctx->stream));
+        ORR_HIP(hipMemcpyAsync(walk.data(), L.walk, M * sizeof(WalkNode), hipMemcpyDeviceToHost, ctx->stream));
+        ORR_HIP(hipMemcpyAsync(nx.data(), L.next, M * sizeof(std::uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+    }
+    const double theta = ctx->tree_last_theta;
+    for (std::uint32_t k = 0; k < M; ++k) {
+        if (mass) mass[k] = mom[k].w;
+        if (cx) cx[k] = mom[k].x;
+        if (cy) cy[k] = mom[k].y;
+        if (cz) cz[k] = mom[k].z;
+        if (acc) {
+            // The FP64 radius of octree.cpp:374-387 (the walk holds it rounded to FP32).
+            const double ox = mom[k].x - geo[k].x, oy = mom[k].y - geo[k].y, oz = mom[k].z - geo[k].z;
+            const double offset = std::sqrt((ox * ox) + (oy * oy) + (oz * oz));
+            if (!(theta > 0)) {
+                acc[k] = std::numeric_limits<double>::infinity();
+            } else {
+                const double r = (geo[k].w / theta) + offset;
+                acc[k] = r * r;
+            }
+        }
+        if (next) next[k] = nx[k];
+        if (first) first[k] = walk[k].b.z;
+        if (count) count[k] = walk[k].b.w;
+    }
 }
 
 }  // namespace orr

@@ -1,0 +1,172 @@
+"""Barnes-Hut on the device against the oracle tree and the FP64 direct sum.
+
+Mirrors tests/solvers/octree_test.cpp, barnes_hut_test.cpp, tree_walk_test.cpp
+and cuda_tree_solver_test.cpp:
+* structure parity: node count, leaf count, depth and the next / first / count
+  arrays equal the reference tree's (the oracle is pinned bit for bit to the
+  reference library), and masses, centres of mass and FP64 acceptance radii are
+  equal bit for bit (the device sums in the reference's order without
+  contraction);
+* interaction counts equal, exactly, those of the oracle's mirror of the
+  device's FP32 acceptance test on the same tree (the reference requires exact
+  count equality only between walks with identical arithmetic,
+  cuda_tree_solver_test.cpp:161-222);
+* theta = 0 is the direct sum with pp = N(N-1), pc = 0 (barnes_hut_test.cpp:111-152);
+* accuracy against the FP64 reference: RMS < 1e-2, worst < 1e-1 at theta = 0.5
+  (cuda_tree_solver_test.cpp:392-412), error monotone in theta.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import orrery_amd as E
+
+pytestmark = pytest.mark.gpu
+
+
+def tree_eval(ctx, p, eps, theta, cap):
+    s = E.HipTreeSolver(E.TreeParameters(theta, cap), E.Softening(eps), context=ctx)
+    acc = [np.full(p.n, np.nan) for _ in range(3)]
+    s.evaluate((p.x, p.y, p.z), p.m, acc)
+    return acc, s
+
+
+def oracle_tree(p, theta, cap):
+    codes, index, origin, size = oracle.morton_order(p.x, p.y, p.z)
+    i = index.astype(np.int64)
+    return oracle.tree_build(p.x[i], p.y[i], p.z[i], p.m[i], codes, origin, size, theta, cap)
+
+
+@pytest.mark.parametrize("gen,n,theta,cap", [("plummer", 5000, 0.5, 16), ("plummer", 3000, 0.6, 1),
+                                             ("plummer", 20000, 0.5, 32), ("disc", 30000, 0.5, 16),
+                                             ("plummer", 7, 0.5, 32), ("plummer", 2000, 0.0, 8)])
+def test_structure_and_moments_bitwise(ctx, gen, n, theta, cap):
+    p = oracle.plummer(n, 20260811) if gen == "plummer" else oracle.disc_collision(n, 2)
+    _, s = tree_eval(ctx, p, 0.05, theta, cap)
+    t = oracle_tree(p, theta, cap)
+    got = s.nodes()
+    summary = s.tree_summary()
+    assert summary["node_count"] == len(t.mass)
+    assert summary["leaf_count"] == t.leaves and summary["depth"] == t.depth
+    assert np.array_equal(got["next"], t.next)
+    assert np.array_equal(got["first"], t.first) and np.array_equal(got["count"], t.count)
+    assert np.array_equal(got["mass"], t.mass)
+    assert np.array_equal(got["com"], t.com)
+    assert np.array_equal(got["acc_r2"], t.acc_r2)
+
+
+def test_identical_points_stop_at_depth_21(ctx):
+    n = 40
+    x = np.full(n, 0.25)
+    p = oracle.Particles(x, x.copy(), x.copy(), np.zeros(n), np.zeros(n), np.zeros(n), np.full(n, 1.0 / n))
+    (ax, ay, az), s = tree_eval(ctx, p, 0.1, 0.5, 4)
+    assert s.tree_summary()["depth"] == 21 and s.tree_summary()["leaf_count"] == 1
+    assert np.all(ax == 0) and np.all(ay == 0) and np.all(az == 0)
+
+
+@pytest.mark.parametrize("gen,n,theta,cap", [("plummer", 20000, 0.5, 16), ("plummer", 20000, 0.8, 32),
+                                             ("disc", 50000, 0.5, 16), ("disc", 50000, 0.6, 16)])
+def test_counts_equal_fp32_policy_mirror(ctx, gen, n, theta, cap):
+    p = oracle.plummer(n, 20260811) if gen == "plummer" else oracle.disc_collision(n, 3)
+    _, s = tree_eval(ctx, p, 0.01, theta, cap)
+    c = s.interaction_count()
+    pp, pc = oracle.barnes_hut_fp32_counts(p.x, p.y, p.z, p.m, theta, cap)
+    assert (c.particle_particle, c.particle_cell) == (pp, pc)
+    # and, on these inputs, the FP64 reference walk's counts as well
+    _, _, _, (rpp, rpc) = oracle.barnes_hut(p.x, p.y, p.z, p.m, 1e-4, theta, cap)
+    print(f"device pp={pp} pc={pc}; FP64 walk pp={rpp} pc={rpc}")
+    assert abs(pp - rpp) <= 1e-4 * rpp and abs(pc - rpc) <= 1e-4 * max(1, rpc)
+
+
+def test_theta_zero_is_direct_sum(ctx):
+    p = oracle.plummer(3000, 4)
+    eps = 0.02
+    (ax, ay, az), s = tree_eval(ctx, p, eps, 0.0, 16)
+    c = s.interaction_count()
+    assert c.particle_particle == 3000 * 2999 and c.particle_cell == 0
+    ref = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps)
+    worst, rms = oracle.error_summary(oracle.relative_errors(ax, ay, az, *ref))
+    assert rms < 1e-6 and worst < 2e-5, (rms, worst)
+
+
+def test_two_body_and_self(ctx):
+    x = np.array([-1.0, 1.0])
+    z = np.zeros(2)
+    p = oracle.Particles(x, z, z.copy(), z.copy(), z.copy(), z.copy(), np.array([2.0, 3.0]))
+    (ax, ay, az), s = tree_eval(ctx, p, 0.0, 0.5, 1)
+    assert ax.tolist() == [0.75, -0.5] and ay.tolist() == [0.0, 0.0]
+    one = oracle.Particles(np.array([2.0]), np.array([-3.0]), np.array([5.0]), *(np.zeros(1) for _ in range(3)),
+                           np.array([4.0]))
+    (ax, ay, az), _ = tree_eval(ctx, one, 0.0, 0.5, 16)
+    assert (ax[0], ay[0], az[0]) == (0.0, 0.0, 0.0)
+
+
+def test_empty_configuration(ctx):
+    s = E.HipTreeSolver(E.TreeParameters(0.5, 16), E.Softening(0.1), context=ctx)
+    e = np.zeros(0)
+    s.evaluate((e, e, e), e, (e.copy(), e.copy(), e.copy()))
+    assert s.interaction_count().evaluations == 1 and s.name() == "hip-tree"
+
+
+def test_parameters_are_corrected(ctx):
+    s = E.HipTreeSolver(E.TreeParameters(1.7, 0), E.Softening(0.1), context=ctx)
+    assert s.parameters().opening_angle == 1.0 and s.parameters().leaf_capacity == 1
+
+
+def test_accuracy_and_monotone_in_theta(ctx):
+    p = oracle.plummer(20000, 20260811, scale_radius=3 * np.pi / 16, cutoff=0.999)
+    eps = 0.05
+    rng = np.random.default_rng(1)
+    tgt = np.sort(rng.choice(p.n, 2048, replace=False)).astype(np.uint64)
+    ref = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps, targets=tgt)
+    i = tgt.astype(np.int64)
+    rms = []
+    for theta in (0.3, 0.5, 0.7):
+        (ax, ay, az), _ = tree_eval(ctx, p, eps, theta, 16)
+        w, r = oracle.error_summary(oracle.relative_errors(ax[i], ay[i], az[i], *ref))
+        print(f"theta={theta} rms={r:.3e} worst={w:.3e}")
+        rms.append(r)
+        if theta == 0.5:
+            assert r < 1e-2 and w < 1e-1
+    assert rms[0] < rms[1] < rms[2]
+
+
+def test_momentum_residual(ctx):
+    # cuda_tree_solver_test.cpp:535-568: sum m a small against sum m |a|.
+    p = oracle.plummer(20000, 7)
+    (ax, ay, az), _ = tree_eval(ctx, p, 0.02, 0.5, 16)
+    scale = np.sum(p.m * np.sqrt(ax * ax + ay * ay + az * az))
+    net = np.abs([np.sum(p.m * ax), np.sum(p.m * ay), np.sum(p.m * az)]) / scale
+    assert np.all(net < 1e-3), net
+
+
+def test_disc_collision_262144_sampled(ctx):
+    p = oracle.disc_collision(262144, 1)
+    eps, theta, cap = 0.01, 0.5, 16
+    (ax, ay, az), s = tree_eval(ctx, p, eps, theta, cap)
+    rng = np.random.default_rng(2)
+    tgt = np.sort(rng.choice(p.n, 1024, replace=False)).astype(np.uint64)
+    ref = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps, targets=tgt)
+    i = tgt.astype(np.int64)
+    w, r = oracle.error_summary(oracle.relative_errors(ax[i], ay[i], az[i], *ref))
+    print(f"disc N=262144 theta=0.5 leaf 16: rms={r:.3e} worst={w:.3e} {s.tree_summary()}")
+    assert r < 1e-2 and w < 1e-1
+
+
+def test_device_resident_tree_stepping(ctx):
+    # Barnes-Hut as the field of device-resident Yoshida-4 steps vs the CPU path.
+    p = oracle.plummer(3000, 12)
+    eps, dt = 0.02, 1e-3
+    sysd = E.DeviceSystem(ctx, p.n)
+    sysd.upload(*p.arrays())
+    cfg = E.StepConfig(integrator="yoshida4", solver="barnes-hut", softening=eps, opening_angle=0.5,
+                       leaf_capacity=16)
+    sysd.refresh(cfg)
+    sysd.step(cfg, dt, 5)
+    st = sysd.download()
+    f = oracle.make_field("yoshida4", "barnes-hut", eps2=eps * eps, theta=0.5, leaf_capacity=16)
+    ref = oracle.Particles(*(a.copy() for a in p.arrays()))
+    acc = oracle.refresh(ref, f)
+    oracle.steps(ref, acc, f, dt, 5)
+    assert np.max(np.abs(st["x"] - ref.x)) < 1e-8
+    sysd.close()
