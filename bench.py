@@ -90,6 +90,106 @@ def cpu_baseline(state, eps: float, seconds: float = 10.0) -> dict:
                       f"work-stealing executor, {dt:.1f} s"}
 
 
+def measured_traffic(n: int):
+    """HBM bytes per direct_tiles launch from the committed, calibrated PMC profile."""
+    path = os.path.join(HERE, "profiles", "direct_tiles_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("traffic_bytes") if f"N={n}," in t.get("workload", "") else None
+
+
+def measure_ceilings(ctx) -> dict:
+    """FP32 FMA, v_rsq_f32, HBM read and copy rates measured by the engine's probes."""
+    ctx.probe("fma", 256)  # warm-up
+    fma, _ = ctx.probe("fma", 4096)
+    rsq, _ = ctx.probe("rsq", 1024)
+    ctx.probe("read", 1 << 28)
+    read, _ = ctx.probe("read", 1 << 31)
+    copy, copy_ms = ctx.probe("copy", 1 << 30)
+    return {"fp32_fma_TFLOPs": fma / 1e12, "rsq_Gops": rsq / 1e9, "hbm_read_GBs": read / 1e9,
+            "hbm_copy_GBs": copy / 1e9, "copy_bytes": 2 * (1 << 30), "copy_ms": copy_ms}
+
+
+def bench_barnes_hut(E, oracle, ctx, n: int, evals: int, error_targets: int, cpu: bool) -> dict:
+    """Barnes-Hut on the two-disc collision: N particles, theta 0.5, leaf 16."""
+    import ctypes
+
+    eps, theta, cap = 0.01, 0.5, 16
+    log(f"[bench] Barnes-Hut: generating two-disc collision N={n}")
+    p = oracle.disc_collision(n, 1)
+    d = [E.DeviceArray.from_host(ctx, a) for a in (p.x, p.y, p.z, p.m)]
+    out = [E.DeviceArray(ctx, n) for _ in range(3)]
+    s = E.HipTreeSolver(E.TreeParameters(theta, cap, quadrupole=False), E.Softening(eps), context=ctx)
+    s.evaluate_device(*(a.ptr for a in d), *(a.ptr for a in out), n)  # warm-up
+    s.reset_interaction_count()
+    ctx.set_timing(True)
+    ctx.reset_timings()
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        s.evaluate_device(*(a.ptr for a in d), *(a.ptr for a in out), n)
+    wall = (time.perf_counter() - t0) / evals
+    phases = {k: v[0] / evals for k, v in ctx.timings().items()}
+    ctx.set_timing(False)
+    c = s.interaction_count()
+    tree = s.tree_summary()
+    acc = [a.to_host() for a in out]
+    rng = np.random.default_rng(2)
+    tgt = np.sort(rng.choice(n, size=min(n, error_targets), replace=False)).astype(np.uint64)
+    ref = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps, targets=tgt, threads=cpu_threads())
+    i = tgt.astype(np.int64)
+    worst, rms = oracle.error_summary(oracle.relative_errors(acc[0][i], acc[1][i], acc[2][i], *ref))
+    M = tree["node_count"]
+    build_phases = ("bbox", "encode", "sort", "gather", "build", "moments")
+    build_ms = sum(phases.get(k, 0.0) for k in build_phases)
+    # Compulsory bytes per evaluation of the streaming stages: 228 N + 64 M
+    # (bbox 24N, encode 36N, sort 24N, gather 52N, build 24N + 32M, walk 28N + 32M,
+    # scatter 40N); the walk's own stream lives in L2 / LDS.
+    alg_bytes = 228 * n + 64 * M
+    pp, pc = c.particle_particle // evals, c.particle_cell // evals
+    out_d = {
+        "workload": "Barnes-Hut, two exponential discs + Hernquist bulges, parabolic orbit",
+        "n": n, "opening_angle": theta, "leaf_capacity": cap, "softening": eps,
+        "ms_per_evaluation": wall * 1e3,
+        "phases_ms": phases,
+        "interactions_per_evaluation": {"particle_particle": pp, "particle_cell": pc},
+        "walk_interactions_per_s": (pp + pc) / (phases.get("walk", 0.0) * 1e-3) if phases.get("walk") else None,
+        "tree": tree,
+        "force_error": {"rms_relative": rms, "max_relative": worst, "targets": int(len(tgt)),
+                        "reference": "FP64 Neumaier direct sum"},
+        "streaming_stages": {"algorithmic_bytes": alg_bytes, "ms": build_ms,
+                             "achieved_GBs": alg_bytes / (build_ms * 1e-3) / 1e9 if build_ms else None,
+                             "peak_GBs": HBM_PEAK_GBS},
+    }
+    for a in d + out:
+        a.release()
+    if cpu:
+        ref_lib = oracle.load_reference()
+        sub = 262144
+        q = oracle.disc_collision(sub, 1)
+        ax, ay, az = np.zeros(sub), np.zeros(sub), np.zeros(sub)
+        counts = np.zeros(2, dtype=np.uint64)
+        from oracle import _ptr, _u64, _u64p
+        t0 = time.perf_counter()
+        if ref_lib is not None:
+            kind = "reference"
+            ref_lib.ref_barnes_hut(_u64(sub), _ptr(q.x), _ptr(q.y), _ptr(q.z), _ptr(q.m),
+                                   ctypes.c_double(eps), ctypes.c_double(theta), _u64(cap), ctypes.c_int(0),
+                                   ctypes.c_int(1), ctypes.c_uint(cpu_threads()), _ptr(ax), _ptr(ay), _ptr(az),
+                                   _ptr(counts, _u64p))
+        else:
+            kind = "port"
+            oracle.barnes_hut(q.x, q.y, q.z, q.m, eps * eps, theta, cap, threads=cpu_threads())
+        dt = time.perf_counter() - t0
+        out_d["cpu_baseline"] = {"value": dt * 1e3, "unit": "ms/evaluation", "cores": cpu_threads(),
+                                 "kind": kind,
+                                 "sample": f"N={sub} two-disc collision (same generator), theta 0.5, "
+                                           f"leaf 16, the reference's threaded BarnesHutSolver"}
+    return out_d
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +199,11 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--error-targets", type=int, default=1024)
+    ap.add_argument("--bh-n", type=int, default=2097152)
+    ap.add_argument("--bh-evals", type=int, default=5)
+    ap.add_argument("--no-barnes-hut", action="store_true")
+    ap.add_argument("--probes-only", action="store_true",
+                    help="run only the machine-ceiling probes (copy probe: counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -111,7 +216,12 @@ def main() -> int:
 
     import oracle
     import orrery_amd as E
-    from orrery_amd import _capi
+
+    if args.probes_only:
+        ctx = E.Context(0)
+        print(json.dumps({"ceilings": measure_ceilings(ctx)}), flush=True)
+        ctx.close()
+        return 0
 
     n, eps, dt = args.n, 0.005, 5e-4
     log(f"[bench] generating Plummer N={n} seed={args.seed}")
@@ -179,8 +289,10 @@ def main() -> int:
         "phases_ms": {k: v[0] for k, v in timings.items()},
         "roofline": {"bound": "fp32-valu", "kernel": "direct_tiles", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None,
-                     "note": "20 flop per interaction x N(N-1) per launch / mean launch time"},
+                     "traffic": measured_traffic(n),
+                     "note": "20 flop per interaction x N(N-1) per launch / mean launch time; "
+                             "traffic = calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch "
+                             "(profiles/direct_tiles_traffic.json)"},
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline")
@@ -188,6 +300,11 @@ def main() -> int:
     else:
         result["cpu_baseline"] = None
     sys_.close()
+    if not args.no_barnes_hut:
+        result["barnes_hut"] = bench_barnes_hut(E, oracle, ctx, args.bh_n, args.bh_evals,
+                                                args.error_targets, not args.no_cpu_baseline)
+    log("[bench] ceilings")
+    result["ceilings"] = measure_ceilings(ctx)
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0

@@ -172,6 +172,20 @@ ORR_API int orr_tree_nodes(orr_ctx* ctx, double* mass, double* com_x, double* co
                    double* acceptance_radius_squared, uint64_t* next,
                    uint64_t* first_particle, uint64_t* particle_count);
 
+/* ---- machine-ceiling probes ----------------------------------------------- */
+
+/* Replaces the reference's CUDA probes (benchmarks/cuda_probes.cu:15-117:
+ * fma_probe, div_sqrt_probe, read_probe). kind: 0 FP32 FMA chains (size =
+ * iterations; rate in flop/s), 1 v_rsq_f32 chains (iterations; rsq/s),
+ * 2 HBM read (size = bytes; bytes/s), 3 HBM copy (bytes copied; bytes read +
+ * written per second). */
+#define ORR_PROBE_FMA 0
+#define ORR_PROBE_RSQ 1
+#define ORR_PROBE_READ 2
+#define ORR_PROBE_COPY 3
+ORR_API int orr_probe(orr_ctx* ctx, int32_t kind, uint64_t size, double* rate_out,
+                      double* milliseconds_out);
+
 /* ---- phase timings (hipEvents on the context stream) -------------------- */
 
 /* Phases: 0 direct prepare, 1 direct kernel, 2 direct reduce, 3 bbox, 4 encode,

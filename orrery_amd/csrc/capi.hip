@@ -324,6 +324,19 @@ int orr_tree_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double* c
     });
 }
 
+int orr_probe(orr_ctx* ctx, int32_t kind, uint64_t size, double* rate, double* ms) {
+    return guarded([&] {
+        require(ctx != nullptr && rate != nullptr && ms != nullptr, "null argument");
+        require(kind >= 0 && kind <= 3, "probe kind must be 0..3");
+        require(size > 0, "probe size must be > 0");
+        require(kind >= 2 || size <= (uint64_t{1} << 24), "probe iterations must be <= 2^24");
+        require(kind < 2 || size <= (uint64_t{1} << 34), "probe buffer must be <= 16 GiB");
+        use_device(ctx);
+        ctx->sync();
+        orr::probe(ctx, kind, size, rate, ms);
+    });
+}
+
 int orr_ctx_set_timing(orr_ctx* ctx, int32_t enabled) {
     return guarded([&] {
         require(ctx != nullptr, "null context");

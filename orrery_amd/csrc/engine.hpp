@@ -141,6 +141,9 @@ void tree_copy_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double*
                      double* acc, std::uint64_t* next, std::uint64_t* first,
                      std::uint64_t* count);
 
+// probes.hip: machine-ceiling probes; kind 0 fma, 1 rsq, 2 read, 3 copy.
+void probe(orr_ctx* ctx, int kind, std::uint64_t size, double* rate, double* ms);
+
 // integrate.hip: FP64 kick / drift / diagnostics on device-resident state.
 void kick(orr_ctx* ctx, std::uint64_t n, double* vx, double* vy, double* vz, const double* ax,
           const double* ay, const double* az, double scale);

@@ -125,6 +125,16 @@ class Context:
         C.check(self._lib.orr_ctx_get_direct_config(self.handle, ctypes.byref(c)))
         return dict(targets_per_lane=c.targets_per_lane, refine=c.refine, block_target=c.block_target)
 
+    # -- machine ceilings (benchmarks/cuda_probes.cu equivalents) -----------
+    PROBES = {"fma": 0, "rsq": 1, "read": 2, "copy": 3}
+
+    def probe(self, kind: str, size: int) -> tuple[float, float]:
+        """(rate, milliseconds): flop/s (fma), rsq/s (rsq) or bytes/s (read, copy)."""
+        rate, ms = ctypes.c_double(), ctypes.c_double()
+        C.check(self._lib.orr_probe(self.handle, self.PROBES[kind], int(size), ctypes.byref(rate),
+                                    ctypes.byref(ms)))
+        return float(rate.value), float(ms.value)
+
     # -- timings -----------------------------------------------------------
     def set_timing(self, enabled: bool) -> None:
         C.check(self._lib.orr_ctx_set_timing(self.handle, 1 if enabled else 0))
