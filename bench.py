@@ -122,25 +122,31 @@ def bench_barnes_hut(E, oracle, ctx, n: int, evals: int, error_targets: int, cpu
     p = oracle.disc_collision(n, 1)
     d = [E.DeviceArray.from_host(ctx, a) for a in (p.x, p.y, p.z, p.m)]
     out = [E.DeviceArray(ctx, n) for _ in range(3)]
-    s = E.HipTreeSolver(E.TreeParameters(theta, cap, quadrupole=False), E.Softening(eps), context=ctx)
-    s.evaluate_device(*(a.ptr for a in d), *(a.ptr for a in out), n)  # warm-up
-    s.reset_interaction_count()
-    ctx.set_timing(True)
-    ctx.reset_timings()
-    t0 = time.perf_counter()
-    for _ in range(evals):
-        s.evaluate_device(*(a.ptr for a in d), *(a.ptr for a in out), n)
-    wall = (time.perf_counter() - t0) / evals
-    phases = {k: v[0] / evals for k, v in ctx.timings().items()}
-    ctx.set_timing(False)
-    c = s.interaction_count()
-    tree = s.tree_summary()
-    acc = [a.to_host() for a in out]
     rng = np.random.default_rng(2)
     tgt = np.sort(rng.choice(n, size=min(n, error_targets), replace=False)).astype(np.uint64)
     ref = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps, targets=tgt, threads=cpu_threads())
     i = tgt.astype(np.int64)
-    worst, rms = oracle.error_summary(oracle.relative_errors(acc[0][i], acc[1][i], acc[2][i], *ref))
+
+    def run(quadrupole: bool):
+        s = E.HipTreeSolver(E.TreeParameters(theta, cap, quadrupole=quadrupole), E.Softening(eps), context=ctx)
+        s.evaluate_device(*(a.ptr for a in d), *(a.ptr for a in out), n)  # warm-up
+        s.reset_interaction_count()
+        ctx.set_timing(True)
+        ctx.reset_timings()
+        t0 = time.perf_counter()
+        for _ in range(evals):
+            s.evaluate_device(*(a.ptr for a in d), *(a.ptr for a in out), n)
+        wall = (time.perf_counter() - t0) / evals
+        phases = {k: v[0] / evals for k, v in ctx.timings().items()}
+        ctx.set_timing(False)
+        acc = [a.to_host() for a in out]
+        err = oracle.error_summary(oracle.relative_errors(acc[0][i], acc[1][i], acc[2][i], *ref))
+        return s, wall, phases, err
+
+    mono_s, mono_wall, mono_phases, (mono_worst, mono_rms) = run(False)
+    s, wall, phases, (worst, rms) = run(True)
+    c = s.interaction_count()
+    tree = s.tree_summary()
     M = tree["node_count"]
     build_phases = ("bbox", "encode", "sort", "gather", "build", "moments")
     build_ms = sum(phases.get(k, 0.0) for k in build_phases)
@@ -151,8 +157,10 @@ def bench_barnes_hut(E, oracle, ctx, n: int, evals: int, error_targets: int, cpu
     pp, pc = c.particle_particle // evals, c.particle_cell // evals
     out_d = {
         "workload": "Barnes-Hut, two exponential discs + Hernquist bulges, parabolic orbit",
-        "n": n, "opening_angle": theta, "leaf_capacity": cap, "softening": eps,
+        "n": n, "opening_angle": theta, "leaf_capacity": cap, "softening": eps, "quadrupole": True,
         "ms_per_evaluation": wall * 1e3,
+        "monopole": {"ms_per_evaluation": mono_wall * 1e3, "phases_ms": mono_phases,
+                     "force_error": {"rms_relative": mono_rms, "max_relative": mono_worst}},
         "phases_ms": phases,
         "interactions_per_evaluation": {"particle_particle": pp, "particle_cell": pc},
         "walk_interactions_per_s": (pp + pc) / (phases.get("walk", 0.0) * 1e-3) if phases.get("walk") else None,

@@ -172,6 +172,11 @@ ORR_API int orr_tree_nodes(orr_ctx* ctx, double* mass, double* com_x, double* co
                    double* acceptance_radius_squared, uint64_t* next,
                    uint64_t* first_particle, uint64_t* particle_count);
 
+/* Quadrupole moments of the last orr_tree_eval run with quadrupole = 1, six
+ * per node {xx, xy, xz, yy, yz, zz} (Quadrupole, octree.hpp:89-96), copied to
+ * a host array of 6 * node_count doubles. */
+ORR_API int orr_tree_quadrupoles(orr_ctx* ctx, double* quadrupoles);
+
 /* ---- machine-ceiling probes ----------------------------------------------- */
 
 /* Replaces the reference's CUDA probes (benchmarks/cuda_probes.cu:15-117:

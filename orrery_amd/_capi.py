@@ -91,6 +91,7 @@ SIGNATURES = {
     "orr_morton_order": [_vp, _vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _d],
     "orr_tree_nodes": [_vp] + [_vp] * 8,
     "orr_probe": [_vp, ctypes.c_int32, ctypes.c_uint64, _d, _d],
+    "orr_tree_quadrupoles": [_vp, _d],
     "orr_ctx_set_timing": [_vp, ctypes.c_int32],
     "orr_ctx_timings": [_vp, ctypes.POINTER(OrrTimings)],
     "orr_ctx_reset_timings": [_vp],

@@ -324,6 +324,14 @@ int orr_tree_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double* c
     });
 }
 
+int orr_tree_quadrupoles(orr_ctx* ctx, double* out) {
+    return guarded([&] {
+        require(ctx != nullptr && out != nullptr, "null argument");
+        use_device(ctx);
+        orr::tree_copy_quadrupoles(ctx, out);
+    });
+}
+
 int orr_probe(orr_ctx* ctx, int32_t kind, uint64_t size, double* rate, double* ms) {
     return guarded([&] {
         require(ctx != nullptr && rate != nullptr && ms != nullptr, "null argument");

@@ -82,6 +82,7 @@ struct orr_ctx {
     std::uint64_t tree_last_n = 0;
     std::size_t tree_last_bound = 0;
     double tree_last_theta = 0.5;
+    bool tree_last_quadrupole = false;
 
     // Phase timing with hipEvents recorded on `stream`.
     bool timing = false;
@@ -137,6 +138,7 @@ void exclusive_scan_u32(orr_ctx* ctx, const std::uint32_t* in, std::uint32_t* ou
 
 // tree.hip: order, build, moments and walk; fills stats (syncs).
 void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats);
+void tree_copy_quadrupoles(orr_ctx* ctx, double* out);
 void tree_copy_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double* cz,
                      double* acc, std::uint64_t* next, std::uint64_t* first,
                      std::uint64_t* count);

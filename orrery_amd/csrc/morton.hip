@@ -10,7 +10,7 @@
 //   * a stable LSD radix sort starting from identity order reproduces the
 //     (code, index) lexicographic total order (morton.hpp:155-175).
 //
-// Sort: 8 passes of 8-bit digits. Each pass is histogram -> exclusive scan ->
+// Sort: 8 passes of 8-bit digits. Each pass is histogram -> device-wide exclusive scan ->
 // scatter. The scatter ranks keys stably inside 256-key tiles with wave64
 // ballots (8 ballots give each lane the mask of lanes sharing its digit) and an
 // LDS table of per-wave digit counts, so no LDS atomics decide an order.
@@ -179,31 +179,6 @@ __global__ __launch_bounds__(kBlock) void radix_histogram(const std::uint64_t* _
     hist[static_cast<std::uint64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = counts[threadIdx.x];
 }
 
-// Exclusive scan of `count` entries by one block of 1024 threads.
-__global__ __launch_bounds__(1024) void exclusive_scan(std::uint32_t* __restrict__ data,
-                                                       std::uint32_t count) {
-    __shared__ std::uint32_t sums[1024];
-    const std::uint32_t per = (count + 1023) / 1024;
-    const std::uint32_t b = threadIdx.x * per;
-    const std::uint32_t e = min(count, b + per);
-    std::uint32_t local = 0;
-    for (std::uint32_t i = b; i < e; ++i) local += data[i];
-    sums[threadIdx.x] = local;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-        const std::uint32_t v = threadIdx.x >= static_cast<unsigned>(off) ? sums[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sums[threadIdx.x] += v;
-        __syncthreads();
-    }
-    std::uint32_t run = sums[threadIdx.x] - local;
-    for (std::uint32_t i = b; i < e; ++i) {
-        const std::uint32_t v = data[i];
-        data[i] = run;
-        run += v;
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __restrict__ keys_in,
                                                         const std::uint32_t* __restrict__ vals_in,
                                                         std::uint64_t n, int shift,
@@ -265,7 +240,8 @@ void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double
     ctx->order_codes_alt.reserve(sizeof(std::uint64_t) * n);
     ctx->order_index.reserve(sizeof(std::uint32_t) * n);
     ctx->order_index_alt.reserve(sizeof(std::uint32_t) * n);
-    ctx->order_hist.reserve(sizeof(std::uint32_t) * 256 * sort_blocks);
+    ctx->order_hist.reserve(sizeof(std::uint32_t) *
+                            (256 * sort_blocks + scan_scratch_words(256 * sort_blocks)));
     double* partial = ctx->order_bbox.as<double>() + 4;
     double* cube = ctx->order_bbox.as<double>();
     {
@@ -294,9 +270,7 @@ void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double
             hipLaunchKernelGGL(radix_histogram, dim3(sort_blocks), dim3(kBlock), 0, ctx->stream,
                                keys[cur], n, shift, hist);
             ORR_LAUNCHED("radix_histogram");
-            hipLaunchKernelGGL(exclusive_scan, dim3(1), dim3(1024), 0, ctx->stream, hist,
-                               256u * sort_blocks);
-            ORR_LAUNCHED("exclusive_scan");
+            exclusive_scan_u32(ctx, hist, hist, 256u * sort_blocks, hist + 256u * sort_blocks, nullptr);
             hipLaunchKernelGGL(radix_scatter, dim3(sort_blocks), dim3(kBlock), 0, ctx->stream,
                                keys[cur], vals[cur], n, shift, hist, keys[1 - cur], vals[1 - cur]);
             ORR_LAUNCHED("radix_scatter");

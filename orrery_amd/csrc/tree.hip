@@ -59,6 +59,11 @@ struct WalkNode {
     uint4 b;   // acceptance radius^2 (FP32 bits), next, first particle, particle count
 };
 
+struct QuadF {
+    float4 a;  // xx, xy, xz, yy
+    float4 b;  // yz, zz, 0, 0
+};
+
 // Device counters, uint32: [0] nodes, [1] leaves, [2] depth.
 // Device statistics, uint64: [0] particle-particle, [1] particle-cell, [2] lane visits,
 // [3] wave steps.
@@ -180,42 +185,47 @@ __global__ __launch_bounds__(kBlock) void tree_nodes(
     std::uint32_t* __restrict__ node_next, std::uint8_t* __restrict__ node_level,
     double4* __restrict__ node_geo, std::uint32_t* __restrict__ leaf_of) {
     const std::uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n || chain[k] == 0) return;
-    const std::uint32_t M = counters[0];
-    const unsigned dk = d[k], lleaf = leaf_level[k];
-    const std::uint32_t base = first_node[k];
-    const std::uint64_t key = codes[k];
-    // Ends from the leaf upwards: each is a lower bound for the next search.
-    std::uint32_t ends[kLevels + 1];
-    std::uint32_t lo = k + 1;
-    for (int l = static_cast<int>(lleaf); l >= static_cast<int>(dk); --l) {
-        lo = prefix_end(codes, n, lo, key, static_cast<unsigned>(l));
-        ends[l] = lo;
+    const bool owns = k < n && chain[k] != 0;
+    unsigned depth = 0;
+    if (owns) {
+        const std::uint32_t M = counters[0];
+        const unsigned dk = d[k], lleaf = leaf_level[k];
+        const std::uint32_t base = first_node[k];
+        const std::uint64_t key = codes[k];
+        // From the leaf upwards: each cell's end bounds the next search from below.
+        std::uint32_t end = k + 1;
+        for (int l = static_cast<int>(lleaf); l >= static_cast<int>(dk); --l) {
+            end = prefix_end(codes, n, end, key, static_cast<unsigned>(l));
+            double size = cube[3];
+            double c[3] = {cube[0] + size / 2, cube[1] + size / 2, cube[2] + size / 2};
+            for (int s = 1; s <= l; ++s) {
+                const unsigned oct = static_cast<unsigned>((key >> (3u * (kLevels - s))) & 7u);
+                const double q = size / 4;
+                c[0] = c[0] + ((oct & 4u) != 0 ? q : -q);
+                c[1] = c[1] + ((oct & 2u) != 0 ? q : -q);
+                c[2] = c[2] + ((oct & 1u) != 0 ? q : -q);
+                size = size / 2;
+            }
+            const std::uint32_t node = base + (static_cast<unsigned>(l) - dk);
+            node_begin[node] = k;
+            node_count[node] = end - k;
+            node_level[node] = static_cast<std::uint8_t>(l);
+            node_next[node] = end >= n ? M : first_node[end];
+            node_geo[node] = make_double4(c[0], c[1], c[2], size);
+            if (static_cast<unsigned>(l) == lleaf) {
+                for (std::uint32_t j = k; j < end; ++j) leaf_of[j] = node;
+            }
+        }
+        depth = lleaf;
     }
-    double size = cube[3];
-    double c[3] = {cube[0] + size / 2, cube[1] + size / 2, cube[2] + size / 2};
-    for (unsigned l = 0; l <= lleaf; ++l) {
-        if (l > 0) {
-            const unsigned oct = static_cast<unsigned>((key >> (3u * (kLevels - l))) & 7u);
-            const double q = size / 4;
-            c[0] = c[0] + ((oct & 4u) != 0 ? q : -q);
-            c[1] = c[1] + ((oct & 2u) != 0 ? q : -q);
-            c[2] = c[2] + ((oct & 1u) != 0 ? q : -q);
-            size = size / 2;
-        }
-        if (l < dk) continue;
-        const std::uint32_t node = base + (l - dk);
-        const std::uint32_t end = ends[l];
-        node_begin[node] = k;
-        node_count[node] = end - k;
-        node_level[node] = static_cast<std::uint8_t>(l);
-        node_next[node] = end >= n ? M : first_node[end];
-        node_geo[node] = make_double4(c[0], c[1], c[2], size);
-        if (l == lleaf) {
-            for (std::uint32_t j = k; j < end; ++j) leaf_of[j] = node;
-            atomicAdd(&counters[1], 1u);
-            atomicMax(&counters[2], l);
-        }
+    // One leaf per chain: count them and their deepest level once per wave.
+    const unsigned long long leaves = __ballot(owns);
+    unsigned deepest = depth;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) deepest = max(deepest, static_cast<unsigned>(__shfl_xor(static_cast<int>(deepest), off, 64)));
+    if ((threadIdx.x & 63) == 0 && leaves != 0) {
+        atomicAdd(&counters[1], static_cast<std::uint32_t>(__popcll(leaves)));
+        atomicMax(&counters[2], deepest);
     }
     (void)cap;
 }
@@ -269,13 +279,36 @@ __device__ inline void finish_node(std::uint32_t node, double mass, double cx, d
     walk[node] = w;
 }
 
-// octree.cpp:91-135 (leaf): index-order sums, fallback to the first particle.
+// octree.cpp:75-84: traceless second moment of a point mass displaced by d.
+__device__ inline void add_displaced(double q[6], double mass, double dx, double dy, double dz) {
+    const double trace = mass * (((dx * dx) + (dy * dy)) + (dz * dz));
+    q[0] += (3 * mass * dx * dx) - trace;
+    q[1] += 3 * mass * dx * dy;
+    q[2] += 3 * mass * dx * dz;
+    q[3] += (3 * mass * dy * dy) - trace;
+    q[4] += 3 * mass * dy * dz;
+    q[5] += (3 * mass * dz * dz) - trace;
+}
+
+__device__ inline void store_quad(std::uint32_t node, const double q[6], double* __restrict__ quad64,
+                                  QuadF* __restrict__ quadf) {
+    for (int k = 0; k < 6; ++k) quad64[6 * static_cast<std::size_t>(node) + k] = q[k];
+    QuadF f;
+    f.a = make_float4(static_cast<float>(q[0]), static_cast<float>(q[1]), static_cast<float>(q[2]),
+                      static_cast<float>(q[3]));
+    f.b = make_float4(static_cast<float>(q[4]), static_cast<float>(q[5]), 0.f, 0.f);
+    quadf[node] = f;
+}
+
+// octree.cpp:91-135 (leaf): index-order sums, fallback to the first particle;
+// with quadrupoles, the displaced moments of the particles about the centre of mass.
 __global__ __launch_bounds__(kBlock) void leaf_moments(
     const std::uint32_t* __restrict__ counters, const std::uint8_t* __restrict__ node_level,
     const std::uint8_t* __restrict__ leaf_level_of_first, const std::uint32_t* __restrict__ node_begin,
     const std::uint32_t* __restrict__ node_count, const std::uint32_t* __restrict__ node_next,
     const double4* __restrict__ node_geo, const double4* __restrict__ sorted, double theta,
-    const double* __restrict__ cube, double4* __restrict__ moment, WalkNode* __restrict__ walk) {
+    const double* __restrict__ cube, double4* __restrict__ moment, WalkNode* __restrict__ walk,
+    double* __restrict__ quad64, QuadF* __restrict__ quadf) {
     const std::uint32_t M = counters[0];
     for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
         const std::uint32_t b = node_begin[node];
@@ -301,16 +334,26 @@ __global__ __launch_bounds__(kBlock) void leaf_moments(
         }
         finish_node(node, mass, cx, cy, cz, node_geo[node], theta, cube, true, b, e - b,
                     node_next[node], moment, walk);
+        if (quad64 != nullptr) {
+            double q[6] = {0, 0, 0, 0, 0, 0};
+            for (std::uint32_t i = b; i < e; ++i) {
+                const double4 p = sorted[i];
+                add_displaced(q, p.w, p.x - cx, p.y - cy, p.z - cz);
+            }
+            store_quad(node, q, quad64, quadf);
+        }
     }
 }
 
-// octree.cpp:142-185 (internal): children in DFS order via the next chain.
+// octree.cpp:142-185 (internal): children in DFS order via the next chain;
+// quadrupoles by the parallel-axis theorem, children's moments first.
 __global__ __launch_bounds__(kBlock) void internal_moments(
     unsigned level, const std::uint32_t* __restrict__ counters,
     const std::uint8_t* __restrict__ node_level, const std::uint8_t* __restrict__ leaf_level_of_first,
     const std::uint32_t* __restrict__ node_begin, const std::uint32_t* __restrict__ node_next,
     const double4* __restrict__ node_geo, double theta, const double* __restrict__ cube,
-    double4* __restrict__ moment, WalkNode* __restrict__ walk) {
+    double4* __restrict__ moment, WalkNode* __restrict__ walk, double* __restrict__ quad64,
+    QuadF* __restrict__ quadf) {
     const std::uint32_t M = counters[0];
     for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
         if (node_level[node] != level) continue;
@@ -343,6 +386,16 @@ __global__ __launch_bounds__(kBlock) void internal_moments(
         }
         finish_node(node, mass, cx, cy, cz, node_geo[node], theta, cube, false, b, 0,
                     node_next[node], moment, walk);
+        if (quad64 != nullptr) {
+            double q[6] = {0, 0, 0, 0, 0, 0};
+            for (std::uint32_t c = node + 1; c < end; c = max(node_next[c], c + 1)) {
+                const double* cq = quad64 + 6 * static_cast<std::size_t>(c);
+                for (int k = 0; k < 6; ++k) q[k] += cq[k];
+                const double4 cm = moment[c];
+                add_displaced(q, cm.w, cm.x - cx, cm.y - cy, cm.z - cz);
+            }
+            store_quad(node, q, quad64, quadf);
+        }
     }
 }
 
@@ -365,9 +418,10 @@ __device__ __forceinline__ void kahan(float p, float& s, float& c) {
     s = t;
 }
 
-template<bool kSafe>
+template<bool kSafe, bool kQuad>
 __global__ __launch_bounds__(kBlock) void tree_walk(
-    const WalkNode* __restrict__ nodes, const double4* __restrict__ node_geo,
+    const WalkNode* __restrict__ nodes, const QuadF* __restrict__ quads,
+    const double4* __restrict__ node_geo,
     const float4* __restrict__ root_rel, const float4* __restrict__ leaf_rel,
     const std::uint32_t* __restrict__ leaf_of, const std::uint32_t* __restrict__ index,
     const std::uint32_t* __restrict__ counters, std::uint32_t n, float eps2, double G,
@@ -408,6 +462,21 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
                 px = __builtin_fmaf(f, dx, px);
                 py = __builtin_fmaf(f, dy, py);
                 pz = __builtin_fmaf(f, dz, pz);
+                if constexpr (kQuad) {
+                    // tree_walk.hpp:156-181: -Q d inv^5 + 2.5 (d.Q d) d inv^7.
+                    const QuadF Q = quads[cur];
+                    const float i2 = inv * inv;
+                    const float i5 = (i2 * i2) * inv;
+                    const float i7 = i5 * i2;
+                    const float qx = __builtin_fmaf(Q.a.z, dz, __builtin_fmaf(Q.a.y, dy, Q.a.x * dx));
+                    const float qy = __builtin_fmaf(Q.b.x, dz, __builtin_fmaf(Q.a.w, dy, Q.a.y * dx));
+                    const float qz = __builtin_fmaf(Q.b.y, dz, __builtin_fmaf(Q.b.x, dy, Q.a.z * dx));
+                    const float quadratic = __builtin_fmaf(dz, qz, __builtin_fmaf(dy, qy, dx * qx));
+                    const float s = (2.5f * quadratic) * i7;
+                    px = __builtin_fmaf(dx, s, __builtin_fmaf(qx, -i5, px));
+                    py = __builtin_fmaf(dy, s, __builtin_fmaf(qy, -i5, py));
+                    pz = __builtin_fmaf(dz, s, __builtin_fmaf(qz, -i5, pz));
+                }
                 ++pc;
                 mine = next;
             } else if (nd.b.w > 0) {
@@ -493,6 +562,8 @@ struct Layout {
     double4* geo;
     double4* moment;
     WalkNode* walk;
+    double* quad64;  // 6 per node, null without quadrupoles
+    QuadF* quadf;
     // Counters.
     std::uint32_t* c32;
     unsigned long long* c64;
@@ -506,12 +577,13 @@ std::size_t node_bound(std::uint64_t n, std::uint64_t cap) {
     return static_cast<std::size_t>(std::min(by_levels, by_chains));
 }
 
-Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool allocate) {
-    // Every carved array is rounded up to 256 bytes: 19 arrays of padding.
+Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole, bool allocate) {
+    // Every carved array is rounded up to 256 bytes: 21 arrays of padding.
     const std::size_t per_particle = 2 * n + 3 * 4 * n + 32 * n + 2 * 16 * n +
                                      4 * scan_scratch_words(n);
-    const std::size_t per_node = nodes * (4 * 3 + 1 + 32 + 32 + sizeof(WalkNode));
-    const std::size_t bytes = per_particle + per_node + 19 * 256 + 64;
+    const std::size_t per_quad = quadrupole ? 6 * sizeof(double) + sizeof(QuadF) : 0;
+    const std::size_t per_node = nodes * (4 * 3 + 1 + 32 + 32 + sizeof(WalkNode) + per_quad);
+    const std::size_t bytes = per_particle + per_node + 21 * 256 + 64;
     if (allocate) ctx->tree_work.reserve(bytes);
     char* p = static_cast<char*>(ctx->tree_work.ptr);
     Layout L{};
@@ -531,6 +603,10 @@ Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool allocate) {
     L.geo = carve<double4>(p, nodes);
     L.moment = carve<double4>(p, nodes);
     L.walk = carve<WalkNode>(p, nodes);
+    if (quadrupole) {
+        L.quad64 = carve<double>(p, 6 * nodes);
+        L.quadf = carve<QuadF>(p, nodes);
+    }
     L.c32 = carve<std::uint32_t>(p, 4);
     L.c64 = carve<unsigned long long>(p, 4);
     return L;
@@ -552,7 +628,8 @@ void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) 
     const std::size_t bound = node_bound(n, cap);
     ctx->tree_last_bound = bound;
     ctx->tree_last_theta = theta;
-    const Layout L = layout(ctx, n, bound, true);
+    ctx->tree_last_quadrupole = a.quadrupole != 0;
+    const Layout L = layout(ctx, n, bound, a.quadrupole != 0, true);
     const std::uint32_t n32 = static_cast<std::uint32_t>(n);
     const std::uint32_t blocks = (n32 + kBlock - 1) / kBlock;
     const std::uint32_t node_blocks = static_cast<std::uint32_t>(std::min<std::size_t>(4096, (bound + kBlock - 1) / kBlock));
@@ -587,27 +664,25 @@ void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) 
         PhaseTimer t(ctx, kMoments);
         hipLaunchKernelGGL(leaf_moments, dim3(node_blocks), dim3(kBlock), 0, ctx->stream, L.c32,
                            L.level, L.leaf_level, L.begin, L.count, L.next, L.geo, L.sorted, theta,
-                           cube, L.moment, L.walk);
+                           cube, L.moment, L.walk, L.quad64, L.quadf);
         ORR_LAUNCHED("leaf_moments");
         for (int level = static_cast<int>(kLevels) - 1; level >= 0; --level) {
             hipLaunchKernelGGL(internal_moments, dim3(node_blocks), dim3(kBlock), 0, ctx->stream,
                                static_cast<unsigned>(level), L.c32, L.level, L.leaf_level, L.begin,
-                               L.next, L.geo, theta, cube, L.moment, L.walk);
+                               L.next, L.geo, theta, cube, L.moment, L.walk, L.quad64, L.quadf);
             ORR_LAUNCHED("internal_moments");
         }
     }
     {
         PhaseTimer t(ctx, kWalk);
         const float eps2 = static_cast<float>(a.softening_squared);
-        if (a.softening_squared > 0) {
-            hipLaunchKernelGGL(tree_walk<false>, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.walk,
-                               L.geo, L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, n32, eps2,
-                               a.gravitational_constant, a.ax, a.ay, a.az, L.c64);
-        } else {
-            hipLaunchKernelGGL(tree_walk<true>, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.walk,
-                               L.geo, L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, n32, eps2,
-                               a.gravitational_constant, a.ax, a.ay, a.az, L.c64);
-        }
+        const bool safe = !(a.softening_squared > 0);
+        const bool quad = a.quadrupole != 0;
+        auto walk = safe ? (quad ? tree_walk<true, true> : tree_walk<true, false>)
+                         : (quad ? tree_walk<false, true> : tree_walk<false, false>);
+        hipLaunchKernelGGL(walk, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.walk, L.quadf, L.geo,
+                           L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, n32, eps2,
+                           a.gravitational_constant, a.ax, a.ay, a.az, L.c64);
         ORR_LAUNCHED("tree_walk");
     }
     if (stats != nullptr) {
@@ -626,11 +701,24 @@ void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) 
     }
 }
 
+void tree_copy_quadrupoles(orr_ctx* ctx, double* out) {
+    const std::uint64_t n = ctx->tree_last_n;
+    if (n == 0 || ctx->tree_work.ptr == nullptr) return;
+    if (!ctx->tree_last_quadrupole) fail(ORR_ERR_INVALID_ARGUMENT, "the last tree had no quadrupoles");
+    const Layout L = layout(ctx, n, ctx->tree_last_bound, true, false);
+    std::uint32_t M = 0;
+    ORR_HIP(hipMemcpyAsync(&M, L.c32, sizeof(M), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    if (M == 0) return;
+    ORR_HIP(hipMemcpyAsync(out, L.quad64, 6 * sizeof(double) * M, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+}
+
 void tree_copy_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double* cz, double* acc,
                      std::uint64_t* next, std::uint64_t* first, std::uint64_t* count) {
     const std::uint64_t n = ctx->tree_last_n;
     if (n == 0 || ctx->tree_work.ptr == nullptr) return;
-    const Layout L = layout(ctx, n, ctx->tree_last_bound, false);
+    const Layout L = layout(ctx, n, ctx->tree_last_bound, ctx->tree_last_quadrupole, false);
     std::uint32_t M = 0;
     ORR_HIP(hipMemcpyAsync(&M, L.c32, sizeof(M), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();

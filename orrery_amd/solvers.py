@@ -398,6 +398,17 @@ class HipTreeSolver:
                     first=first, count=count)
 
 
+def _tree_quadrupoles(self) -> np.ndarray:
+    """Quadrupole moments of the last evaluation, shape (node_count, 6)."""
+    M = int(self._stats.node_count)
+    out = np.zeros((M, 6))
+    C.check(C.load().orr_tree_quadrupoles(self.ctx.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+HipTreeSolver.quadrupoles = _tree_quadrupoles
+
+
 def morton_order(ctx: Context, x: np.ndarray, y: np.ndarray, z: np.ndarray):
     """Device Morton ordering of host positions: (codes, index, origin, size)."""
     n = len(x)

@@ -55,6 +55,36 @@ def test_structure_and_moments_bitwise(ctx, gen, n, theta, cap):
     assert np.array_equal(got["acc_r2"], t.acc_r2)
 
 
+@pytest.mark.parametrize("gen,n,cap", [("plummer", 5000, 16), ("disc", 20000, 16), ("plummer", 3000, 1)])
+def test_quadrupole_moments_bitwise(ctx, gen, n, cap):
+    # octree.cpp:75-84, 120-135, 165-184: leaf displaced moments, parallel-axis gather.
+    p = oracle.plummer(n, 20260811) if gen == "plummer" else oracle.disc_collision(n, 4)
+    s = E.HipTreeSolver(E.TreeParameters(0.5, cap, quadrupole=True), E.Softening(0.05), context=ctx)
+    acc = [np.zeros(p.n) for _ in range(3)]
+    s.evaluate((p.x, p.y, p.z), p.m, acc)
+    codes, index, origin, size = oracle.morton_order(p.x, p.y, p.z)
+    i = index.astype(np.int64)
+    t = oracle.tree_build(p.x[i], p.y[i], p.z[i], p.m[i], codes, origin, size, 0.5, cap, quadrupole=True)
+    assert np.array_equal(s.quadrupoles(), t.quad)
+
+
+def test_quadrupole_accuracy_reference_setup(ctx):
+    # docs/performance/tree_accuracy.csv setup: Plummer standard units, N = 16384,
+    # eps = 0.05, leaf 32; the reference's quadrupole walk: RMS 9.76e-4 at theta 0.5.
+    p = oracle.plummer(16384, 20260810, scale_radius=3 * np.pi / 16, cutoff=0.999)
+    eps = 0.05
+    ref = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps)
+    out = {}
+    for quad in (False, True):
+        s = E.HipTreeSolver(E.TreeParameters(0.5, 32, quadrupole=quad), E.Softening(eps), context=ctx)
+        acc = [np.zeros(p.n) for _ in range(3)]
+        s.evaluate((p.x, p.y, p.z), p.m, acc)
+        out[quad] = oracle.error_summary(oracle.relative_errors(*acc, *ref))
+        print(f"quadrupole={quad}: worst={out[quad][0]:.3e} rms={out[quad][1]:.3e}")
+    assert out[True][1] < 1e-3 and out[True][1] < 0.7 * out[False][1]
+    assert out[False][1] < 2.5e-3
+
+
 def test_identical_points_stop_at_depth_21(ctx):
     n = 40
     x = np.full(n, 0.25)
