@@ -42,6 +42,36 @@ def _compile(src: str) -> str:
     return obj
 
 
+ROOT = os.path.dirname(HERE)
+HOST_LIBRARY = os.path.join(HERE, "liborrery_host.so")
+HOST_TEST = os.path.join(ROOT, "tests", "cpp", "test_host_solvers")
+CXX = os.environ.get("CXX", "g++")
+HOST_INCLUDES = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host", "include"),
+                 "-I", os.path.join(HERE, "host", "compat")]
+
+
+def _run(cmd: list[str], what: str) -> None:
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{' '.join(cmd)}\n{res.stderr}")
+
+
+def build_host(verbose: bool = False) -> None:
+    """C++20 ForceSolver hosts over the C ABI, and their C++ test driver."""
+    tmp = HOST_LIBRARY + ".tmp"
+    _run([CXX, "-std=c++20", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", *HOST_INCLUDES,
+          os.path.join(HERE, "host", "hip_solvers.cpp"), "-L", HERE, "-lorrery_hip",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp], "host library")
+    os.replace(tmp, HOST_LIBRARY)
+    tmp = HOST_TEST + ".tmp"
+    _run([CXX, "-std=c++20", "-O2", "-Wall", *HOST_INCLUDES,
+          os.path.join(ROOT, "tests", "cpp", "test_host_solvers.cpp"), "-L", HERE, "-lorrery_host",
+          "-lorrery_hip", "-Wl,-rpath,$ORIGIN/../../orrery_amd", "-o", tmp], "host test driver")
+    os.replace(tmp, HOST_TEST)
+    if verbose:
+        print(f"built {HOST_LIBRARY} and {HOST_TEST}")
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as pool:
@@ -54,6 +84,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIBRARY)
     if verbose:
         print(f"built {LIBRARY}")
+    build_host(verbose)
     return LIBRARY
 
 

@@ -1,0 +1,42 @@
+"""The C++20 ForceSolver hosts (orrery_amd/host) over the C ABI.
+
+* CPU: the hosts compile against the reference's own interface headers (the
+  drop-in claim), and without a device try_create returns null
+  (cuda_direct_solver_test.cpp:132-136).
+* GPU: tests/cpp/test_host_solvers runs the reference's closed-form and
+  agreement cases through HipDirectSolver / HipTreeSolver.
+"""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BINARY = os.path.join(ROOT, "tests", "cpp", "test_host_solvers")
+REFERENCE_INCLUDE = "/root/reference/include"
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_INCLUDE), reason="reference checkout absent")
+def test_hosts_compile_against_reference_headers():
+    cmd = ["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-Werror", "-I", REFERENCE_INCLUDE,
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "orrery_amd", "host", "include"),
+           os.path.join(ROOT, "orrery_amd", "host", "hip_solvers.cpp")]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(BINARY), reason="host test driver not built")
+def test_try_create_is_null_without_device():
+    import orrery_amd as E
+    if E.device_count() != 0:
+        pytest.skip("a device is present")
+    res = subprocess.run([BINARY, "--expect-no-device"], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0, res.stdout + res.stderr
+
+
+@pytest.mark.gpu
+def test_host_solvers_on_device():
+    assert os.path.exists(BINARY), "build first: python -m orrery_amd.build"
+    res = subprocess.run([BINARY], capture_output=True, text=True, timeout=120)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr
