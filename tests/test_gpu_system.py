@@ -71,6 +71,38 @@ def test_agreement_fixture_energy(ctx):
     s.close()
 
 
+def test_config1_plummer_4096_energy_drift(ctx):
+    # Config 1: Plummer N = 4096, M = 1, a = 1, r < 10 a, eps = 0.01, velocity
+    # Verlet dt = 1e-3 x 100 steps, seed 1; oracle: energy drift and FP64 forces.
+    p = oracle.plummer(4096, 1)
+    eps, dt, steps = 0.01, 1e-3, 100
+    f = oracle.make_field("vv", "direct", eps2=eps * eps)
+    ref = oracle.Particles(*(a.copy() for a in p.arrays()))
+    e0 = oracle.diagnostics(ref, eps * eps)["total"]
+    acc = oracle.refresh(ref, f)
+    f64 = [a.copy() for a in acc]
+    oracle.steps(ref, acc, f, dt, steps)
+    e_cpu = oracle.diagnostics(ref, eps * eps)["total"]
+
+    s = E.DeviceSystem(ctx, p.n)
+    s.upload(*p.arrays())
+    cfg = E.StepConfig(integrator="velocity-verlet", solver="direct", softening=eps)
+    s.refresh(cfg)
+    a0 = s.download()
+    worst, rms = oracle.error_summary(oracle.relative_errors(a0["ax"], a0["ay"], a0["az"], *f64))
+    assert rms < 1e-6 and worst < 1e-5, (rms, worst)
+    d0 = s.diagnostics(eps)
+    assert d0["total"] == pytest.approx(e0, rel=1e-14)
+    s.step(cfg, dt, steps)
+    e_gpu = s.diagnostics(eps)["total"]
+    drift_cpu = (e_cpu - e0) / abs(e0)
+    drift_gpu = (e_gpu - e0) / abs(e0)
+    print(f"config 1: drift FP64 CPU {drift_cpu:.4e}, device {drift_gpu:.4e}")
+    # Same integrator, forces to ~1e-7: the drifts agree to a small fraction.
+    assert abs(drift_gpu - drift_cpu) <= 0.05 * abs(drift_cpu) + 1e-9
+    s.close()
+
+
 def test_device_diagnostics_match_oracle(ctx):
     p = oracle.plummer(3000, 9)
     s = E.DeviceSystem(ctx, p.n)
