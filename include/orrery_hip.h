@@ -239,6 +239,74 @@ ORR_API int orr_system_step(orr_system* sys, const orr_step_config* config, doub
 ORR_API int orr_system_diagnostics(orr_system* sys, double softening_squared,
                            double gravitational_constant, double out[8]);
 
+/* ---- batches: many independent systems of uneven size, stepped together --
+ *
+ * No counterpart in the reference, which steps one Simulation at a time
+ * (sim/simulation.cpp:26-64 over integrators/velocity_verlet.cpp:59-72). A batch
+ * is the device-resident system above for many systems at once: their FP64
+ * state lies end to end in HBM, and one step of the whole batch is one
+ * sequence of launches whatever the number of systems -- the direct systems
+ * as one persistent work-queue kernel, the Barnes-Hut systems as one
+ * segmented Morton sort, octree build and walk -- replayed as a hipGraph.
+ * A system's result does not depend on the other systems in the batch.
+ * One integrator per batch; softening, G and the timestep per system. */
+typedef struct orr_batch orr_batch;
+
+typedef struct {
+    uint64_t count;                  /* particles (0 allowed: the system is inert) */
+    int32_t solver;                  /* 0 direct, 1 Barnes-Hut, -1 by size (direct_below) */
+    int32_t reserved;
+    double softening_squared;        /* Barnes-Hut systems of one batch share it and G */
+    double gravitational_constant;
+    double timestep;
+} orr_batch_system;
+
+typedef struct {
+    int32_t integrator;              /* 0 velocity Verlet, 1 Yoshida-4 */
+    int32_t quadrupole;              /* Barnes-Hut systems */
+    double opening_angle;
+    uint32_t leaf_capacity;          /* 0: chosen by orr_batch_autotune (else 16) */
+    uint32_t tiles_per_item;         /* direct work-item length in 64-source tiles; 0: tuned */
+    uint32_t direct_below;           /* solver -1: direct when count < direct_below */
+    int32_t use_graph;               /* capture one step as a hipGraph and replay it */
+} orr_batch_config;
+
+typedef struct {
+    uint32_t systems, direct_systems, tree_systems;
+    uint32_t tiles_per_item, leaf_capacity;
+    uint32_t direct_tiles, direct_items;
+    int32_t graph;                   /* a captured step is in use */
+    uint64_t particles, direct_particles, tree_particles;
+    uint64_t direct_pairs;           /* sum of N(N-1) over direct systems, per evaluation */
+} orr_batch_info;
+
+ORR_API int orr_batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t count,
+                             const orr_batch_config* config, orr_batch** out);
+ORR_API int orr_batch_destroy(orr_batch* batch);
+ORR_API int orr_batch_info_get(orr_batch* batch, orr_batch_info* out);
+/* Host arrays of the system's `count` FP64 values each (null: left as is). */
+ORR_API int orr_batch_upload(orr_batch* batch, uint32_t system, const double* x, const double* y,
+                             const double* z, const double* vx, const double* vy,
+                             const double* vz, const double* m);
+ORR_API int orr_batch_download(orr_batch* batch, uint32_t system, double* x, double* y, double* z,
+                               double* vx, double* vy, double* vz, double* ax, double* ay,
+                               double* az);
+/* Device-to-device copy of one system's whole state (positions, velocities,
+ * accelerations, masses) between batches; counts must match. */
+ORR_API int orr_batch_copy_system(orr_batch* dst, uint32_t dst_system, orr_batch* src,
+                                  uint32_t src_system);
+/* Accelerations for the current positions of every system. */
+ORR_API int orr_batch_refresh(orr_batch* batch);
+/* `steps` steps of every system (synchronous). */
+ORR_API int orr_batch_step(orr_batch* batch, uint64_t steps);
+/* Times candidate tiles_per_item and leaf_capacity values on the batch's
+ * current state and keeps the fastest (for fields left 0 in the config). */
+ORR_API int orr_batch_autotune(orr_batch* batch);
+ORR_API int orr_batch_diagnostics(orr_batch* batch, uint32_t system, double out[8]);
+/* Interactions of the last evaluation of one system: particle_particle and
+ * particle_cell (direct: N(N-1) and 0). */
+ORR_API int orr_batch_counts(orr_batch* batch, uint32_t system, uint64_t out[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,28 @@ class OrrStepConfig(ctypes.Structure):
                 ("quadrupole", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class OrrBatchSystem(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("solver", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("softening_squared", ctypes.c_double), ("gravitational_constant", ctypes.c_double),
+                ("timestep", ctypes.c_double)]
+
+
+class OrrBatchConfig(ctypes.Structure):
+    _fields_ = [("integrator", ctypes.c_int32), ("quadrupole", ctypes.c_int32),
+                ("opening_angle", ctypes.c_double), ("leaf_capacity", ctypes.c_uint32),
+                ("tiles_per_item", ctypes.c_uint32), ("direct_below", ctypes.c_uint32),
+                ("use_graph", ctypes.c_int32)]
+
+
+class OrrBatchInfo(ctypes.Structure):
+    _fields_ = [("systems", ctypes.c_uint32), ("direct_systems", ctypes.c_uint32),
+                ("tree_systems", ctypes.c_uint32), ("tiles_per_item", ctypes.c_uint32),
+                ("leaf_capacity", ctypes.c_uint32), ("direct_tiles", ctypes.c_uint32),
+                ("direct_items", ctypes.c_uint32), ("graph", ctypes.c_int32),
+                ("particles", ctypes.c_uint64), ("direct_particles", ctypes.c_uint64),
+                ("tree_particles", ctypes.c_uint64), ("direct_pairs", ctypes.c_uint64)]
+
+
 # (name, argtypes) of every entry point the header declares.
 SIGNATURES = {
     "orr_device_count": [ctypes.POINTER(ctypes.c_int32)],
@@ -102,6 +124,18 @@ SIGNATURES = {
     "orr_system_refresh": [_vp, ctypes.POINTER(OrrStepConfig)],
     "orr_system_step": [_vp, ctypes.POINTER(OrrStepConfig), ctypes.c_double, ctypes.c_uint64],
     "orr_system_diagnostics": [_vp, ctypes.c_double, ctypes.c_double, _d],
+    "orr_batch_create": [_vp, ctypes.POINTER(OrrBatchSystem), ctypes.c_uint32,
+                         ctypes.POINTER(OrrBatchConfig), ctypes.POINTER(_vp)],
+    "orr_batch_destroy": [_vp],
+    "orr_batch_info_get": [_vp, ctypes.POINTER(OrrBatchInfo)],
+    "orr_batch_upload": [_vp, ctypes.c_uint32] + [_vp] * 7,
+    "orr_batch_download": [_vp, ctypes.c_uint32] + [_vp] * 9,
+    "orr_batch_copy_system": [_vp, ctypes.c_uint32, _vp, ctypes.c_uint32],
+    "orr_batch_refresh": [_vp],
+    "orr_batch_step": [_vp, ctypes.c_uint64],
+    "orr_batch_autotune": [_vp],
+    "orr_batch_diagnostics": [_vp, ctypes.c_uint32, _d],
+    "orr_batch_counts": [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
 }
 
 _LIB = None

@@ -16,32 +16,14 @@ namespace {
 
 thread_local std::string g_last_error;
 
-template<typename F> int guarded(F&& body) noexcept {
-    try {
-        body();
-        return ORR_OK;
-    } catch (const orr::Error& e) {
-        g_last_error = e.what();
-        return e.status();
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return ORR_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return ORR_ERR_RUNTIME;
-    } catch (...) {
-        g_last_error = "unknown failure";
-        return ORR_ERR_RUNTIME;
-    }
-}
-
-void require(bool ok, const char* what) {
-    if (!ok) orr::fail(ORR_ERR_INVALID_ARGUMENT, what);
-}
+using orr::guarded;
+using orr::require;
 
 void use_device(orr_ctx* ctx) { ORR_HIP(hipSetDevice(ctx->device)); }
 
 }  // namespace
+
+void orr::set_last_error(const char* what) { g_last_error = what; }
 
 struct orr_system {
     orr_ctx* ctx = nullptr;
@@ -104,10 +86,8 @@ void vv_substep(orr_system* s, const orr_step_config& c, double dt) {
               s->array(8), half);
 }
 
-// yoshida4.cpp:24-41 weights, evaluated in FP64 exactly as the reference's constexprs.
-constexpr double kCubeRootOfTwo = 1.2599210498948731647672106;
-constexpr double kOuterWeight = 1.0 / (2 - kCubeRootOfTwo);
-constexpr double kInnerWeight = -kCubeRootOfTwo / (2 - kCubeRootOfTwo);
+using orr::kInnerWeight;
+using orr::kOuterWeight;
 
 void check_config(const orr_step_config* c) {
     require(c != nullptr, "step config is null");
@@ -159,39 +139,12 @@ int orr_ctx_create(int32_t device, orr_ctx** out) {
         const hipError_t e = hipGetDeviceCount(&n);
         if (e != hipSuccess || n == 0) orr::fail(ORR_ERR_NO_DEVICE, "no HIP device");
         require(device >= 0 && device < n, "device index out of range");
-        ORR_HIP(hipSetDevice(device));
-        auto ctx = std::make_unique<orr_ctx>();
-        ctx->device = device;
-        hipDeviceProp_t p{};
-        ORR_HIP(hipGetDeviceProperties(&p, device));
-        ctx->compute_units = p.multiProcessorCount;
-        // Default priority on purpose: streams of mixed priority are not used.
-        ORR_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        *out = ctx.release();
+        *out = orr::make_context(device);
     });
 }
 
 int orr_ctx_destroy(orr_ctx* ctx) {
-    return guarded([&] {
-        if (ctx == nullptr) return;
-        use_device(ctx);
-        (void)hipStreamSynchronize(ctx->stream);
-        for (orr::Buffer* b : {&ctx->direct_packed, &ctx->direct_centres, &ctx->direct_partials,
-                               &ctx->order_codes, &ctx->order_index, &ctx->order_codes_alt,
-                               &ctx->order_index_alt, &ctx->order_hist, &ctx->order_bbox,
-                               &ctx->order_flags, &ctx->tree_nodes, &ctx->tree_aux,
-                               &ctx->tree_work, &ctx->tree_counts, &ctx->tree_sorted,
-                               &ctx->tree_leafref}) {
-            b->release();
-        }
-        for (auto& p : ctx->pending) {
-            (void)hipEventDestroy(p.start);
-            (void)hipEventDestroy(p.stop);
-        }
-        for (hipEvent_t e : ctx->free_events) (void)hipEventDestroy(e);
-        (void)hipStreamDestroy(ctx->stream);
-        delete ctx;
-    });
+    return guarded([&] { orr::free_context(ctx); });
 }
 
 int orr_malloc(orr_ctx* ctx, uint64_t bytes, void** out) {

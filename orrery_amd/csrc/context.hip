@@ -5,7 +5,9 @@
 // status-to-error mapping, written against the HIP runtime directly.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "engine.hpp"
@@ -23,8 +25,11 @@ void check_hip(hipError_t e, const char* what, const char* file, int line) {
     fail(ORR_ERR_RUNTIME, msg);
 }
 
+std::atomic<std::uint64_t> g_reallocations{0};
+
 void Buffer::reserve(std::size_t wanted) {
     if (wanted <= bytes) return;
+    g_reallocations.fetch_add(1, std::memory_order_relaxed);
     release();
     // Grow by a quarter beyond the request so that slowly growing systems do
     // not reallocate on every step.
@@ -39,6 +44,38 @@ void Buffer::release() noexcept {
     if (ptr != nullptr) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
+}
+
+orr_ctx* make_context(int device) {
+    ORR_HIP(hipSetDevice(device));
+    auto ctx = std::make_unique<orr_ctx>();
+    ctx->device = device;
+    hipDeviceProp_t p{};
+    ORR_HIP(hipGetDeviceProperties(&p, device));
+    ctx->compute_units = p.multiProcessorCount;
+    // Default priority on purpose: streams of mixed priority are not used.
+    ORR_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    return ctx.release();
+}
+
+void free_context(orr_ctx* ctx) noexcept {
+    if (ctx == nullptr) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (Buffer* b : {&ctx->direct_packed, &ctx->direct_centres, &ctx->direct_partials,
+                      &ctx->order_codes, &ctx->order_index, &ctx->order_codes_alt,
+                      &ctx->order_index_alt, &ctx->order_hist, &ctx->order_bbox,
+                      &ctx->order_flags, &ctx->tree_nodes, &ctx->tree_aux, &ctx->tree_work,
+                      &ctx->tree_counts, &ctx->tree_sorted, &ctx->tree_leafref}) {
+        b->release();
+    }
+    for (auto& p : ctx->pending) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (hipEvent_t e : ctx->free_events) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
 }
 
 }  // namespace orr

@@ -55,24 +55,15 @@ __device__ inline double wave_max(double v) {
     return v;
 }
 
-// One wave per tile: gather (optionally through `order`), centre, pack.
-// centres[t] = {cx, cy, cz, largest half-extent}; extents[t] = {hx, hy, hz, w}.
-__global__ __launch_bounds__(64) void direct_prepare(const double* __restrict__ x,
-                                                     const double* __restrict__ y,
-                                                     const double* __restrict__ z,
-                                                     const double* __restrict__ m,
-                                                     const std::uint32_t* __restrict__ order,
-                                                     std::uint64_t n, float4* __restrict__ packed,
-                                                     float4* __restrict__ packed_lo,
-                                                     double4* __restrict__ centres,
-                                                     float4* __restrict__ extents) {
-    const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * kTile + threadIdx.x;
-    const bool valid = k < n;
-    const std::uint64_t src = valid ? (order != nullptr ? order[k] : k) : 0;
-    const double px = valid ? x[src] : 0.0;
-    const double py = valid ? y[src] : 0.0;
-    const double pz = valid ? z[src] : 0.0;
-    const double pm = valid ? m[src] : 0.0;
+// One wave packs one tile: centre its (valid) particles, split each offset
+// into an FP32 value and its FP32 residual. centres[tile] = {cx, cy, cz,
+// largest half-extent}; extents[tile] = {hx, hy, hz, w}.
+__device__ __forceinline__ void pack_tile(double px, double py, double pz, double pm, bool valid,
+                                          std::uint64_t k, std::uint32_t tile,
+                                          float4* __restrict__ packed,
+                                          float4* __restrict__ packed_lo,
+                                          double4* __restrict__ centres,
+                                          float4* __restrict__ extents) {
     const double inf = __builtin_huge_val();
     const double lx = wave_min(valid ? px : inf), hx = wave_max(valid ? px : -inf);
     const double ly = wave_min(valid ? py : inf), hy = wave_max(valid ? py : -inf);
@@ -89,11 +80,28 @@ __global__ __launch_bounds__(64) void direct_prepare(const double* __restrict__ 
         packed[k] = make_float4(0.f, 0.f, 0.f, 0.f);
         packed_lo[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (threadIdx.x == 0) {
-        centres[blockIdx.x] = make_double4(cx, cy, cz, w);
-        extents[blockIdx.x] = make_float4(static_cast<float>(0.5 * (hx - lx)), static_cast<float>(0.5 * (hy - ly)),
-                                          static_cast<float>(0.5 * (hz - lz)), static_cast<float>(w));
+    if ((threadIdx.x & (kTile - 1)) == 0) {
+        centres[tile] = make_double4(cx, cy, cz, w);
+        extents[tile] = make_float4(static_cast<float>(0.5 * (hx - lx)), static_cast<float>(0.5 * (hy - ly)),
+                                    static_cast<float>(0.5 * (hz - lz)), static_cast<float>(w));
     }
+}
+
+// One wave per tile of one system: gather (optionally through `order`) and pack.
+__global__ __launch_bounds__(64) void direct_prepare(const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ z,
+                                                     const double* __restrict__ m,
+                                                     const std::uint32_t* __restrict__ order,
+                                                     std::uint64_t n, float4* __restrict__ packed,
+                                                     float4* __restrict__ packed_lo,
+                                                     double4* __restrict__ centres,
+                                                     float4* __restrict__ extents) {
+    const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * kTile + threadIdx.x;
+    const bool valid = k < n;
+    const std::uint64_t src = valid ? (order != nullptr ? order[k] : k) : 0;
+    pack_tile(valid ? x[src] : 0.0, valid ? y[src] : 0.0, valid ? z[src] : 0.0,
+              valid ? m[src] : 0.0, valid, k, blockIdx.x, packed, packed_lo, centres, extents);
 }
 
 // One pairwise term accumulated into (px, py, pz). kSafe masks r^2 == 0 (zero
@@ -125,6 +133,88 @@ __device__ __forceinline__ void kahan(float p, float& s, float& c) {
     const float t = s + y;
     c = (t - s) - y;
     s = t;
+}
+
+// Targets' sums against one 64-source tile J: the tile's terms summed plainly
+// in FP32, then Kahan-added per component into (s, k). kUnroll sets how many
+// far-path sources are in flight (their SGPR loads included).
+template<int R, bool kSafe, bool kRefine, int kUnroll = 16>
+__device__ __forceinline__ void source_tile(
+    const float4* __restrict__ packed, const float4* __restrict__ packed_lo,
+    const double4* __restrict__ centres, const float4* __restrict__ extents, std::uint32_t J,
+    float eps2, const float (&ox)[R], const float (&oy)[R], const float (&oz)[R],
+    const float (&lx)[R], const float (&ly)[R], const float (&lz)[R], const double (&cx)[R],
+    const double (&cy)[R], const double (&cz)[R], const float4 (&ce)[R], float (&sx)[R],
+    float (&sy)[R], float (&sz)[R], float (&kx)[R], float (&ky)[R], float (&kz)[R]) {
+    const double4 C = centres[J];
+    const float4 E = extents[J];
+    const float4* __restrict__ src = packed + static_cast<std::size_t>(J) * kTile;
+    // Near: the gap between the two tiles' boxes is at most kNearGap of their
+    // summed half-widths. Outside it every separation exceeds that gap, so the
+    // FP32 offsets round it to O(epsilon) relative error. All lanes of a wave
+    // share one target tile per r, so the branch is uniform.
+    bool near = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const float gx = fmaxf(0.f, fabsf(static_cast<float>(C.x - cx[r])) - ce[r].x - E.x);
+        const float gy = fmaxf(0.f, fabsf(static_cast<float>(C.y - cy[r])) - ce[r].y - E.y);
+        const float gz = fmaxf(0.f, fabsf(static_cast<float>(C.z - cz[r])) - ce[r].z - E.z);
+        const float reach = kNearGap * (ce[r].w + E.w);
+        near = near || (gx * gx + gy * gy + gz * gz <= reach * reach);
+    }
+    float px[R], py[R], pz[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) px[r] = py[r] = pz[r] = 0.f;
+    if (!near) {
+        float ux[R], uy[R], uz[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            // Target offset re-expressed relative to the source tile's centre.
+            ux[r] = ox[r] - static_cast<float>(C.x - cx[r]);
+            uy[r] = oy[r] - static_cast<float>(C.y - cy[r]);
+            uz[r] = oz[r] - static_cast<float>(C.z - cz[r]);
+        }
+#pragma unroll kUnroll
+        for (int j = 0; j < kTile; ++j) {
+            const float4 q = src[j];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                term<kSafe, kRefine>(q.x - ux[r], q.y - uy[r], q.z - uz[r], q.w, eps2, px[r],
+                                     py[r], pz[r]);
+        }
+    } else {
+        // Double-float target position relative to the source tile's centre.
+        float th[R][3], tl[R][3];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double t[3] = {(static_cast<double>(ox[r]) + lx[r]) - (C.x - cx[r]),
+                                 (static_cast<double>(oy[r]) + ly[r]) - (C.y - cy[r]),
+                                 (static_cast<double>(oz[r]) + lz[r]) - (C.z - cz[r])};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                th[r][c] = static_cast<float>(t[c]);
+                tl[r][c] = static_cast<float>(t[c] - th[r][c]);
+            }
+        }
+        const float4* __restrict__ src_lo = packed_lo + static_cast<std::size_t>(J) * kTile;
+#pragma unroll 8
+        for (int j = 0; j < kTile; ++j) {
+            const float4 q = src[j];
+            const float4 ql = src_lo[j];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                term<kSafe, kRefine>((q.x - th[r][0]) + (ql.x - tl[r][0]),
+                                     (q.y - th[r][1]) + (ql.y - tl[r][1]),
+                                     (q.z - th[r][2]) + (ql.z - tl[r][2]), q.w, eps2, px[r],
+                                     py[r], pz[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        kahan(px[r], sx[r], kx[r]);
+        kahan(py[r], sy[r], ky[r]);
+        kahan(pz[r], sz[r], kz[r]);
+    }
 }
 
 template<int R, bool kSafe, bool kRefine>
@@ -164,77 +254,9 @@ __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict_
         kx[r] = ky[r] = kz[r] = 0.f;
     }
 
-    for (std::uint32_t J = j0; J < j1; ++J) {
-        const double4 C = centres[J];
-        const float4 E = extents[J];
-        const float4* __restrict__ src = packed + static_cast<std::size_t>(J) * kTile;
-        // Near: the gap between the two tiles' boxes is at most kNearGap of their
-        // summed half-widths. Outside it every separation exceeds that gap, so the
-        // FP32 offsets round it to O(epsilon) relative error. All lanes of a wave
-        // share one target tile per r, so the branch is uniform.
-        bool near = false;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float gx = fmaxf(0.f, fabsf(static_cast<float>(C.x - cx[r])) - ce[r].x - E.x);
-            const float gy = fmaxf(0.f, fabsf(static_cast<float>(C.y - cy[r])) - ce[r].y - E.y);
-            const float gz = fmaxf(0.f, fabsf(static_cast<float>(C.z - cz[r])) - ce[r].z - E.z);
-            const float reach = kNearGap * (ce[r].w + E.w);
-            near = near || (gx * gx + gy * gy + gz * gz <= reach * reach);
-        }
-        float px[R], py[R], pz[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) px[r] = py[r] = pz[r] = 0.f;
-        if (!near) {
-            float ux[R], uy[R], uz[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                // Target offset re-expressed relative to the source tile's centre.
-                ux[r] = ox[r] - static_cast<float>(C.x - cx[r]);
-                uy[r] = oy[r] - static_cast<float>(C.y - cy[r]);
-                uz[r] = oz[r] - static_cast<float>(C.z - cz[r]);
-            }
-#pragma unroll 16
-            for (int j = 0; j < kTile; ++j) {
-                const float4 q = src[j];
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-                    term<kSafe, kRefine>(q.x - ux[r], q.y - uy[r], q.z - uz[r], q.w, eps2, px[r],
-                                         py[r], pz[r]);
-            }
-        } else {
-            // Double-float target position relative to the source tile's centre.
-            float th[R][3], tl[R][3];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const double t[3] = {(static_cast<double>(ox[r]) + lx[r]) - (C.x - cx[r]),
-                                     (static_cast<double>(oy[r]) + ly[r]) - (C.y - cy[r]),
-                                     (static_cast<double>(oz[r]) + lz[r]) - (C.z - cz[r])};
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    th[r][c] = static_cast<float>(t[c]);
-                    tl[r][c] = static_cast<float>(t[c] - th[r][c]);
-                }
-            }
-            const float4* __restrict__ src_lo = packed_lo + static_cast<std::size_t>(J) * kTile;
-#pragma unroll 8
-            for (int j = 0; j < kTile; ++j) {
-                const float4 q = src[j];
-                const float4 ql = src_lo[j];
-#pragma unroll
-                for (int r = 0; r < R; ++r)
-                    term<kSafe, kRefine>((q.x - th[r][0]) + (ql.x - tl[r][0]),
-                                         (q.y - th[r][1]) + (ql.y - tl[r][1]),
-                                         (q.z - th[r][2]) + (ql.z - tl[r][2]), q.w, eps2, px[r],
-                                         py[r], pz[r]);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            kahan(px[r], sx[r], kx[r]);
-            kahan(py[r], sy[r], ky[r]);
-            kahan(pz[r], sz[r], kz[r]);
-        }
-    }
+    for (std::uint32_t J = j0; J < j1; ++J)
+        source_tile<R, kSafe, kRefine>(packed, packed_lo, centres, extents, J, eps2, ox, oy, oz,
+                                       lx, ly, lz, cx, cy, cz, ce, sx, sy, sz, kx, ky, kz);
 
     float* out = partial + static_cast<std::size_t>(split) * 6 * npad;
 #pragma unroll
@@ -268,6 +290,124 @@ __global__ __launch_bounds__(256) void direct_reduce(const float* __restrict__ p
         a1 += static_cast<double>(p[npad + k]) - static_cast<double>(p[4 * npad + k]);
         a2 += static_cast<double>(p[2 * npad + k]) - static_cast<double>(p[5 * npad + k]);
     }
+    const std::uint64_t dst = order != nullptr ? order[k] : k;
+    ax[dst] = G * a0;
+    ay[dst] = G * a1;
+    az[dst] = G * a2;
+}
+
+// ---- batches: many independent systems in one launch ---------------------
+//
+// Each system of a batch owns whole tiles (its last tile padded with massless
+// lanes), so a tile never mixes systems. tile[t] = {first particle, valid
+// count, system, first partial slot}. A work item {target tile, first source
+// tile, end source tile, partial slot} is one wave's job; the split of a
+// system's source range depends on that system's size only, so its result does
+// not depend on which other systems share the batch.
+
+// One wave per tile of the batch.
+__global__ __launch_bounds__(64) void batch_prepare(const double* __restrict__ x,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ z,
+                                                    const double* __restrict__ m,
+                                                    const std::uint32_t* __restrict__ order,
+                                                    const uint4* __restrict__ tile,
+                                                    float4* __restrict__ packed,
+                                                    float4* __restrict__ packed_lo,
+                                                    double4* __restrict__ centres,
+                                                    float4* __restrict__ extents) {
+    const uint4 T = tile[blockIdx.x];
+    const bool valid = threadIdx.x < T.y;
+    const std::uint64_t k = static_cast<std::uint64_t>(T.x) + threadIdx.x;
+    const std::uint64_t src = valid ? (order != nullptr ? order[k] : k) : 0;
+    pack_tile(valid ? x[src] : 0.0, valid ? y[src] : 0.0, valid ? z[src] : 0.0,
+              valid ? m[src] : 0.0, valid,
+              static_cast<std::uint64_t>(blockIdx.x) * kTile + threadIdx.x, blockIdx.x, packed,
+              packed_lo, centres, extents);
+}
+
+// Persistent: every wave takes work items from a queue (one counter, zero on
+// entry) until the list is exhausted, so long items started first (the list is
+// sorted by length) keep all CUs busy to the end. Each item writes its own
+// partial slot: the result does not depend on which wave ran it.
+constexpr int kItemUnroll = 8;
+
+// Eight waves per SIMD: without the hint the scheduler spends registers on
+// instruction-level parallelism (87 VGPRs, SGPRs spilled to VGPR lanes).
+template<bool kSafe, bool kRefine>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void direct_items(const float4* __restrict__ packed,
+                                                       const float4* __restrict__ packed_lo,
+                                                       const double4* __restrict__ centres,
+                                                       const float4* __restrict__ extents,
+                                                       const uint4* __restrict__ tile,
+                                                       const uint4* __restrict__ item,
+                                                       std::uint32_t items,
+                                                       const float* __restrict__ eps2_of,
+                                                       unsigned* __restrict__ queue,
+                                                       float* __restrict__ partial) {
+    const unsigned lane = threadIdx.x & (kTile - 1);
+    for (;;) {
+        unsigned taken = 0;
+        if (lane == 0) taken = atomicAdd(queue, 1u);
+        const std::uint32_t next = __builtin_amdgcn_readfirstlane(taken);
+        if (next >= items) break;
+        const uint4 it = item[next];
+        const float eps2 = eps2_of[tile[it.x].z];
+        const std::uint32_t k = it.x * kTile + lane;
+        const float4 o = packed[k];
+        const float4 l = packed_lo[k];
+        const double4 c = centres[it.x];
+        const float ox[1] = {o.x}, oy[1] = {o.y}, oz[1] = {o.z};
+        const float lx[1] = {l.x}, ly[1] = {l.y}, lz[1] = {l.z};
+        const double cx[1] = {c.x}, cy[1] = {c.y}, cz[1] = {c.z};
+        const float4 ce[1] = {extents[it.x]};
+        float sx[1] = {0.f}, sy[1] = {0.f}, sz[1] = {0.f}, kx[1] = {0.f}, ky[1] = {0.f},
+              kz[1] = {0.f};
+        for (std::uint32_t J = it.y; J < it.z; ++J)
+            source_tile<1, kSafe, kRefine, kItemUnroll>(packed, packed_lo, centres, extents, J, eps2,
+                                                        ox, oy, oz, lx, ly, lz, cx, cy, cz, ce, sx,
+                                                        sy, sz, kx, ky, kz);
+        float* out = partial + static_cast<std::size_t>(it.w) * 6 * kTile;
+        out[lane] = sx[0];
+        out[kTile + lane] = sy[0];
+        out[2 * kTile + lane] = sz[0];
+        out[3 * kTile + lane] = kx[0];
+        out[4 * kTile + lane] = ky[0];
+        out[5 * kTile + lane] = kz[0];
+    }
+}
+
+// One thread per particle of the batch (sorted position k): fold its tile's
+// slots in split order in FP64, multiply by the system's G, scatter through
+// `order`. sys[s] = {first particle, count, first tile, splits}. Leaves the
+// work queue at zero for the next launch.
+__global__ __launch_bounds__(256) void batch_reduce(const float* __restrict__ partial,
+                                                    const uint4* __restrict__ sys,
+                                                    const uint4* __restrict__ tile,
+                                                    const std::uint32_t* __restrict__ seg_of,
+                                                    const double* __restrict__ G_of,
+                                                    std::uint64_t n,
+                                                    const std::uint32_t* __restrict__ order,
+                                                    unsigned* __restrict__ queue,
+                                                    double* __restrict__ ax,
+                                                    double* __restrict__ ay,
+                                                    double* __restrict__ az) {
+    const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k == 0) *queue = 0;
+    if (k >= n) return;
+    const std::uint32_t s = seg_of[k];
+    const uint4 S = sys[s];
+    const std::uint32_t local = static_cast<std::uint32_t>(k) - S.x;
+    const std::uint32_t lane = local & (kTile - 1);
+    const std::uint32_t first_slot = tile[S.z + local / kTile].w;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (std::uint32_t i = 0; i < S.w; ++i) {
+        const float* p = partial + static_cast<std::size_t>(first_slot + i) * 6 * kTile + lane;
+        a0 += static_cast<double>(p[0]) - static_cast<double>(p[3 * kTile]);
+        a1 += static_cast<double>(p[kTile]) - static_cast<double>(p[4 * kTile]);
+        a2 += static_cast<double>(p[2 * kTile]) - static_cast<double>(p[5 * kTile]);
+    }
+    const double G = G_of[s];
     const std::uint64_t dst = order != nullptr ? order[k] : k;
     ax[dst] = G * a0;
     ay[dst] = G * a1;
@@ -344,6 +484,46 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
         hipLaunchKernelGGL(direct_reduce, dim3(blocks), dim3(256), 0, ctx->stream, partials, splits,
                            npad, n, order, a.gravitational_constant, a.ax, a.ay, a.az);
         ORR_LAUNCHED("direct_reduce");
+    }
+}
+
+void direct_batch_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                          const double* m, const std::uint32_t* order, const DirectBatchView& b,
+                          double* ax, double* ay, double* az) {
+    if (b.count == 0 || b.tiles == 0) return;
+    const std::size_t npad = static_cast<std::size_t>(b.tiles) * kTile;
+    ctx->direct_packed.reserve(2 * sizeof(float4) * npad);
+    ctx->direct_centres.reserve((sizeof(double4) + sizeof(float4)) * b.tiles);
+    ctx->direct_partials.reserve(sizeof(float) * 6 * kTile * static_cast<std::size_t>(b.slots));
+    float4* packed = ctx->direct_packed.as<float4>();
+    float4* packed_lo = packed + npad;
+    double4* centres = ctx->direct_centres.as<double4>();
+    float4* extents = reinterpret_cast<float4*>(centres + b.tiles);
+    float* partials = ctx->direct_partials.as<float>();
+    {
+        PhaseTimer t(ctx, kDirectPrepare);
+        hipLaunchKernelGGL(batch_prepare, dim3(b.tiles), dim3(kTile), 0, ctx->stream, x, y, z, m,
+                           order, b.tile, packed, packed_lo, centres, extents);
+        ORR_LAUNCHED("batch_prepare");
+    }
+    {
+        PhaseTimer t(ctx, kDirectKernel);
+        // Eight blocks of four waves per CU; never more waves than items.
+        const std::uint32_t cap = static_cast<std::uint32_t>(std::max(1, ctx->compute_units)) * 8;
+        const std::uint32_t blocks = std::max<std::uint32_t>(1, std::min(cap, (b.items + 3) / 4));
+        const bool refine = ctx->direct_refine != 0;
+        auto kernel = b.safe ? (refine ? direct_items<true, true> : direct_items<true, false>)
+                             : (refine ? direct_items<false, true> : direct_items<false, false>);
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, ctx->stream, packed, packed_lo,
+                           centres, extents, b.tile, b.item, b.items, b.eps2, b.queue, partials);
+        ORR_LAUNCHED("direct_items");
+    }
+    {
+        PhaseTimer t(ctx, kDirectReduce);
+        const std::uint32_t blocks = static_cast<std::uint32_t>((b.count + 255) / 256);
+        hipLaunchKernelGGL(batch_reduce, dim3(blocks), dim3(256), 0, ctx->stream, partials, b.sys,
+                           b.tile, b.seg_of, b.G, b.count, order, b.queue, ax, ay, az);
+        ORR_LAUNCHED("batch_reduce");
     }
 }
 

@@ -8,8 +8,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -30,8 +32,39 @@ private:
 [[noreturn]] void fail(int status, const std::string& what);
 void check_hip(hipError_t e, const char* what, const char* file, int line);
 
+// The extern "C" layer's error channel: the message behind orr_last_error().
+void set_last_error(const char* what);
+
+// Runs body, mapping any exception to its status and message.
+template<typename F> int guarded(F&& body) noexcept {
+    try {
+        body();
+        return ORR_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.status();
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return ORR_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return ORR_ERR_RUNTIME;
+    } catch (...) {
+        set_last_error("unknown failure");
+        return ORR_ERR_RUNTIME;
+    }
+}
+
+inline void require(bool ok, const char* what) {
+    if (!ok) fail(ORR_ERR_INVALID_ARGUMENT, what);
+}
+
 #define ORR_HIP(expr) ::orr::check_hip((expr), #expr, __FILE__, __LINE__)
 #define ORR_LAUNCHED(what) ::orr::check_hip(hipGetLastError(), what, __FILE__, __LINE__)
+
+// Count of device reallocations by any Buffer: a captured graph holds raw
+// pointers, so it is valid only while this count is unchanged.
+extern std::atomic<std::uint64_t> g_reallocations;
 
 // A device allocation that only grows (the reference's solver pimpl rule,
 // cuda_direct_solver.cpp:78-137).
@@ -83,6 +116,7 @@ struct orr_ctx {
     std::size_t tree_last_bound = 0;
     double tree_last_theta = 0.5;
     bool tree_last_quadrupole = false;
+    std::uint32_t tree_last_segments = 1;
 
     // Phase timing with hipEvents recorded on `stream`.
     bool timing = false;
@@ -103,6 +137,16 @@ struct orr_ctx {
 
 namespace orr {
 
+// yoshida4.cpp:24-41 weights, evaluated in FP64 exactly as the reference's constexprs.
+inline constexpr double kCubeRootOfTwo = 1.2599210498948731647672106;
+inline constexpr double kOuterWeight = 1.0 / (2 - kCubeRootOfTwo);
+inline constexpr double kInnerWeight = -kCubeRootOfTwo / (2 - kCubeRootOfTwo);
+
+// context.hip: a context with its own stream (default priority); freeing
+// synchronises it and releases every buffer.
+orr_ctx* make_context(int device);
+void free_context(orr_ctx* ctx) noexcept;
+
 // RAII phase marker: records events around the launches it brackets.
 class PhaseTimer {
 public:
@@ -122,6 +166,40 @@ private:
 // direct.hip: enqueue the direct sum (no synchronisation).
 void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order);
 
+// direct.hip, batched: the direct systems of a batch as one tile space.
+// Tables are device arrays; tile/item/slot meaning is given in direct.hip.
+struct DirectBatchView {
+    std::uint64_t count = 0;                 // particles (sorted positions 0..count-1)
+    std::uint32_t systems = 0, tiles = 0, items = 0, slots = 0;
+    const uint4* sys = nullptr;              // [systems] {first, count, first tile, splits}
+    const uint4* tile = nullptr;             // [tiles] {first, valid, system, first slot}
+    const uint4* item = nullptr;             // [items] {tile, source tile begin, end, slot}
+    const std::uint32_t* seg_of = nullptr;   // [count] system of each sorted position
+    const float* eps2 = nullptr;             // [systems] softening squared
+    const double* G = nullptr;               // [systems] gravitational constant
+    unsigned* queue = nullptr;               // work counter: zero on entry, left zero
+    bool safe = false;                       // some system has zero softening
+};
+void direct_batch_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                          const double* m, const std::uint32_t* order, const DirectBatchView& b,
+                          double* ax, double* ay, double* az);
+
+// Independent systems laid end to end in one set of arrays (a batch). All
+// pointers are device arrays; count == 1 with null pointers is one system.
+struct SegmentView {
+    std::uint32_t count = 1;
+    const std::uint32_t* start = nullptr;        // [count + 1] first particle of each system
+    const std::uint32_t* seg_of = nullptr;       // [n] system of each particle
+    const std::uint32_t* chunks = nullptr;       // [chunk_count][3] {system, begin, end}
+    const std::uint32_t* chunk_first = nullptr;  // [count + 1] first chunk of each system
+    std::uint32_t chunk_count = 0;
+};
+
+// morton.hip, segmented: one cube per system (cubes at ctx->order_bbox[4 * s]),
+// order by (system, code, index). Systems keep their particle ranges.
+void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                             std::uint64_t n, const SegmentView& segments);
+
 // morton.hip: bounding cube, codes and stable radix sort from identity order.
 // Leaves ascending codes and permutation in ctx->order_codes / order_index
 // (uint64 / uint32). Returns nothing on the host; cube lands in ctx->order_bbox.
@@ -138,7 +216,15 @@ void exclusive_scan_u32(orr_ctx* ctx, const std::uint32_t* in, std::uint32_t* ou
 
 // tree.hip: order, build, moments and walk; fills stats (syncs).
 void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats);
+// Segmented: one octree per system. seg_stats (device, may be null) receives
+// {particle_particle, particle_cell, node_visits, 0} per system and must be
+// zeroed by the caller; stats (may be null) gets the totals (syncs).
+void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentView& segments,
+                            unsigned long long* seg_stats, orr_tree_stats* stats);
 void tree_copy_quadrupoles(orr_ctx* ctx, double* out);
+// {particle_particle, particle_cell, node_visits, 0} per segment of the last
+// segmented evaluation that kept its counts internally (seg_stats null); syncs.
+void tree_copy_segment_counts(orr_ctx* ctx, std::uint64_t* out);
 void tree_copy_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double* cz,
                      double* acc, std::uint64_t* next, std::uint64_t* first,
                      std::uint64_t* count);

@@ -16,7 +16,9 @@
 // LDS table of per-wave digit counts, so no LDS atomics decide an order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <utility>
 
 #include "engine.hpp"
 
@@ -70,6 +72,9 @@ __global__ __launch_bounds__(kBlock) void bbox_partial(const double* __restrict_
     if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
+// The cube arithmetic of morton.cpp:87-103 from exact extrema: cube = {origin, size}.
+__device__ void make_cube(const double lo[3], const double hi[3], double* __restrict__ cube);
+
 // One block folds the block extrema (min and max are exact, so the order does
 // not matter), then one thread does the cube arithmetic of morton.cpp:87-103.
 // cube = {origin x, y, z, size}.
@@ -109,6 +114,65 @@ __global__ __launch_bounds__(kBlock) void bbox_final(const double* __restrict__ 
         lo[c] = s[c][0];
         hi[c] = s[3 + c][0];
     }
+    make_cube(lo, hi, cube);
+}
+
+// One block per chunk {segment, begin, end} of one segment: its extrema.
+__global__ __launch_bounds__(kBlock) void bbox_chunks(const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z,
+                                                      const std::uint32_t* __restrict__ chunks,
+                                                      double* __restrict__ partial) {
+    __shared__ double s[6][kBlock];
+    const double inf = __builtin_huge_val();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    const std::uint32_t b = chunks[3 * blockIdx.x + 1], e = chunks[3 * blockIdx.x + 2];
+    for (std::uint32_t i = b + threadIdx.x; i < e; i += kBlock) {
+        const double v[3] = {x[i], y[i], z[i]};
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = v[c] < lo[c] ? v[c] : lo[c];
+            hi[c] = v[c] > hi[c] ? v[c] : hi[c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        s[c][threadIdx.x] = lo[c];
+        s[3 + c][threadIdx.x] = hi[c];
+    }
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            for (int c = 0; c < 3; ++c) {
+                const double a = s[c][threadIdx.x + w];
+                if (a < s[c][threadIdx.x]) s[c][threadIdx.x] = a;
+                const double h = s[3 + c][threadIdx.x + w];
+                if (h > s[3 + c][threadIdx.x]) s[3 + c][threadIdx.x] = h;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// One thread per segment: fold its chunks' extrema, then its cube.
+__global__ __launch_bounds__(kBlock) void bbox_segments(const double* __restrict__ partial,
+                                                        const std::uint32_t* __restrict__ chunk_first,
+                                                        std::uint32_t segments,
+                                                        double* __restrict__ cubes) {
+    const std::uint32_t seg = blockIdx.x * kBlock + threadIdx.x;
+    if (seg >= segments) return;
+    const double inf = __builtin_huge_val();
+    double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+    for (std::uint32_t c = chunk_first[seg]; c < chunk_first[seg + 1]; ++c) {
+        for (int k = 0; k < 3; ++k) {
+            const double a = partial[c * 6 + k], h = partial[c * 6 + 3 + k];
+            lo[k] = a < lo[k] ? a : lo[k];
+            hi[k] = h > hi[k] ? h : hi[k];
+        }
+    }
+    make_cube(lo, hi, cubes + 4 * static_cast<std::size_t>(seg));
+}
+
+__device__ void make_cube(const double lo[3], const double hi[3], double* __restrict__ cube) {
     const double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
     // std::max({ex, ey, ez}) keeps the first of equals; the value is what matters.
     double size = ex;
@@ -144,11 +208,13 @@ __device__ inline std::uint32_t to_grid(double coordinate, double origin, double
 __global__ __launch_bounds__(kBlock) void encode(const double* __restrict__ x,
                                                  const double* __restrict__ y,
                                                  const double* __restrict__ z, std::uint64_t n,
-                                                 const double* __restrict__ cube,
+                                                 const double* __restrict__ cubes,
+                                                 const std::uint32_t* __restrict__ seg_of,
                                                  std::uint64_t* __restrict__ codes,
                                                  std::uint32_t* __restrict__ index) {
     const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
+    const double* cube = cubes + (seg_of != nullptr ? 4 * static_cast<std::size_t>(seg_of[i]) : 0);
     const double size = cube[3];
     std::uint64_t code = 0;
     if (size > 0) {
@@ -164,8 +230,18 @@ __global__ __launch_bounds__(kBlock) void encode(const double* __restrict__ x,
 // ---- radix sort -------------------------------------------------------------
 
 // hist is digit-major: hist[d * blocks + b] = count of digit d in block b.
+// With seg_of, the digit is taken from the segment of each element's value
+// instead of its key: the passes that order by segment after the code passes.
+__device__ inline std::uint32_t digit_of(std::uint64_t key, std::uint32_t val, int shift,
+                                         const std::uint32_t* __restrict__ seg_of) {
+    const std::uint64_t k = seg_of != nullptr ? static_cast<std::uint64_t>(seg_of[val]) : key;
+    return static_cast<std::uint32_t>((k >> shift) & 255u);
+}
+
 __global__ __launch_bounds__(kBlock) void radix_histogram(const std::uint64_t* __restrict__ keys,
+                                                          const std::uint32_t* __restrict__ vals,
                                                           std::uint64_t n, int shift,
+                                                          const std::uint32_t* __restrict__ seg_of,
                                                           std::uint32_t* __restrict__ hist) {
     __shared__ std::uint32_t counts[256];
     counts[threadIdx.x] = 0;
@@ -173,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void radix_histogram(const std::uint64_t* _
     const std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * kKeysPerBlock;
     for (int t = 0; t < kTilesPerBlock; ++t) {
         const std::uint64_t i = base + static_cast<std::uint64_t>(t) * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&counts[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&counts[digit_of(keys[i], vals[i], shift, seg_of)], 1u);
     }
     __syncthreads();
     hist[static_cast<std::uint64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = counts[threadIdx.x];
@@ -182,6 +258,7 @@ __global__ __launch_bounds__(kBlock) void radix_histogram(const std::uint64_t* _
 __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __restrict__ keys_in,
                                                         const std::uint32_t* __restrict__ vals_in,
                                                         std::uint64_t n, int shift,
+                                                        const std::uint32_t* __restrict__ seg_of,
                                                         const std::uint32_t* __restrict__ offsets,
                                                         std::uint64_t* __restrict__ keys_out,
                                                         std::uint32_t* __restrict__ vals_out) {
@@ -199,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __r
         const bool valid = i < n;
         const std::uint64_t key = valid ? keys_in[i] : 0;
         const std::uint32_t val = valid ? vals_in[i] : 0;
-        const std::uint32_t d = static_cast<std::uint32_t>((key >> shift) & 255u);
+        const std::uint32_t d = valid ? digit_of(key, val, shift, seg_of) : 0u;
         // Lanes of this wave holding a valid key with the same digit.
         std::uint64_t same = __ballot(valid);
 #pragma unroll
@@ -231,52 +308,81 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __r
 
 void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
                     std::uint64_t n) {
+    morton_enqueue_segments(ctx, x, y, z, n, SegmentView{});
+}
+
+void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                             std::uint64_t n, const SegmentView& segs) {
     if (n == 0) return;
     if (n > (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "morton: more than 2^31 particles");
-    const std::uint32_t bbox_blocks = static_cast<std::uint32_t>(std::min<std::uint64_t>(1024, (n + kBlock - 1) / kBlock));
+    const bool segmented = segs.count > 1;
+    const std::uint32_t bbox_blocks = segmented
+        ? segs.chunk_count
+        : static_cast<std::uint32_t>(std::min<std::uint64_t>(1024, (n + kBlock - 1) / kBlock));
     const std::uint32_t sort_blocks = static_cast<std::uint32_t>((n + kKeysPerBlock - 1) / kKeysPerBlock);
-    ctx->order_bbox.reserve(sizeof(double) * (6 * bbox_blocks + 4));
+    ctx->order_bbox.reserve(sizeof(double) * (6 * bbox_blocks + 4 * static_cast<std::size_t>(segs.count)));
     ctx->order_codes.reserve(sizeof(std::uint64_t) * n);
     ctx->order_codes_alt.reserve(sizeof(std::uint64_t) * n);
     ctx->order_index.reserve(sizeof(std::uint32_t) * n);
     ctx->order_index_alt.reserve(sizeof(std::uint32_t) * n);
     ctx->order_hist.reserve(sizeof(std::uint32_t) *
                             (256 * sort_blocks + scan_scratch_words(256 * sort_blocks)));
-    double* partial = ctx->order_bbox.as<double>() + 4;
-    double* cube = ctx->order_bbox.as<double>();
+    double* cube = ctx->order_bbox.as<double>();  // cubes[4 * segment]
+    double* partial = cube + 4 * static_cast<std::size_t>(segs.count);
     {
         PhaseTimer t(ctx, kBbox);
-        hipLaunchKernelGGL(bbox_partial, dim3(bbox_blocks), dim3(kBlock), 0, ctx->stream, x, y, z, n,
-                           partial);
-        ORR_LAUNCHED("bbox_partial");
-        hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBlock), 0, ctx->stream, partial, bbox_blocks,
-                           cube);
-        ORR_LAUNCHED("bbox_final");
+        if (segmented) {
+            hipLaunchKernelGGL(bbox_chunks, dim3(segs.chunk_count), dim3(kBlock), 0, ctx->stream, x, y,
+                               z, segs.chunks, partial);
+            ORR_LAUNCHED("bbox_chunks");
+            hipLaunchKernelGGL(bbox_segments, dim3((segs.count + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                               ctx->stream, partial, segs.chunk_first, segs.count, cube);
+            ORR_LAUNCHED("bbox_segments");
+        } else {
+            hipLaunchKernelGGL(bbox_partial, dim3(bbox_blocks), dim3(kBlock), 0, ctx->stream, x, y, z,
+                               n, partial);
+            ORR_LAUNCHED("bbox_partial");
+            hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kBlock), 0, ctx->stream, partial,
+                               bbox_blocks, cube);
+            ORR_LAUNCHED("bbox_final");
+        }
     }
     std::uint64_t* keys[2] = {ctx->order_codes.as<std::uint64_t>(), ctx->order_codes_alt.as<std::uint64_t>()};
     std::uint32_t* vals[2] = {ctx->order_index.as<std::uint32_t>(), ctx->order_index_alt.as<std::uint32_t>()};
+    const std::uint32_t* seg_of = segmented ? segs.seg_of : nullptr;
     {
         PhaseTimer t(ctx, kEncode);
         hipLaunchKernelGGL(encode, dim3(static_cast<std::uint32_t>((n + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, ctx->stream, x, y, z, n, cube, keys[0], vals[0]);
+                           dim3(kBlock), 0, ctx->stream, x, y, z, n, cube, seg_of, keys[0], vals[0]);
         ORR_LAUNCHED("encode");
     }
     {
         PhaseTimer t(ctx, kSort);
         std::uint32_t* hist = ctx->order_hist.as<std::uint32_t>();
+        // Eight passes over the 63-bit codes, then (segments > 1) one or two
+        // passes over the segment of each element: segment-major, then (code, index).
+        int passes = 8;
+        if (segmented) passes += segs.count > 256 ? 2 : 1;
         int cur = 0;
-        for (int pass = 0; pass < 8; ++pass) {
-            const int shift = 8 * pass;
+        for (int pass = 0; pass < passes; ++pass) {
+            const bool by_segment = pass >= 8;
+            const int shift = by_segment ? 8 * (pass - 8) : 8 * pass;
+            const std::uint32_t* digits = by_segment ? seg_of : nullptr;
             hipLaunchKernelGGL(radix_histogram, dim3(sort_blocks), dim3(kBlock), 0, ctx->stream,
-                               keys[cur], n, shift, hist);
+                               keys[cur], vals[cur], n, shift, digits, hist);
             ORR_LAUNCHED("radix_histogram");
             exclusive_scan_u32(ctx, hist, hist, 256u * sort_blocks, hist + 256u * sort_blocks, nullptr);
             hipLaunchKernelGGL(radix_scatter, dim3(sort_blocks), dim3(kBlock), 0, ctx->stream,
-                               keys[cur], vals[cur], n, shift, hist, keys[1 - cur], vals[1 - cur]);
+                               keys[cur], vals[cur], n, shift, digits, hist, keys[1 - cur],
+                               vals[1 - cur]);
             ORR_LAUNCHED("radix_scatter");
             cur = 1 - cur;
         }
-        // Eight passes end back in the primary buffers (cur == 0).
+        if (cur != 0) {
+            // An odd pass count ends in the alternate buffers: swap the roles.
+            std::swap(ctx->order_codes, ctx->order_codes_alt);
+            std::swap(ctx->order_index, ctx->order_index_alt);
+        }
     }
 }
 

@@ -42,6 +42,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -71,18 +72,23 @@ struct QuadF {
 __device__ inline std::uint32_t grid_stride_start() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline std::uint32_t grid_stride_step() { return gridDim.x * blockDim.x; }
 
-__global__ void clear_counters(std::uint32_t* __restrict__ c32, unsigned long long* __restrict__ c64) {
+__global__ void clear_counters(std::uint32_t* __restrict__ c32, unsigned long long* __restrict__ c64,
+                               std::uint32_t count64) {
     if (threadIdx.x < 4) c32[threadIdx.x] = 0;
-    if (threadIdx.x < 4) c64[threadIdx.x] = 0;
+    if (c64 != nullptr) {
+        for (std::uint32_t k = threadIdx.x; k < count64; k += blockDim.x) c64[k] = 0;
+    }
 }
 
-// d(k) from adjacent sorted codes.
+// d(k) from adjacent sorted codes; 0 where a system starts (each system of a
+// batch is its own tree, and a level-0 boundary ends every window at it).
 __global__ __launch_bounds__(kBlock) void tree_levels(const std::uint64_t* __restrict__ codes,
+                                                      const std::uint32_t* __restrict__ seg_of,
                                                       std::uint32_t n, std::uint8_t* __restrict__ d) {
     const std::uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= n) return;
-    if (k == 0) {
-        d[0] = 0;
+    if (k == 0 || (seg_of != nullptr && seg_of[k] != seg_of[k - 1])) {
+        d[k] = 0;
         return;
     }
     const std::uint64_t x = codes[k] ^ codes[k - 1];
@@ -106,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void tree_chain(const std::uint8_t* __restr
     unsigned lleaf = dk;
     if (dk <= kLevels) {
         bool head = true;
-        if (k > 0) {
+        if (dk > 0) {  // d = 0: the root of its system
             // Parent cell of level dk-1 spans [p, q) around k; it is a leaf (so
             // no node starts at k) iff q - p <= cap.
             const unsigned P = dk - 1;
@@ -180,7 +186,8 @@ __global__ __launch_bounds__(kBlock) void tree_nodes(
     const std::uint64_t* __restrict__ codes, const std::uint8_t* __restrict__ d,
     const std::uint8_t* __restrict__ leaf_level, const std::uint32_t* __restrict__ chain,
     const std::uint32_t* __restrict__ first_node, std::uint32_t n, std::uint32_t cap,
-    const double* __restrict__ cube, std::uint32_t* __restrict__ counters,
+    const std::uint32_t* __restrict__ seg_of, const std::uint32_t* __restrict__ seg_start,
+    const double* __restrict__ cubes, std::uint32_t* __restrict__ counters,
     std::uint32_t* __restrict__ node_begin, std::uint32_t* __restrict__ node_count,
     std::uint32_t* __restrict__ node_next, std::uint8_t* __restrict__ node_level,
     double4* __restrict__ node_geo, std::uint32_t* __restrict__ leaf_of) {
@@ -192,10 +199,13 @@ __global__ __launch_bounds__(kBlock) void tree_nodes(
         const unsigned dk = d[k], lleaf = leaf_level[k];
         const std::uint32_t base = first_node[k];
         const std::uint64_t key = codes[k];
+        const std::uint32_t seg = seg_of != nullptr ? seg_of[k] : 0u;
+        const std::uint32_t seg_end = seg_start != nullptr ? seg_start[seg + 1] : n;
+        const double* cube = cubes + 4 * static_cast<std::size_t>(seg);
         // From the leaf upwards: each cell's end bounds the next search from below.
         std::uint32_t end = k + 1;
         for (int l = static_cast<int>(lleaf); l >= static_cast<int>(dk); --l) {
-            end = prefix_end(codes, n, end, key, static_cast<unsigned>(l));
+            end = prefix_end(codes, seg_end, end, key, static_cast<unsigned>(l));
             double size = cube[3];
             double c[3] = {cube[0] + size / 2, cube[1] + size / 2, cube[2] + size / 2};
             for (int s = 1; s <= l; ++s) {
@@ -235,7 +245,8 @@ __global__ __launch_bounds__(kBlock) void tree_nodes(
 __global__ __launch_bounds__(kBlock) void tree_gather(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ m, const std::uint32_t* __restrict__ index, std::uint32_t n,
-    const double* __restrict__ cube, const std::uint32_t* __restrict__ leaf_of,
+    const double* __restrict__ cubes, const std::uint32_t* __restrict__ seg_of,
+    const std::uint32_t* __restrict__ leaf_of,
     const double4* __restrict__ node_geo, double4* __restrict__ sorted,
     float4* __restrict__ root_rel, float4* __restrict__ leaf_rel) {
     const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -243,6 +254,7 @@ __global__ __launch_bounds__(kBlock) void tree_gather(
     const std::uint32_t src = index[i];
     const double px = x[src], py = y[src], pz = z[src], pm = m[src];
     sorted[i] = make_double4(px, py, pz, pm);
+    const double* cube = cubes + (seg_of != nullptr ? 4 * static_cast<std::size_t>(seg_of[i]) : 0);
     const double half = cube[3] / 2;
     const double c0x = cube[0] + half, c0y = cube[1] + half, c0z = cube[2] + half;
     root_rel[i] = make_float4(static_cast<float>(px - c0x), static_cast<float>(py - c0y),
@@ -307,12 +319,14 @@ __global__ __launch_bounds__(kBlock) void leaf_moments(
     const std::uint8_t* __restrict__ leaf_level_of_first, const std::uint32_t* __restrict__ node_begin,
     const std::uint32_t* __restrict__ node_count, const std::uint32_t* __restrict__ node_next,
     const double4* __restrict__ node_geo, const double4* __restrict__ sorted, double theta,
-    const double* __restrict__ cube, double4* __restrict__ moment, WalkNode* __restrict__ walk,
+    const double* __restrict__ cubes, const std::uint32_t* __restrict__ seg_of,
+    double4* __restrict__ moment, WalkNode* __restrict__ walk,
     double* __restrict__ quad64, QuadF* __restrict__ quadf) {
     const std::uint32_t M = counters[0];
     for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
         const std::uint32_t b = node_begin[node];
         if (node_level[node] != leaf_level_of_first[b]) continue;  // internal
+        const double* cube = cubes + (seg_of != nullptr ? 4 * static_cast<std::size_t>(seg_of[b]) : 0);
         const std::uint32_t e = b + node_count[node];
         double mass = 0, wx = 0, wy = 0, wz = 0;
         for (std::uint32_t i = b; i < e; ++i) {
@@ -351,14 +365,15 @@ __global__ __launch_bounds__(kBlock) void internal_moments(
     unsigned level, const std::uint32_t* __restrict__ counters,
     const std::uint8_t* __restrict__ node_level, const std::uint8_t* __restrict__ leaf_level_of_first,
     const std::uint32_t* __restrict__ node_begin, const std::uint32_t* __restrict__ node_next,
-    const double4* __restrict__ node_geo, double theta, const double* __restrict__ cube,
-    double4* __restrict__ moment, WalkNode* __restrict__ walk, double* __restrict__ quad64,
-    QuadF* __restrict__ quadf) {
+    const double4* __restrict__ node_geo, double theta, const double* __restrict__ cubes,
+    const std::uint32_t* __restrict__ seg_of, double4* __restrict__ moment,
+    WalkNode* __restrict__ walk, double* __restrict__ quad64, QuadF* __restrict__ quadf) {
     const std::uint32_t M = counters[0];
     for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
         if (node_level[node] != level) continue;
         const std::uint32_t b = node_begin[node];
         if (leaf_level_of_first[b] == level) continue;  // a leaf
+        const double* cube = cubes + (seg_of != nullptr ? 4 * static_cast<std::size_t>(seg_of[b]) : 0);
         const std::uint32_t end = node_next[node];
         double mass = 0, wx = 0, wy = 0, wz = 0, mx = 0, my = 0, mz = 0;
         std::uint32_t children = 0;
@@ -424,9 +439,12 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     const double4* __restrict__ node_geo,
     const float4* __restrict__ root_rel, const float4* __restrict__ leaf_rel,
     const std::uint32_t* __restrict__ leaf_of, const std::uint32_t* __restrict__ index,
-    const std::uint32_t* __restrict__ counters, std::uint32_t n, float eps2, double G,
+    const std::uint32_t* __restrict__ counters, const std::uint32_t* __restrict__ seg_of,
+    const std::uint32_t* __restrict__ seg_start, const std::uint32_t* __restrict__ first_node,
+    std::uint32_t n, float eps2, double G,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     unsigned long long* __restrict__ stats) {
+    constexpr std::uint32_t kDone = 0xffffffffu;
     const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < n;
     const std::uint32_t M = counters[0];
@@ -435,17 +453,25 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     const float4 oi = leaf_rel[ic];
     const std::uint32_t own = leaf_of[ic];
     const double4 ca = node_geo[own];
-    std::uint32_t mine = valid ? 0u : M;
+    // This lane's system: its nodes are [root, node_end) in DFS order.
+    const std::uint32_t seg = seg_of != nullptr ? seg_of[ic] : 0u;
+    std::uint32_t root = 0, node_end = M;
+    if (seg_start != nullptr) {
+        root = first_node[seg_start[seg]];
+        const std::uint32_t e = seg_start[seg + 1];
+        node_end = e >= n ? M : first_node[e];
+    }
+    std::uint32_t mine = valid ? root : kDone;
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     float px = 0.f, py = 0.f, pz = 0.f;
     unsigned long long pp = 0, pc = 0, visits = 0, steps = 0;
     std::uint32_t cur = __builtin_amdgcn_readfirstlane(wave_min_u32(mine));
-    while (cur < M) {
+    while (cur != kDone) {
         ++steps;
         const WalkNode nd = nodes[cur];
         // An escape index always lies past its node; the guard only makes a
         // corrupted tree end the walk instead of looping.
-        const std::uint32_t next = nd.b.y > cur ? nd.b.y : M;
+        const std::uint32_t next = nd.b.y > cur ? nd.b.y : node_end;
         bool leaf_pp = false;
         if (mine == cur) {
             ++visits;
@@ -513,6 +539,7 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
                 px = py = pz = 0.f;
             }
         }
+        if (mine >= node_end) mine = kDone;
         const std::uint32_t step = cur + 1;
         cur = (__ballot(mine == step) != 0) ? step : wave_min_u32(mine);
         cur = __builtin_amdgcn_readfirstlane(cur);
@@ -526,14 +553,27 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
         ay[dst] = G * (static_cast<double>(sy) - static_cast<double>(ky));
         az[dst] = G * (static_cast<double>(sz) - static_cast<double>(kz));
     }
-    pp = wave_sum_u64(pp);
-    pc = wave_sum_u64(pc);
-    visits = wave_sum_u64(visits);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&stats[0], pp);
-        atomicAdd(&stats[1], pc);
-        atomicAdd(&stats[2], visits);
-        atomicAdd(&stats[3], steps);
+    // Counts per system: one atomic per wave when the wave holds one system.
+    const std::uint32_t seg0 = __builtin_amdgcn_readfirstlane(seg);
+    if (__ballot(valid && seg != seg0) == 0) {
+        pp = wave_sum_u64(pp);
+        pc = wave_sum_u64(pc);
+        visits = wave_sum_u64(visits);
+        if ((threadIdx.x & 63) == 0) {
+            unsigned long long* st = stats + 4 * static_cast<std::size_t>(seg0);
+            atomicAdd(&st[0], pp);
+            atomicAdd(&st[1], pc);
+            atomicAdd(&st[2], visits);
+            atomicAdd(&st[3], steps);
+        }
+    } else {
+        if (valid) {
+            unsigned long long* st = stats + 4 * static_cast<std::size_t>(seg);
+            atomicAdd(&st[0], pp);
+            atomicAdd(&st[1], pc);
+            atomicAdd(&st[2], visits);
+        }
+        if ((threadIdx.x & 63) == 0) atomicAdd(&stats[4 * static_cast<std::size_t>(seg0) + 3], steps);
     }
 }
 
@@ -577,13 +617,15 @@ std::size_t node_bound(std::uint64_t n, std::uint64_t cap) {
     return static_cast<std::size_t>(std::min(by_levels, by_chains));
 }
 
-Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole, bool allocate) {
+Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole, std::uint32_t segments,
+              bool allocate) {
     // Every carved array is rounded up to 256 bytes: 21 arrays of padding.
     const std::size_t per_particle = 2 * n + 3 * 4 * n + 32 * n + 2 * 16 * n +
                                      4 * scan_scratch_words(n);
     const std::size_t per_quad = quadrupole ? 6 * sizeof(double) + sizeof(QuadF) : 0;
     const std::size_t per_node = nodes * (4 * 3 + 1 + 32 + 32 + sizeof(WalkNode) + per_quad);
-    const std::size_t bytes = per_particle + per_node + 21 * 256 + 64;
+    const std::size_t counters = 16 + 8 * 4 * static_cast<std::size_t>(segments);
+    const std::size_t bytes = per_particle + per_node + counters + 21 * 256 + 64;
     if (allocate) ctx->tree_work.reserve(bytes);
     char* p = static_cast<char*>(ctx->tree_work.ptr);
     Layout L{};
@@ -608,55 +650,66 @@ Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole,
         L.quadf = carve<QuadF>(p, nodes);
     }
     L.c32 = carve<std::uint32_t>(p, 4);
-    L.c64 = carve<unsigned long long>(p, 4);
+    L.c64 = carve<unsigned long long>(p, 4 * static_cast<std::size_t>(segments));
     return L;
 }
 
 }  // namespace
 
 void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) {
+    tree_evaluate_segments(ctx, a, SegmentView{}, nullptr, stats);
+}
+
+void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentView& segs,
+                            unsigned long long* seg_stats, orr_tree_stats* stats) {
     const std::uint64_t n = a.count;
     if (stats != nullptr) std::memset(stats, 0, sizeof(*stats));
     ctx->last_tree = orr_tree_stats{};
     ctx->tree_last_n = n;
+    ctx->tree_last_segments = segs.count;
     if (n == 0) return;
     if (n >= (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "tree: 2^31 or more particles");
     // octree.cpp:189-194 parameter correction.
     const double theta = std::clamp(a.opening_angle, 0.0, 1.0);
     const std::uint64_t cap64 = std::max<std::uint64_t>(a.leaf_capacity, 1);
     const std::uint32_t cap = static_cast<std::uint32_t>(std::min<std::uint64_t>(cap64, n));
-    const std::size_t bound = node_bound(n, cap);
+    const std::size_t bound = node_bound(n, cap) + 22 * static_cast<std::size_t>(segs.count);
     ctx->tree_last_bound = bound;
     ctx->tree_last_theta = theta;
     ctx->tree_last_quadrupole = a.quadrupole != 0;
-    const Layout L = layout(ctx, n, bound, a.quadrupole != 0, true);
+    const Layout L = layout(ctx, n, bound, a.quadrupole != 0, segs.count, true);
     const std::uint32_t n32 = static_cast<std::uint32_t>(n);
     const std::uint32_t blocks = (n32 + kBlock - 1) / kBlock;
     const std::uint32_t node_blocks = static_cast<std::uint32_t>(std::min<std::size_t>(4096, (bound + kBlock - 1) / kBlock));
+    const std::uint32_t* seg_of = segs.count > 1 ? segs.seg_of : nullptr;
+    const std::uint32_t* seg_start = segs.count > 1 ? segs.start : nullptr;
+    unsigned long long* counts64 = seg_stats != nullptr ? seg_stats : L.c64;
 
-    morton_enqueue(ctx, a.x, a.y, a.z, n);
+    morton_enqueue_segments(ctx, a.x, a.y, a.z, n, segs);
     const std::uint64_t* codes = ctx->order_codes.as<std::uint64_t>();
     const std::uint32_t* index = ctx->order_index.as<std::uint32_t>();
     const double* cube = ctx->order_bbox.as<double>();
     {
         PhaseTimer t(ctx, kBuild);
-        hipLaunchKernelGGL(clear_counters, dim3(1), dim3(64), 0, ctx->stream, L.c32, L.c64);
+        hipLaunchKernelGGL(clear_counters, dim3(1), dim3(kBlock), 0, ctx->stream, L.c32,
+                           seg_stats != nullptr ? nullptr : L.c64, 4 * segs.count);
         ORR_LAUNCHED("clear_counters");
-        hipLaunchKernelGGL(tree_levels, dim3(blocks), dim3(kBlock), 0, ctx->stream, codes, n32, L.d);
+        hipLaunchKernelGGL(tree_levels, dim3(blocks), dim3(kBlock), 0, ctx->stream, codes, seg_of, n32,
+                           L.d);
         ORR_LAUNCHED("tree_levels");
         hipLaunchKernelGGL(tree_chain, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.d, n32, cap,
                            L.chain, L.leaf_level);
         ORR_LAUNCHED("tree_chain");
         exclusive_scan_u32(ctx, L.chain, L.first_node, n, L.scan_scratch, &L.c32[0]);
         hipLaunchKernelGGL(tree_nodes, dim3(blocks), dim3(kBlock), 0, ctx->stream, codes, L.d,
-                           L.leaf_level, L.chain, L.first_node, n32, cap, cube, L.c32, L.begin,
-                           L.count, L.next, L.level, L.geo, L.leaf_of);
+                           L.leaf_level, L.chain, L.first_node, n32, cap, seg_of, seg_start, cube,
+                           L.c32, L.begin, L.count, L.next, L.level, L.geo, L.leaf_of);
         ORR_LAUNCHED("tree_nodes");
     }
     {
         PhaseTimer t(ctx, kGather);
         hipLaunchKernelGGL(tree_gather, dim3(blocks), dim3(kBlock), 0, ctx->stream, a.x, a.y, a.z,
-                           a.m, index, n32, cube, L.leaf_of, L.geo, L.sorted, L.root_rel,
+                           a.m, index, n32, cube, seg_of, L.leaf_of, L.geo, L.sorted, L.root_rel,
                            L.leaf_rel);
         ORR_LAUNCHED("tree_gather");
     }
@@ -664,12 +717,13 @@ void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) 
         PhaseTimer t(ctx, kMoments);
         hipLaunchKernelGGL(leaf_moments, dim3(node_blocks), dim3(kBlock), 0, ctx->stream, L.c32,
                            L.level, L.leaf_level, L.begin, L.count, L.next, L.geo, L.sorted, theta,
-                           cube, L.moment, L.walk, L.quad64, L.quadf);
+                           cube, seg_of, L.moment, L.walk, L.quad64, L.quadf);
         ORR_LAUNCHED("leaf_moments");
         for (int level = static_cast<int>(kLevels) - 1; level >= 0; --level) {
             hipLaunchKernelGGL(internal_moments, dim3(node_blocks), dim3(kBlock), 0, ctx->stream,
                                static_cast<unsigned>(level), L.c32, L.level, L.leaf_level, L.begin,
-                               L.next, L.geo, theta, cube, L.moment, L.walk, L.quad64, L.quadf);
+                               L.next, L.geo, theta, cube, seg_of, L.moment, L.walk, L.quad64,
+                               L.quadf);
             ORR_LAUNCHED("internal_moments");
         }
     }
@@ -681,31 +735,50 @@ void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) 
         auto walk = safe ? (quad ? tree_walk<true, true> : tree_walk<true, false>)
                          : (quad ? tree_walk<false, true> : tree_walk<false, false>);
         hipLaunchKernelGGL(walk, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.walk, L.quadf, L.geo,
-                           L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, n32, eps2,
-                           a.gravitational_constant, a.ax, a.ay, a.az, L.c64);
+                           L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, seg_of, seg_start,
+                           L.first_node, n32, eps2, a.gravitational_constant, a.ax, a.ay, a.az,
+                           counts64);
         ORR_LAUNCHED("tree_walk");
     }
     if (stats != nullptr) {
         std::uint32_t c32[4];
-        unsigned long long c64[4];
+        std::vector<unsigned long long> c64(4 * static_cast<std::size_t>(segs.count));
         ORR_HIP(hipMemcpyAsync(c32, L.c32, sizeof(c32), hipMemcpyDeviceToHost, ctx->stream));
-        ORR_HIP(hipMemcpyAsync(c64, L.c64, sizeof(c64), hipMemcpyDeviceToHost, ctx->stream));
+        ORR_HIP(hipMemcpyAsync(c64.data(), counts64, c64.size() * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, ctx->stream));
         ctx->sync();
         stats->node_count = c32[0];
         stats->leaf_count = c32[1];
         stats->depth = c32[2];
-        stats->particle_particle = c64[0];
-        stats->particle_cell = c64[1];
-        stats->node_visits = c64[2];
+        for (std::uint32_t sgm = 0; sgm < segs.count; ++sgm) {
+            stats->particle_particle += c64[4 * sgm];
+            stats->particle_cell += c64[4 * sgm + 1];
+            stats->node_visits += c64[4 * sgm + 2];
+        }
         ctx->last_tree = *stats;
     }
+}
+
+void tree_copy_segment_counts(orr_ctx* ctx, std::uint64_t* out) {
+    const std::uint64_t n = ctx->tree_last_n;
+    const std::size_t words = 4 * static_cast<std::size_t>(ctx->tree_last_segments);
+    if (n == 0 || ctx->tree_work.ptr == nullptr) {
+        for (std::size_t k = 0; k < words; ++k) out[k] = 0;
+        return;
+    }
+    const Layout L = layout(ctx, n, ctx->tree_last_bound, ctx->tree_last_quadrupole,
+                            ctx->tree_last_segments, false);
+    static_assert(sizeof(unsigned long long) == sizeof(std::uint64_t));
+    ORR_HIP(hipMemcpyAsync(out, L.c64, words * sizeof(std::uint64_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ctx->sync();
 }
 
 void tree_copy_quadrupoles(orr_ctx* ctx, double* out) {
     const std::uint64_t n = ctx->tree_last_n;
     if (n == 0 || ctx->tree_work.ptr == nullptr) return;
     if (!ctx->tree_last_quadrupole) fail(ORR_ERR_INVALID_ARGUMENT, "the last tree had no quadrupoles");
-    const Layout L = layout(ctx, n, ctx->tree_last_bound, true, false);
+    const Layout L = layout(ctx, n, ctx->tree_last_bound, true, ctx->tree_last_segments, false);
     std::uint32_t M = 0;
     ORR_HIP(hipMemcpyAsync(&M, L.c32, sizeof(M), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
@@ -718,7 +791,7 @@ void tree_copy_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double*
                      std::uint64_t* next, std::uint64_t* first, std::uint64_t* count) {
     const std::uint64_t n = ctx->tree_last_n;
     if (n == 0 || ctx->tree_work.ptr == nullptr) return;
-    const Layout L = layout(ctx, n, ctx->tree_last_bound, ctx->tree_last_quadrupole, false);
+    const Layout L = layout(ctx, n, ctx->tree_last_bound, ctx->tree_last_quadrupole, ctx->tree_last_segments, false);
     std::uint32_t M = 0;
     ORR_HIP(hipMemcpyAsync(&M, L.c32, sizeof(M), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();

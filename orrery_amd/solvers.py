@@ -501,3 +501,108 @@ class DeviceSystem:
                                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return dict(kinetic=out[0], potential=out[1], total=out[0] + out[1],
                     momentum=out[2:5].copy(), angular_momentum=out[5:8].copy())
+
+
+@dataclass
+class BatchSystem:
+    """One system of a batch: size, solver (-1 by size, 0 direct, 1 Barnes-Hut),
+    softening length, G and timestep."""
+    count: int
+    softening: float
+    timestep: float
+    solver: int = -1
+    gravitational_constant: float = 1.0
+
+    def native(self) -> C.OrrBatchSystem:
+        return C.OrrBatchSystem(int(self.count), int(self.solver), 0,
+                                float(self.softening) * float(self.softening),
+                                float(self.gravitational_constant), float(self.timestep))
+
+
+@dataclass
+class BatchConfig:
+    """Batch-wide settings; 0 for leaf_capacity / tiles_per_item leaves them to
+    DeviceBatch.autotune (16 until then)."""
+    integrator: str = "velocity-verlet"
+    opening_angle: float = 0.5
+    leaf_capacity: int = 16
+    quadrupole: bool = False
+    tiles_per_item: int = 16
+    direct_below: int = 4096
+    use_graph: bool = True
+
+    def native(self) -> C.OrrBatchConfig:
+        return C.OrrBatchConfig(INTEGRATORS[self.integrator], int(bool(self.quadrupole)),
+                                float(self.opening_angle), int(self.leaf_capacity),
+                                int(self.tiles_per_item), int(self.direct_below),
+                                int(bool(self.use_graph)))
+
+
+class DeviceBatch:
+    """Many independent systems resident in HBM, stepped together (one launch
+    sequence per step for all of them, optionally replayed as a hipGraph)."""
+
+    def __init__(self, ctx: Context, systems, config: BatchConfig):
+        self.ctx = ctx
+        self.systems = list(systems)
+        arr = (C.OrrBatchSystem * max(1, len(self.systems)))(*(s.native() for s in self.systems))
+        cfg = config.native()
+        h = ctypes.c_void_p()
+        C.check(C.load().orr_batch_create(ctx.handle, arr, len(self.systems), ctypes.byref(cfg),
+                                          ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.check(C.load().orr_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        out = C.OrrBatchInfo()
+        C.check(C.load().orr_batch_info_get(self.handle, ctypes.byref(out)))
+        return {name: getattr(out, name) for name, _ in C.OrrBatchInfo._fields_}
+
+    def upload(self, system: int, x, y, z, vx, vy, vz, m) -> None:
+        arrs = [DeviceSystem._p(a) for a in (x, y, z, vx, vy, vz, m)]
+        ptrs = [None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in arrs]
+        C.check(C.load().orr_batch_upload(self.handle, int(system), *ptrs))
+
+    def download(self, system: int) -> dict:
+        n = int(self.systems[system].count)
+        out = [np.zeros(n) for _ in range(9)]
+        C.check(C.load().orr_batch_download(self.handle, int(system),
+                                            *(a.ctypes.data_as(ctypes.c_void_p) for a in out)))
+        return dict(x=out[0], y=out[1], z=out[2], vx=out[3], vy=out[4], vz=out[5],
+                    ax=out[6], ay=out[7], az=out[8])
+
+    def copy_system(self, system: int, source: "DeviceBatch", source_system: int) -> None:
+        C.check(C.load().orr_batch_copy_system(self.handle, int(system), source.handle,
+                                               int(source_system)))
+
+    def refresh(self) -> None:
+        C.check(C.load().orr_batch_refresh(self.handle))
+
+    def step(self, steps: int = 1) -> None:
+        C.check(C.load().orr_batch_step(self.handle, int(steps)))
+
+    def autotune(self) -> dict:
+        C.check(C.load().orr_batch_autotune(self.handle))
+        return self.info()
+
+    def diagnostics(self, system: int) -> dict:
+        out = np.zeros(8)
+        C.check(C.load().orr_batch_diagnostics(self.handle, int(system),
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return dict(kinetic=out[0], potential=out[1], total=out[0] + out[1],
+                    momentum=out[2:5].copy(), angular_momentum=out[5:8].copy())
+
+    def counts(self, system: int) -> tuple[int, int]:
+        out = (ctypes.c_uint64 * 2)()
+        C.check(C.load().orr_batch_counts(self.handle, int(system), out))
+        return int(out[0]), int(out[1])
