@@ -1,20 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X force engine on Orrery's headline direct workload.
+"""Benchmark of the MI355X force engine on Orrery's headline workloads.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu-baseline]
 
-Workload (direct O(N^2), one GPU): a Plummer sphere of N = 262,144 equal-mass
-particles (total mass 1, G = 1, scale radius a = 1, radii by inverse CDF
-truncated at r < 10 a, Aarseth-rejection speeds, seed 1), Plummer softening
-eps = 0.005, Yoshida-4 with dt = 5e-4. A step is one Yoshida-4 step on
-device-resident FP64 state: three force evaluations (FP32 pairwise terms,
+Headline (``value``), direct O(N^2) on one GPU: a Plummer sphere of
+N = 262,144 equal-mass particles (total mass 1, G = 1, scale radius a = 1, radii
+by inverse CDF truncated at r < 10 a, rejection-sampled speeds, seed 1), Plummer
+softening eps = 0.005, Yoshida-4 with dt = 5e-4. A step is one Yoshida-4 step
+on device-resident FP64 state: three force evaluations (FP32 pairwise terms,
 compensated accumulation) and the FP64 kick/drift updates around them.
+``value`` is interactions per second over the K timed steps, N(N-1) per
+evaluation, inputs resident in HBM. ``roofline`` is for the dominant kernel
+(direct_tiles) from HIP events recorded on the engine's own stream.
 
-Prints one JSON line. ``value`` is interactions per second over the K timed
-steps, N(N-1) per evaluation, inputs resident in HBM. ``roofline`` is for the
-dominant kernel (direct_tiles) from HIP events recorded on the engine's own
-stream; ``cpu_baseline`` times the reference's threaded AVX2 FP64 kernel on
-this host's cores over a bounded sample of targets of the same input.
+Further legs in the same JSON line:
+* ``barnes_hut`` -- two-disc collision at N = 2,097,152, theta 0.5, leaf 16;
+* ``batched``    -- 512 Plummer spheres, N log-uniform in [64, 16384], eps 0.02,
+  direct below 4096 particles and Barnes-Hut (theta 0.6) above, velocity Verlet
+  dt = 1e-3, one batch step per step replayed as a hipGraph, against the same
+  systems stepped one by one;
+* ``cpu_baseline`` and the ``force_error`` figures -- the CPU leg: the
+  reference's threaded AVX2 FP64 kernel (or the oracle's restatement) timed on
+  this host's cores over a bounded sample of the same input, and the FP64
+  Neumaier direct sums the GPU accelerations are checked against.
+
+Inputs come from orrery_amd.workloads (seeded numpy generators of the same models).
 """
 from __future__ import annotations
 
@@ -29,6 +39,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
+METRIC = "direct interactions/s N=256k; BH ms/eval N=2M; served p50/p99 latency"
 FLOP_PER_INTERACTION = 20          # the reference's convention (docs/performance.md:103-109)
 FP32_PEAK_TFLOPS = 157.3           # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
@@ -47,47 +58,87 @@ def cpu_threads() -> int:
     return max(1, min(16, n))
 
 
-def cpu_baseline(state, eps: float, seconds: float = 10.0) -> dict:
-    """Time the reference's AVX2 FP64 direct kernel (or the oracle's restatement)."""
-    import ctypes
-    import oracle
-    from oracle import _ptr, _u64
+class CpuLeg:
+    """Everything this benchmark runs on the host CPU: the reference's own
+    kernels (oracle/_ref, else the oracle's restatement) timed as the baseline,
+    and the FP64 reference forces the GPU results are checked against. None of
+    it is part of the measured GPU path."""
 
-    threads = cpu_threads()
-    n = state.n
-    ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
-    ref = oracle.load_reference()
-    if ref is not None:
-        kind = "reference"
+    def __init__(self):
+        import oracle
+        self.oracle = oracle
+        self.ref = oracle.load_reference()
+        self.threads = cpu_threads()
 
-        def run(tb, te):
-            ref.ref_direct_targets(_u64(n), _ptr(state.x), _ptr(state.y), _ptr(state.z),
-                                   _ptr(state.m), ctypes.c_double(eps), ctypes.c_int(1),
-                                   ctypes.c_uint(threads), _u64(tb), _u64(te),
-                                   _ptr(ax), _ptr(ay), _ptr(az))
-    else:
-        kind = "port"
-        lib = oracle.load()
+    def fp64_errors(self, state, eps: float, acc, targets) -> dict:
+        o = self.oracle
+        rx, ry, rz = o.reference_acceleration(state.x, state.y, state.z, state.m, eps * eps,
+                                              targets=targets.astype(np.uint64), threads=self.threads)
+        i = targets.astype(np.int64)
+        worst, rms = o.error_summary(o.relative_errors(acc[0][i], acc[1][i], acc[2][i], rx, ry, rz))
+        return {"rms_relative": rms, "max_relative": worst, "targets": int(len(targets)),
+                "reference": "FP64 Neumaier direct sum"}
 
-        def run(tb, te):
-            lib.orc_direct(_u64(n), _ptr(state.x), _ptr(state.y), _ptr(state.z), _ptr(state.m),
-                           ctypes.c_double(eps * eps), ctypes.c_double(1.0), ctypes.c_int(1),
-                           ctypes.c_uint(threads), _u64(tb), _u64(te), _ptr(ax), _ptr(ay), _ptr(az))
+    def direct_baseline(self, state, eps: float, seconds: float = 10.0) -> dict:
+        import ctypes
+        from oracle import _ptr, _u64
+        n = state.n
+        ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
+        if self.ref is not None:
+            kind = "reference"
 
-    probe = min(n, 256)
-    run(0, probe)  # warm-up
-    t0 = time.perf_counter()
-    run(0, probe)
-    dt = time.perf_counter() - t0
-    rate = probe * (n - 1) / max(dt, 1e-9)
-    targets = int(min(n, max(probe, seconds * rate / max(1, n - 1))))
-    t0 = time.perf_counter()
-    run(0, targets)
-    dt = time.perf_counter() - t0
-    inter = targets * (n - 1)
-    return {"value": inter / dt, "unit": "interactions/s", "cores": threads, "kind": kind,
-            "sample": f"{targets} of {n} targets x all {n} sources, FP64 AVX2 kernel with the "
-                      f"work-stealing executor, {dt:.1f} s"}
+            def run(tb, te):
+                self.ref.ref_direct_targets(_u64(n), _ptr(state.x), _ptr(state.y), _ptr(state.z),
+                                            _ptr(state.m), ctypes.c_double(eps), ctypes.c_int(1),
+                                            ctypes.c_uint(self.threads), _u64(tb), _u64(te),
+                                            _ptr(ax), _ptr(ay), _ptr(az))
+        else:
+            kind = "port"
+            lib = self.oracle.load()
+
+            def run(tb, te):
+                lib.orc_direct(_u64(n), _ptr(state.x), _ptr(state.y), _ptr(state.z), _ptr(state.m),
+                               ctypes.c_double(eps * eps), ctypes.c_double(1.0), ctypes.c_int(1),
+                               ctypes.c_uint(self.threads), _u64(tb), _u64(te), _ptr(ax), _ptr(ay),
+                               _ptr(az))
+
+        probe = min(n, 256)
+        run(0, probe)  # warm-up
+        t0 = time.perf_counter()
+        run(0, probe)
+        dt = time.perf_counter() - t0
+        rate = probe * (n - 1) / max(dt, 1e-9)
+        targets = int(min(n, max(probe, seconds * rate / max(1, n - 1))))
+        t0 = time.perf_counter()
+        run(0, targets)
+        dt = time.perf_counter() - t0
+        inter = targets * (n - 1)
+        return {"value": inter / dt, "unit": "interactions/s", "cores": self.threads, "kind": kind,
+                "sample": f"{targets} of {n} targets x all {n} sources, FP64 AVX2 kernel with the "
+                          f"work-stealing executor, {dt:.1f} s"}
+
+    def barnes_hut_baseline(self, state, eps: float, theta: float, cap: int) -> dict:
+        import ctypes
+        from oracle import _ptr, _u64, _u64p
+        n = state.n
+        ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
+        counts = np.zeros(2, dtype=np.uint64)
+        t0 = time.perf_counter()
+        if self.ref is not None:
+            kind = "reference"
+            self.ref.ref_barnes_hut(_u64(n), _ptr(state.x), _ptr(state.y), _ptr(state.z),
+                                    _ptr(state.m), ctypes.c_double(eps), ctypes.c_double(theta),
+                                    _u64(cap), ctypes.c_int(0), ctypes.c_int(1),
+                                    ctypes.c_uint(self.threads), _ptr(ax), _ptr(ay), _ptr(az),
+                                    _ptr(counts, _u64p))
+        else:
+            kind = "port"
+            self.oracle.barnes_hut(state.x, state.y, state.z, state.m, eps * eps, theta, cap,
+                                   threads=self.threads)
+        dt = time.perf_counter() - t0
+        return {"value": dt * 1e3, "unit": "ms/evaluation", "cores": self.threads, "kind": kind,
+                "sample": f"N={n} two-disc collision (same generator), theta {theta}, leaf {cap}, "
+                          f"the reference's threaded BarnesHutSolver"}
 
 
 def measured_traffic(n: int):
@@ -113,19 +164,15 @@ def measure_ceilings(ctx) -> dict:
             "hbm_copy_GBs": copy / 1e9, "copy_bytes": 2 * (1 << 30), "copy_ms": copy_ms}
 
 
-def bench_barnes_hut(E, oracle, ctx, n: int, evals: int, error_targets: int, cpu: bool) -> dict:
+def bench_barnes_hut(E, W, ctx, n: int, evals: int, error_targets: int, cpu) -> dict:
     """Barnes-Hut on the two-disc collision: N particles, theta 0.5, leaf 16."""
-    import ctypes
-
     eps, theta, cap = 0.01, 0.5, 16
     log(f"[bench] Barnes-Hut: generating two-disc collision N={n}")
-    p = oracle.disc_collision(n, 1)
+    p = W.disc_collision(n, 1)
     d = [E.DeviceArray.from_host(ctx, a) for a in (p.x, p.y, p.z, p.m)]
     out = [E.DeviceArray(ctx, n) for _ in range(3)]
     rng = np.random.default_rng(2)
-    tgt = np.sort(rng.choice(n, size=min(n, error_targets), replace=False)).astype(np.uint64)
-    ref = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps, targets=tgt, threads=cpu_threads())
-    i = tgt.astype(np.int64)
+    tgt = np.sort(rng.choice(n, size=min(n, error_targets), replace=False))
 
     def run(quadrupole: bool):
         s = E.HipTreeSolver(E.TreeParameters(theta, cap, quadrupole=quadrupole), E.Softening(eps), context=ctx)
@@ -140,11 +187,11 @@ def bench_barnes_hut(E, oracle, ctx, n: int, evals: int, error_targets: int, cpu
         phases = {k: v[0] / evals for k, v in ctx.timings().items()}
         ctx.set_timing(False)
         acc = [a.to_host() for a in out]
-        err = oracle.error_summary(oracle.relative_errors(acc[0][i], acc[1][i], acc[2][i], *ref))
+        err = cpu.fp64_errors(p, eps, acc, tgt) if cpu is not None else None
         return s, wall, phases, err
 
-    mono_s, mono_wall, mono_phases, (mono_worst, mono_rms) = run(False)
-    s, wall, phases, (worst, rms) = run(True)
+    mono_s, mono_wall, mono_phases, mono_err = run(False)
+    s, wall, phases, err = run(True)
     c = s.interaction_count()
     tree = s.tree_summary()
     M = tree["node_count"]
@@ -160,42 +207,138 @@ def bench_barnes_hut(E, oracle, ctx, n: int, evals: int, error_targets: int, cpu
         "n": n, "opening_angle": theta, "leaf_capacity": cap, "softening": eps, "quadrupole": True,
         "ms_per_evaluation": wall * 1e3,
         "monopole": {"ms_per_evaluation": mono_wall * 1e3, "phases_ms": mono_phases,
-                     "force_error": {"rms_relative": mono_rms, "max_relative": mono_worst}},
+                     "force_error": mono_err},
         "phases_ms": phases,
         "interactions_per_evaluation": {"particle_particle": pp, "particle_cell": pc},
         "walk_interactions_per_s": (pp + pc) / (phases.get("walk", 0.0) * 1e-3) if phases.get("walk") else None,
         "tree": tree,
-        "force_error": {"rms_relative": rms, "max_relative": worst, "targets": int(len(tgt)),
-                        "reference": "FP64 Neumaier direct sum"},
+        "force_error": err,
         "streaming_stages": {"algorithmic_bytes": alg_bytes, "ms": build_ms,
                              "achieved_GBs": alg_bytes / (build_ms * 1e-3) / 1e9 if build_ms else None,
                              "peak_GBs": HBM_PEAK_GBS},
     }
     for a in d + out:
         a.release()
-    if cpu:
-        ref_lib = oracle.load_reference()
-        sub = 262144
-        q = oracle.disc_collision(sub, 1)
-        ax, ay, az = np.zeros(sub), np.zeros(sub), np.zeros(sub)
-        counts = np.zeros(2, dtype=np.uint64)
-        from oracle import _ptr, _u64, _u64p
-        t0 = time.perf_counter()
-        if ref_lib is not None:
-            kind = "reference"
-            ref_lib.ref_barnes_hut(_u64(sub), _ptr(q.x), _ptr(q.y), _ptr(q.z), _ptr(q.m),
-                                   ctypes.c_double(eps), ctypes.c_double(theta), _u64(cap), ctypes.c_int(0),
-                                   ctypes.c_int(1), ctypes.c_uint(cpu_threads()), _ptr(ax), _ptr(ay), _ptr(az),
-                                   _ptr(counts, _u64p))
-        else:
-            kind = "port"
-            oracle.barnes_hut(q.x, q.y, q.z, q.m, eps * eps, theta, cap, threads=cpu_threads())
-        dt = time.perf_counter() - t0
-        out_d["cpu_baseline"] = {"value": dt * 1e3, "unit": "ms/evaluation", "cores": cpu_threads(),
-                                 "kind": kind,
-                                 "sample": f"N={sub} two-disc collision (same generator), theta 0.5, "
-                                           f"leaf 16, the reference's threaded BarnesHutSolver"}
+    if cpu is not None:
+        out_d["cpu_baseline"] = cpu.barnes_hut_baseline(W.disc_collision(262144, 1), eps, theta, cap)
     return out_d
+
+
+def bench_batched(E, W, ctx, count: int, steps: int, alone_steps: int, cpu) -> dict:
+    """512 Plummer spheres of uneven N in one batch against the same systems one by one."""
+    eps, dt, theta, below, seed = 0.02, 1e-3, 0.6, 4096, 4
+    log(f"[bench] batched: generating {count} Plummer spheres")
+    ns = W.batched_sizes(count, 64, 16384, seed)
+    spheres = W.batched_spheres(count, 64, 16384, seed)
+    systems = [E.BatchSystem(n, eps, dt) for n in ns]
+
+    def fill(b):
+        for k, s in enumerate(spheres):
+            b.upload(k, *s.arrays())
+
+    cfg = E.BatchConfig(integrator="velocity-verlet", opening_angle=theta, leaf_capacity=0,
+                        tiles_per_item=0, direct_below=below, use_graph=True)
+    b = E.DeviceBatch(ctx, systems, cfg)
+    fill(b)
+    t0 = time.perf_counter()
+    info = b.autotune()
+    tune_s = time.perf_counter() - t0
+    log(f"[bench] batched: autotuned tiles_per_item={info['tiles_per_item']} "
+        f"leaf_capacity={info['leaf_capacity']} in {tune_s:.2f} s")
+    b.refresh()
+    before = [b.diagnostics(k) for k in range(count)]
+    b.step(2)  # the first step runs eagerly and captures the graph
+    t0 = time.perf_counter()
+    b.step(steps)
+    wall = time.perf_counter() - t0
+    info = b.info()
+    tree_pairs = 0
+    tree_cells = 0
+    for k, n in enumerate(ns):
+        if n >= below:
+            pp, pc = b.counts(k)
+            tree_pairs += pp
+            tree_cells += pc
+    per_eval = info["direct_pairs"] + tree_pairs + tree_cells
+    after = [b.diagnostics(k) for k in range(count)]
+    drift = [abs((a["total"] - e["total"]) / e["total"]) for a, e in zip(after, before)]
+
+    # Accuracy of the last evaluation on a sample of systems, vs FP64.
+    errors = None
+    if cpu is not None:
+        rng = np.random.default_rng(9)
+        picks = sorted(set(int(k) for k in rng.choice(count, size=12, replace=False)))
+        worst_d = rms_d = worst_t = rms_t = 0.0
+        for k in picks:
+            st = b.download(k)
+            state = W.Workload(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"], spheres[k].m)
+            tg = np.sort(rng.choice(ns[k], size=min(ns[k], 256), replace=False))
+            e = cpu.fp64_errors(state, eps, (st["ax"], st["ay"], st["az"]), tg)
+            if ns[k] < below:
+                worst_d, rms_d = max(worst_d, e["max_relative"]), max(rms_d, e["rms_relative"])
+            else:
+                worst_t, rms_t = max(worst_t, e["max_relative"]), max(rms_t, e["rms_relative"])
+        errors = {"systems_sampled": len(picks), "targets_per_system": 256,
+                  "direct": {"max_rms_relative": rms_d, "max_relative": worst_d},
+                  "barnes_hut": {"max_rms_relative": rms_t, "max_relative": worst_t},
+                  "reference": "FP64 Neumaier direct sum"}
+    b.close()
+
+    # The same batch without the graph: per-launch host overhead included.
+    cfg_eager = E.BatchConfig(integrator="velocity-verlet", opening_angle=theta,
+                              leaf_capacity=info["leaf_capacity"],
+                              tiles_per_item=info["tiles_per_item"], direct_below=below,
+                              use_graph=False)
+    b = E.DeviceBatch(ctx, systems, cfg_eager)
+    fill(b)
+    b.refresh()
+    b.step(2)
+    eager_steps = max(1, min(steps, 20))
+    t0 = time.perf_counter()
+    b.step(eager_steps)
+    eager_ms = (time.perf_counter() - t0) * 1e3 / eager_steps
+    b.close()
+
+    # Each system alone: the plain single-system device path, one system at a time.
+    log(f"[bench] batched: stepping the {count} systems one by one ({alone_steps} steps each)")
+    alone_ms = 0.0
+    for k, s in enumerate(spheres):
+        solver = "direct" if ns[k] < below else "barnes-hut"
+        c = E.StepConfig(integrator="velocity-verlet", solver=solver, softening=eps,
+                         opening_angle=theta, leaf_capacity=info["leaf_capacity"])
+        one = E.DeviceSystem(ctx, ns[k])
+        one.upload(*s.arrays())
+        one.refresh(c)
+        one.step(c, dt, 1)
+        t0 = time.perf_counter()
+        one.step(c, dt, alone_steps)
+        alone_ms += (time.perf_counter() - t0) * 1e3 / alone_steps
+        one.close()
+
+    ms = wall * 1e3 / steps
+    return {
+        "workload": f"{count} Plummer spheres, N log-uniform in [64, 16384] (seed {seed}), "
+                    f"direct below {below}, Barnes-Hut theta {theta} above, velocity Verlet",
+        "systems": count, "particles": info["particles"],
+        "direct_systems": info["direct_systems"], "tree_systems": info["tree_systems"],
+        "softening": eps, "timestep": dt, "steps": steps,
+        "autotuned": {"tiles_per_item": info["tiles_per_item"],
+                      "leaf_capacity": info["leaf_capacity"], "seconds": tune_s},
+        "ms_per_step": ms, "steps_per_s": 1e3 / ms,
+        "interactions_per_step": {"direct_pairs": info["direct_pairs"],
+                                  "tree_particle_particle": tree_pairs,
+                                  "tree_particle_cell": tree_cells},
+        "interactions_per_s": per_eval / (ms * 1e-3),
+        "flop_rate_TFLOPs": FLOP_PER_INTERACTION * per_eval / (ms * 1e-3) / 1e12,
+        "eager_ms_per_step": eager_ms,
+        "one_by_one_ms_per_step": alone_ms,
+        "speedup_vs_one_by_one": alone_ms / ms,
+        "one_by_one": f"each system its own device-resident system (orr_system), stepped alone "
+                      f"({alone_steps} steps each, after one warm-up step)",
+        "energy_drift": {"max_abs_relative": max(drift), "median_abs_relative": float(np.median(drift)),
+                         "steps": steps + 2},
+        "force_error": errors,
+    }
 
 
 def main() -> int:
@@ -210,6 +353,11 @@ def main() -> int:
     ap.add_argument("--bh-n", type=int, default=2097152)
     ap.add_argument("--bh-evals", type=int, default=5)
     ap.add_argument("--no-barnes-hut", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--alone-steps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-direct", action="store_true", help="skip the headline direct leg")
     ap.add_argument("--probes-only", action="store_true",
                     help="run only the machine-ceiling probes (copy probe: counter calibration)")
     args = ap.parse_args()
@@ -222,8 +370,8 @@ def main() -> int:
                           "n_gpus": args.gpus}), flush=True)
         return 2
 
-    import oracle
     import orrery_amd as E
+    from orrery_amd import workloads as W
 
     if args.probes_only:
         ctx = E.Context(0)
@@ -231,86 +379,77 @@ def main() -> int:
         ctx.close()
         return 0
 
-    n, eps, dt = args.n, 0.005, 5e-4
-    log(f"[bench] generating Plummer N={n} seed={args.seed}")
-    state = oracle.plummer(n, args.seed, total_mass=1.0, scale_radius=1.0)
-
+    cpu = None if args.no_cpu_baseline else CpuLeg()
     ctx = E.Context(0)
     dev = E.describe(0)
     log(f"[bench] device {dev['name']} {dev['arch']} CUs={dev['compute_units']}")
-    sys_ = E.DeviceSystem(ctx, n)
-    sys_.upload(state.x, state.y, state.z, state.vx, state.vy, state.vz, state.m)
-    cfg = E.StepConfig(integrator="yoshida4", solver="direct", softening=eps)
-
-    # Force accuracy of one evaluation at the initial state vs the FP64 oracle.
-    log("[bench] refresh + accuracy sample")
-    sys_.refresh(cfg)
-    acc = sys_.download()
-    rng = np.random.default_rng(12345)
-    tgt = np.sort(rng.choice(n, size=min(n, args.error_targets), replace=False)).astype(np.uint64)
-    rx, ry, rz = oracle.reference_acceleration(state.x, state.y, state.z, state.m, eps * eps,
-                                               targets=tgt, threads=cpu_threads())
-    idx = tgt.astype(np.int64)
-    rel = oracle.relative_errors(acc["ax"][idx], acc["ay"][idx], acc["az"][idx], rx, ry, rz)
-    worst, rms = oracle.error_summary(rel)
-
-    log(f"[bench] warmup {args.warmup} steps")
-    sys_.step(cfg, dt, args.warmup)
-    ctx.set_timing(True)
-    ctx.reset_timings()
-    log(f"[bench] timing {args.steps} steps")
-    t0 = time.perf_counter()
-    sys_.step(cfg, dt, args.steps)           # returns after a stream synchronise
-    elapsed = time.perf_counter() - t0
-    timings = ctx.timings()
-    ctx.set_timing(False)
-
-    evals = 3 * args.steps
-    inter = evals * n * (n - 1)
-    value = inter / elapsed
-    k_ms, k_launches = timings.get("direct_kernel", (0.0, 0))
-    kernel_ms = k_ms / max(1, k_launches)
-    eval_ms = sum(timings.get(p, (0.0, 0))[0] for p in
-                  ("direct_prepare", "direct_kernel", "direct_reduce", "bbox", "encode", "sort")) / max(1, evals)
-    achieved = FLOP_PER_INTERACTION * n * (n - 1) / (kernel_ms * 1e-3) / 1e12
-
+    n, eps, dt = args.n, 0.005, 5e-4
     result = {
-        "metric": "direct interactions/s N=256k",
-        "value": value,
-        "unit": "interactions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / max(1, args.steps),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+        "metric": METRIC, "value": None, "unit": "interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 pairwise terms, fp64 state and update",
         "data": "synthetic Plummer sphere (seeded generator), device-resident",
         "config": {"workload": "direct O(N^2) Plummer, Yoshida-4, one GPU", "n": n, "softening": eps,
                    "timestep": dt, "integrator": "yoshida4", "evaluations_per_step": 3,
                    "seed": args.seed},
-        "ms_per_evaluation": eval_ms,
-        "kernel_ms_per_evaluation": kernel_ms,
-        "force_error": {"rms_relative": rms, "max_relative": worst, "targets": int(len(tgt)),
-                        "reference": "FP64 Neumaier direct sum"},
-        "phases_ms": {k: v[0] for k, v in timings.items()},
-        "roofline": {"bound": "fp32-valu", "kernel": "direct_tiles", "achieved": achieved,
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": measured_traffic(n),
-                     "note": "20 flop per interaction x N(N-1) per launch / mean launch time; "
-                             "traffic = calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch "
-                             "(profiles/direct_tiles_traffic.json)"},
     }
-    if not args.no_cpu_baseline:
-        log("[bench] CPU baseline")
-        result["cpu_baseline"] = cpu_baseline(state, eps)
-    else:
-        result["cpu_baseline"] = None
-    sys_.close()
+    if not args.no_direct:
+        log(f"[bench] generating Plummer N={n} seed={args.seed}")
+        state = W.plummer(n, args.seed)
+        sys_ = E.DeviceSystem(ctx, n)
+        sys_.upload(*state.arrays())
+        cfg = E.StepConfig(integrator="yoshida4", solver="direct", softening=eps)
+        log("[bench] refresh")
+        sys_.refresh(cfg)
+        acc = sys_.download()
+        log(f"[bench] warmup {args.warmup} steps")
+        sys_.step(cfg, dt, args.warmup)
+        ctx.set_timing(True)
+        ctx.reset_timings()
+        log(f"[bench] timing {args.steps} steps")
+        t0 = time.perf_counter()
+        sys_.step(cfg, dt, args.steps)           # returns after a stream synchronise
+        elapsed = time.perf_counter() - t0
+        timings = ctx.timings()
+        ctx.set_timing(False)
+        sys_.close()
+
+        evals = 3 * args.steps
+        value = evals * n * (n - 1) / elapsed
+        k_ms, k_launches = timings.get("direct_kernel", (0.0, 0))
+        kernel_ms = k_ms / max(1, k_launches)
+        eval_ms = sum(timings.get(p, (0.0, 0))[0] for p in
+                      ("direct_prepare", "direct_kernel", "direct_reduce", "bbox", "encode", "sort")) / max(1, evals)
+        achieved = FLOP_PER_INTERACTION * n * (n - 1) / (kernel_ms * 1e-3) / 1e12
+        result.update({
+            "value": value,
+            "ms_per_step": elapsed * 1e3 / max(1, args.steps),
+            "ms_per_evaluation": eval_ms,
+            "kernel_ms_per_evaluation": kernel_ms,
+            "phases_ms": {k: v[0] for k, v in timings.items()},
+            "roofline": {"bound": "fp32-valu", "kernel": "direct_tiles", "achieved": achieved,
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                         "traffic": measured_traffic(n),
+                         "note": "20 flop per interaction x N(N-1) per launch / mean launch time; "
+                                 "traffic = calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per "
+                                 "launch (profiles/direct_tiles_traffic.json)"},
+        })
+        if cpu is not None:
+            log("[bench] CPU leg: FP64 reference forces and the AVX2 baseline")
+            rng = np.random.default_rng(12345)
+            tgt = np.sort(rng.choice(n, size=min(n, args.error_targets), replace=False))
+            result["force_error"] = cpu.fp64_errors(state, eps, (acc["ax"], acc["ay"], acc["az"]), tgt)
+            result["cpu_baseline"] = cpu.direct_baseline(state, eps)
+        else:
+            result["force_error"] = None
+            result["cpu_baseline"] = None
     if not args.no_barnes_hut:
-        result["barnes_hut"] = bench_barnes_hut(E, oracle, ctx, args.bh_n, args.bh_evals,
-                                                args.error_targets, not args.no_cpu_baseline)
+        result["barnes_hut"] = bench_barnes_hut(E, W, ctx, args.bh_n, args.bh_evals,
+                                                args.error_targets, cpu)
+    if not args.no_batched:
+        result["batched"] = bench_batched(E, W, ctx, args.batch_systems, args.batch_steps,
+                                          args.alone_steps, cpu)
     log("[bench] ceilings")
     result["ceilings"] = measure_ceilings(ctx)
     ctx.close()
