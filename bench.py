@@ -300,9 +300,11 @@ def bench_batched(E, W, ctx, count: int, steps: int, alone_steps: int, cpu) -> d
     b.close()
 
     # Each system alone: the plain single-system device path, one system at a time.
-    log(f"[bench] batched: stepping the {count} systems one by one ({alone_steps} steps each)")
-    alone_ms = 0.0
-    for k, s in enumerate(spheres):
+    alone_ms = None
+    if alone_steps > 0:
+        log(f"[bench] batched: stepping the {count} systems one by one ({alone_steps} steps each)")
+        alone_ms = 0.0
+    for k, s in enumerate(spheres if alone_steps > 0 else []):
         solver = "direct" if ns[k] < below else "barnes-hut"
         c = E.StepConfig(integrator="velocity-verlet", solver=solver, softening=eps,
                          opening_angle=theta, leaf_capacity=info["leaf_capacity"])
@@ -332,7 +334,7 @@ def bench_batched(E, W, ctx, count: int, steps: int, alone_steps: int, cpu) -> d
         "flop_rate_TFLOPs": FLOP_PER_INTERACTION * per_eval / (ms * 1e-3) / 1e12,
         "eager_ms_per_step": eager_ms,
         "one_by_one_ms_per_step": alone_ms,
-        "speedup_vs_one_by_one": alone_ms / ms,
+        "speedup_vs_one_by_one": alone_ms / ms if alone_ms else None,
         "one_by_one": f"each system its own device-resident system (orr_system), stepped alone "
                       f"({alone_steps} steps each, after one warm-up step)",
         "energy_drift": {"max_abs_relative": max(drift), "median_abs_relative": float(np.median(drift)),

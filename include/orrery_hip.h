@@ -307,6 +307,73 @@ ORR_API int orr_batch_diagnostics(orr_batch* batch, uint32_t system, double out[
  * particle_cell (direct: N(N-1) and 0). */
 ORR_API int orr_batch_counts(orr_batch* batch, uint32_t system, uint64_t out[2]);
 
+/* ---- serving: a request loop with continuous batching --------------------
+ *
+ * No counterpart in the reference, which serves one Simulation per process
+ * (its web service runs the wasm build per client, wasm/orrery_wasm.h:113-200).
+ * Requests (an initial state, an integrator and a step count) are admitted
+ * against an HBM budget as earlier ones finish; admitted requests stay on the
+ * device until done and are stepped in batches (small systems of one
+ * integrator together, large ones alone), all batches concurrently, in
+ * scheduling rounds. A request's result does not depend on what else runs. */
+typedef struct orr_server orr_server;
+
+typedef struct {
+    uint64_t memory_budget;          /* bytes of HBM for admitted requests; 0: the device's
+                                        free memory at creation, at most 288 GB */
+    double round_ms;                 /* target duration of one scheduling round (0: 50 ms) */
+    uint32_t direct_below;           /* direct solver below this many particles (0: 4096) */
+    uint32_t alone_from;             /* requests this large get a batch of their own (0: 131072) */
+    double opening_angle;            /* Barnes-Hut requests (0: 0.5) */
+    uint32_t leaf_capacity;          /* 0: 16 */
+    uint32_t tiles_per_item;         /* 0: 16 */
+    int32_t quadrupole;
+    int32_t use_graph;
+} orr_server_config;
+
+typedef struct {
+    uint64_t count;
+    int32_t integrator;              /* 0 velocity Verlet, 1 Yoshida-4 */
+    int32_t reserved;
+    double softening_squared;
+    double gravitational_constant;
+    double timestep;
+    uint64_t steps;
+    /* Host arrays of `count` values, copied by orr_server_submit. */
+    const double *x, *y, *z, *vx, *vy, *vz, *m;
+} orr_request;
+
+typedef struct {
+    int32_t state;                   /* 0 queued, 1 running, 2 done */
+    int32_t reserved;
+    uint64_t steps_done;
+    double submitted_s, admitted_s, completed_s;   /* server clock, seconds */
+    double diagnostics_start[8];     /* at admission: {kinetic, potential, p, L} */
+    double diagnostics_end[8];       /* at completion */
+} orr_request_status;
+
+typedef struct {
+    uint64_t rounds, batch_builds, particle_steps, completed;
+    uint64_t resident_bytes, peak_resident_bytes, memory_budget;
+    uint32_t active, queued;
+    double busy_s;                   /* wall time inside rounds */
+} orr_server_stats;
+
+ORR_API int orr_server_create(orr_ctx* ctx, const orr_server_config* config, orr_server** out);
+ORR_API int orr_server_destroy(orr_server* server);
+/* Seconds since the server was created (the clock of the status times). */
+ORR_API int orr_server_now(orr_server* server, double* seconds);
+ORR_API int orr_server_submit(orr_server* server, const orr_request* request, uint64_t* id);
+/* One scheduling round: admit, (re)batch, step every batch concurrently,
+ * retire. `pending` gets the number of requests queued or running after it. */
+ORR_API int orr_server_round(orr_server* server, uint32_t* pending);
+ORR_API int orr_server_status(orr_server* server, uint64_t id, orr_request_status* out);
+/* Final state of a finished request (host arrays, null: skipped); then its
+ * host copy is released. */
+ORR_API int orr_server_result(orr_server* server, uint64_t id, double* x, double* y, double* z,
+                              double* vx, double* vy, double* vz);
+ORR_API int orr_server_stats_get(orr_server* server, orr_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

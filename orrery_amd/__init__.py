@@ -5,9 +5,9 @@ of include/orrery_hip.h). This package is its Python host; it has no CPU
 fallback and fails loudly when the library is not built.
 """
 from .solvers import (BatchConfig, BatchSystem, Context, DeviceArray, DeviceBatch, DeviceSystem,
-                      HipDirectSolver, HipTreeSolver, InteractionCount, Softening, StepConfig,
-                      TreeParameters, describe, device_count, morton_order)
+                      HipDirectSolver, HipTreeSolver, InteractionCount, Server, ServerConfig,
+                      Softening, StepConfig, TreeParameters, describe, device_count, morton_order)
 
 __all__ = ["BatchConfig", "BatchSystem", "Context", "DeviceArray", "DeviceBatch", "DeviceSystem",
-           "HipDirectSolver", "HipTreeSolver", "InteractionCount", "Softening", "StepConfig",
-           "TreeParameters", "describe", "device_count", "morton_order"]
+           "HipDirectSolver", "HipTreeSolver", "InteractionCount", "Server", "ServerConfig",
+           "Softening", "StepConfig", "TreeParameters", "describe", "device_count", "morton_order"]

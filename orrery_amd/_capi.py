@@ -94,6 +94,36 @@ class OrrBatchInfo(ctypes.Structure):
                 ("tree_particles", ctypes.c_uint64), ("direct_pairs", ctypes.c_uint64)]
 
 
+class OrrServerConfig(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_uint64), ("round_ms", ctypes.c_double),
+                ("direct_below", ctypes.c_uint32), ("alone_from", ctypes.c_uint32),
+                ("opening_angle", ctypes.c_double), ("leaf_capacity", ctypes.c_uint32),
+                ("tiles_per_item", ctypes.c_uint32), ("quadrupole", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32)]
+
+
+class OrrRequest(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("integrator", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("softening_squared", ctypes.c_double), ("gravitational_constant", ctypes.c_double),
+                ("timestep", ctypes.c_double), ("steps", ctypes.c_uint64),
+                ("x", _vp), ("y", _vp), ("z", _vp), ("vx", _vp), ("vy", _vp), ("vz", _vp), ("m", _vp)]
+
+
+class OrrRequestStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("reserved", ctypes.c_int32), ("steps_done", ctypes.c_uint64),
+                ("submitted_s", ctypes.c_double), ("admitted_s", ctypes.c_double),
+                ("completed_s", ctypes.c_double), ("diagnostics_start", ctypes.c_double * 8),
+                ("diagnostics_end", ctypes.c_double * 8)]
+
+
+class OrrServerStats(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_uint64), ("batch_builds", ctypes.c_uint64),
+                ("particle_steps", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("resident_bytes", ctypes.c_uint64), ("peak_resident_bytes", ctypes.c_uint64),
+                ("memory_budget", ctypes.c_uint64), ("active", ctypes.c_uint32),
+                ("queued", ctypes.c_uint32), ("busy_s", ctypes.c_double)]
+
+
 # (name, argtypes) of every entry point the header declares.
 SIGNATURES = {
     "orr_device_count": [ctypes.POINTER(ctypes.c_int32)],
@@ -136,6 +166,14 @@ SIGNATURES = {
     "orr_batch_autotune": [_vp],
     "orr_batch_diagnostics": [_vp, ctypes.c_uint32, _d],
     "orr_batch_counts": [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
+    "orr_server_create": [_vp, ctypes.POINTER(OrrServerConfig), ctypes.POINTER(_vp)],
+    "orr_server_destroy": [_vp],
+    "orr_server_now": [_vp, _d],
+    "orr_server_submit": [_vp, ctypes.POINTER(OrrRequest), ctypes.POINTER(ctypes.c_uint64)],
+    "orr_server_round": [_vp, ctypes.POINTER(ctypes.c_uint32)],
+    "orr_server_status": [_vp, ctypes.c_uint64, ctypes.POINTER(OrrRequestStatus)],
+    "orr_server_result": [_vp, ctypes.c_uint64] + [_vp] * 6,
+    "orr_server_stats_get": [_vp, ctypes.POINTER(OrrServerStats)],
 }
 
 _LIB = None

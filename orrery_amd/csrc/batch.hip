@@ -357,7 +357,9 @@ void capture(orr_batch* b) {
     b->graph_epoch = epoch;
 }
 
-void step(orr_batch* b, std::uint64_t steps) {
+// Enqueues `steps` steps on the batch's streams. Only the first step after a
+// change of plan or buffers waits (it runs eagerly, then the step is captured).
+void step_enqueue(orr_batch* b, std::uint64_t steps) {
     if (b->n == 0) return;
     for (std::uint64_t k = 0; k < steps; ++k) {
         if (b->cfg.use_graph == 0) {
@@ -374,6 +376,10 @@ void step(orr_batch* b, std::uint64_t steps) {
         sync_all(b);
         capture(b);
     }
+}
+
+void step(orr_batch* b, std::uint64_t steps) {
+    step_enqueue(b, steps);
     sync_all(b);
 }
 
@@ -445,6 +451,11 @@ void destroy(orr_batch* b) noexcept {
 }
 
 }  // namespace
+
+void batch_step_enqueue(orr_batch* b, std::uint64_t steps) { step_enqueue(b, steps); }
+void batch_wait(orr_batch* b) { sync_all(b); }
+hipStream_t batch_stream(orr_batch* b) { return b->dctx->stream; }
+
 }  // namespace orr
 
 using orr::guarded;

@@ -229,6 +229,13 @@ void tree_copy_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double*
                      double* acc, std::uint64_t* next, std::uint64_t* first,
                      std::uint64_t* count);
 
+// batch.hip, for the serving loop: enqueue steps of a batch on its own streams
+// without waiting (batches run concurrently), then wait for them.
+void batch_step_enqueue(orr_batch* b, std::uint64_t steps);
+void batch_wait(orr_batch* b);
+// The stream every step of the batch starts and ends on.
+hipStream_t batch_stream(orr_batch* b);
+
 // probes.hip: machine-ceiling probes; kind 0 fma, 1 rsq, 2 read, 3 copy.
 void probe(orr_ctx* ctx, int kind, std::uint64_t size, double* rate, double* ms);
 

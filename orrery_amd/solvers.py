@@ -606,3 +606,96 @@ class DeviceBatch:
         out = (ctypes.c_uint64 * 2)()
         C.check(C.load().orr_batch_counts(self.handle, int(system), out))
         return int(out[0]), int(out[1])
+
+
+@dataclass
+class ServerConfig:
+    """Serving-loop settings (0 / None: the engine's defaults)."""
+    memory_budget: int = 0
+    round_ms: float = 50.0
+    direct_below: int = 4096
+    alone_from: int = 131072
+    opening_angle: float = 0.5
+    leaf_capacity: int = 16
+    tiles_per_item: int = 16
+    quadrupole: bool = False
+    use_graph: bool = True
+
+    def native(self) -> C.OrrServerConfig:
+        return C.OrrServerConfig(int(self.memory_budget), float(self.round_ms), int(self.direct_below),
+                                 int(self.alone_from), float(self.opening_angle),
+                                 int(self.leaf_capacity), int(self.tiles_per_item),
+                                 int(bool(self.quadrupole)), int(bool(self.use_graph)))
+
+
+class Server:
+    """Continuous-batching request loop on one device (orr_server_*)."""
+
+    STATES = {0: "queued", 1: "running", 2: "done"}
+
+    def __init__(self, ctx: Context, config: ServerConfig | None = None):
+        self.ctx = ctx
+        cfg = (config or ServerConfig()).native()
+        h = ctypes.c_void_p()
+        C.check(C.load().orr_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self.counts = {}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.check(C.load().orr_server_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now(self) -> float:
+        t = ctypes.c_double()
+        C.check(C.load().orr_server_now(self.handle, ctypes.byref(t)))
+        return t.value
+
+    def submit(self, state, integrator: str, softening: float, timestep: float, steps: int,
+               gravitational_constant: float = 1.0) -> int:
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in state.arrays()]
+        req = C.OrrRequest(len(arrs[6]), INTEGRATORS[integrator], 0, float(softening) ** 2,
+                           float(gravitational_constant), float(timestep), int(steps),
+                           *(a.ctypes.data_as(ctypes.c_void_p) for a in arrs))
+        rid = ctypes.c_uint64()
+        C.check(C.load().orr_server_submit(self.handle, ctypes.byref(req), ctypes.byref(rid)))
+        self.counts[rid.value] = len(arrs[6])
+        return rid.value
+
+    def round(self) -> int:
+        pending = ctypes.c_uint32()
+        C.check(C.load().orr_server_round(self.handle, ctypes.byref(pending)))
+        return pending.value
+
+    def status(self, rid: int) -> dict:
+        st = C.OrrRequestStatus()
+        C.check(C.load().orr_server_status(self.handle, int(rid), ctypes.byref(st)))
+
+        def diag(a):
+            return dict(kinetic=a[0], potential=a[1], total=a[0] + a[1], momentum=list(a[2:5]),
+                        angular_momentum=list(a[5:8]))
+        return dict(state=self.STATES[st.state], steps_done=st.steps_done, submitted=st.submitted_s,
+                    admitted=st.admitted_s, completed=st.completed_s,
+                    start=diag(list(st.diagnostics_start)), end=diag(list(st.diagnostics_end)))
+
+    def result(self, rid: int) -> dict:
+        n = self.counts[rid]
+        out = [np.zeros(n) for _ in range(6)]
+        C.check(C.load().orr_server_result(self.handle, int(rid),
+                                           *(a.ctypes.data_as(ctypes.c_void_p) for a in out)))
+        return dict(zip(("x", "y", "z", "vx", "vy", "vz"), out))
+
+    def stats(self) -> dict:
+        st = C.OrrServerStats()
+        C.check(C.load().orr_server_stats_get(self.handle, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in C.OrrServerStats._fields_}
+
+    def drain(self) -> None:
+        while self.round() > 0:
+            pass

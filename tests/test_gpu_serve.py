@@ -1,0 +1,131 @@
+"""The serving loop (include/orrery_hip.h orr_server_*).
+
+Contract: requests that arrive while others run are admitted as memory frees
+up, run concurrently in batches, and each result is the same alone as in the
+mix (bit for bit: a system's arithmetic does not depend on its batch or on how
+its steps are split into rounds). Energy drift is compared with the oracle's
+FP64 integrator on the same initial state.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import orrery_amd as E
+from orrery_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+EPS = 0.01
+
+
+def requests():
+    # (workload, integrator, steps): direct and Barnes-Hut sizes, both integrators.
+    return [
+        (W.plummer(1000, 1), "velocity-verlet", 30),
+        (W.plummer(5000, 2), "yoshida4", 12),
+        (W.disc_collision(4000, 3), "velocity-verlet", 25),
+        (W.plummer(2500, 4), "yoshida4", 9),
+        (W.disc_collision(9000, 5), "velocity-verlet", 17),
+        (W.plummer(1200, 6), "velocity-verlet", 40),
+    ]
+
+
+def config(**kw):
+    base = dict(round_ms=3.0, direct_below=4096, alone_from=8192, opening_angle=0.5,
+                leaf_capacity=16, tiles_per_item=16)
+    base.update(kw)
+    return E.ServerConfig(**base)
+
+
+def run_alone(ctx, state, integrator, steps):
+    s = E.Server(ctx, config())
+    rid = s.submit(state, integrator, EPS, 1e-3, steps)
+    s.drain()
+    out = s.result(rid)
+    st = s.status(rid)
+    s.close()
+    return out, st
+
+
+def test_alone_equals_mix_bitwise(ctx):
+    reqs = requests()
+    s = E.Server(ctx, config())
+    ids = [s.submit(w, integ, EPS, 1e-3, steps) for w, integ, steps in reqs[:3]]
+    s.round()
+    s.round()
+    # Arrivals while the first three are on the device.
+    ids += [s.submit(w, integ, EPS, 1e-3, steps) for w, integ, steps in reqs[3:]]
+    s.drain()
+    stats = s.stats()
+    assert stats["completed"] == len(reqs) and stats["active"] == 0 and stats["queued"] == 0
+    assert stats["batch_builds"] >= 3 and stats["rounds"] >= 4
+    mixed = [(s.result(i), s.status(i)) for i in ids]
+    s.close()
+    for (w, integ, steps), (got, st) in zip(reqs, mixed):
+        assert st["state"] == "done" and st["steps_done"] == steps
+        alone, st1 = run_alone(ctx, w, integ, steps)
+        for key in ("x", "y", "z", "vx", "vy", "vz"):
+            assert np.array_equal(got[key], alone[key]), (w.n, integ, key)
+        assert st["end"]["total"] == st1["end"]["total"]
+
+
+def test_matches_the_fp64_integrator_and_its_energy_drift(ctx):
+    p = W.plummer(1500, 11)
+    steps, dt = 60, 1e-3
+    s = E.Server(ctx, config())
+    rid = s.submit(p, "velocity-verlet", EPS, dt, steps)
+    s.drain()
+    got, st = s.result(rid), s.status(rid)
+    s.close()
+    f = oracle.make_field("vv", "direct", eps2=EPS * EPS)
+    ref = oracle.Particles(*(a.copy() for a in p.arrays()))
+    e0 = oracle.diagnostics(ref, EPS * EPS)["total"]
+    acc = oracle.refresh(ref, f)
+    oracle.steps(ref, acc, f, dt, steps)
+    e1 = oracle.diagnostics(ref, EPS * EPS)["total"]
+    assert st["start"]["total"] == pytest.approx(e0, rel=1e-14)
+    assert np.max(np.abs(got["x"] - ref.x)) < 1e-9
+    drift_gpu = (st["end"]["total"] - st["start"]["total"]) / abs(st["start"]["total"])
+    drift_cpu = (e1 - e0) / abs(e0)
+    assert abs(drift_gpu - drift_cpu) <= 0.05 * abs(drift_cpu) + 1e-10, (drift_gpu, drift_cpu)
+
+
+def test_memory_budget_bounds_admission(ctx):
+    reqs = requests()
+    # Room for about two of these requests at a time (280 B per particle + 1 MiB each).
+    budget = 2 * (1 << 20) + 280 * 14000
+    s = E.Server(ctx, config(memory_budget=budget))
+    ids = [s.submit(w, integ, EPS, 1e-3, steps) for w, integ, steps in reqs]
+    s.round()
+    st = s.stats()
+    assert st["queued"] > 0 and st["resident_bytes"] <= budget
+    s.drain()
+    st = s.stats()
+    assert st["completed"] == len(reqs) and st["peak_resident_bytes"] <= budget
+    assert st["resident_bytes"] == 0
+    times = [s.status(i) for i in ids]
+    assert all(t["state"] == "done" and t["admitted"] >= t["submitted"] for t in times)
+    # Some request waited for memory to free up.
+    assert max(t["admitted"] - t["submitted"] for t in times) > 0
+    s.close()
+
+
+def test_invalid_requests(ctx):
+    s = E.Server(ctx, config(memory_budget=50 << 20))
+    w = W.plummer(100, 1)
+    with pytest.raises(E.solvers.C.EngineError) as e:
+        s.submit(w, "velocity-verlet", EPS, 1e-3, 0)
+    assert e.value.status == 1
+    big = W.plummer(200000, 1)
+    with pytest.raises(E.solvers.C.EngineError) as e:
+        s.submit(big, "velocity-verlet", EPS, 1e-3, 5)
+    assert e.value.status == 5
+    rid = s.submit(w, "yoshida4", EPS, 1e-3, 3)
+    with pytest.raises(E.solvers.C.EngineError):
+        s.result(rid)
+    s.drain()
+    assert s.status(rid)["state"] == "done"
+    s.result(rid)
+    with pytest.raises(E.solvers.C.EngineError):
+        s.result(rid)
+    s.close()
