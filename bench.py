@@ -19,6 +19,10 @@ Further legs in the same JSON line:
   direct below 4096 particles and Barnes-Hut (theta 0.6) above, velocity Verlet
   dt = 1e-3, one batch step per step replayed as a hipGraph, against the same
   systems stepped one by one;
+* ``served``     -- Poisson arrivals of run requests (N log-uniform in [1e3, 2e6],
+  70/30 Plummer / two-disc collision, 100-1000 Yoshida-4 or velocity Verlet
+  steps) through the continuous-batching server: p50/p99 time to completion,
+  throughput, per-request energy drift against the CPU reference;
 * ``cpu_baseline`` and the ``force_error`` figures -- the CPU leg: the
   reference's threaded AVX2 FP64 kernel (or the oracle's restatement) timed on
   this host's cores over a bounded sample of the same input, and the FP64
@@ -211,6 +215,7 @@ def bench_barnes_hut(E, W, ctx, n: int, evals: int, error_targets: int, cpu) -> 
         "phases_ms": phases,
         "interactions_per_evaluation": {"particle_particle": pp, "particle_cell": pc},
         "walk_interactions_per_s": (pp + pc) / (phases.get("walk", 0.0) * 1e-3) if phases.get("walk") else None,
+        "walk_lane_efficiency": s.walk_lane_efficiency(),
         "tree": tree,
         "force_error": err,
         "streaming_stages": {"algorithmic_bytes": alg_bytes, "ms": build_ms,
@@ -343,6 +348,90 @@ def bench_batched(E, W, ctx, count: int, steps: int, alone_steps: int, cpu) -> d
     }
 
 
+def bench_serving(E, W, ctx, count: int, rate: float, seed: int, cpu) -> dict:
+    """Poisson arrivals of run requests served by the continuous-batching loop."""
+    eps, dt, below = 0.01, 1e-3, 4096
+    rng = np.random.default_rng(seed)
+    sizes = [int(v) for v in np.exp(rng.uniform(np.log(1e3), np.log(2e6), count))]
+    plummer = rng.random(count) < 0.7
+    steps = [int(v) for v in rng.integers(100, 1001, count)]
+    integ = ["yoshida4" if u < 0.5 else "velocity-verlet" for u in rng.random(count)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    log(f"[bench] serving: generating {count} requests ({sum(sizes)} particles)")
+    work = [W.plummer(n, 1000 + k) if plummer[k] else W.disc_collision(n, 1000 + k)
+            for k, n in enumerate(sizes)]
+    server = E.Server(ctx, E.ServerConfig(round_ms=50.0, direct_below=below, alone_from=131072,
+                                          opening_angle=0.5, leaf_capacity=16, tiles_per_item=16))
+    log(f"[bench] serving: {count} requests at {rate}/s")
+    ids = {}
+    start = server.now()
+    nxt = 0
+    pending = 0
+    while nxt < count or pending > 0:
+        now = server.now() - start
+        while nxt < count and arrivals[nxt] <= now:
+            ids[nxt] = server.submit(work[nxt], integ[nxt], eps, dt, steps[nxt])
+            nxt += 1
+            pending += 1
+        if pending == 0:
+            time.sleep(max(0.0, arrivals[nxt] - (server.now() - start)))
+            continue
+        pending = server.round()
+    makespan = server.now() - start
+    st = [server.status(ids[k]) for k in range(count)]
+    stats = server.stats()
+    server.close()
+    latency = np.array([s["completed"] - s["submitted"] for s in st])
+    queued = np.array([s["admitted"] - s["submitted"] for s in st])
+    drift = [(s["end"]["total"] - s["start"]["total"]) / abs(s["start"]["total"]) for s in st]
+    particle_steps = sum(n * k for n, k in zip(sizes, steps))
+
+    # Energy drift against the CPU reference (FP64 direct, same integrator and
+    # initial state) for the requests the CPU can redo in a bounded time.
+    cpu_drift = None
+    if cpu is not None:
+        o = cpu.oracle
+        budget, spent, rows = 20.0, 0.0, []
+        for k in sorted(range(count), key=lambda j: sizes[j] * steps[j]):
+            if sizes[k] >= below:
+                break
+            cost = sizes[k] ** 2 * steps[k] * (3 if integ[k] == "yoshida4" else 1) / 2e10
+            if spent + cost > budget:
+                break
+            t0 = time.perf_counter()
+            f = o.make_field("yoshida4" if integ[k] == "yoshida4" else "vv", "direct", eps2=eps * eps,
+                             threads=cpu.threads)
+            ref = o.Particles(*(a.copy() for a in work[k].arrays()))
+            e0 = o.diagnostics(ref, eps * eps)["total"]
+            acc = o.refresh(ref, f)
+            o.steps(ref, acc, f, dt, steps[k])
+            e1 = o.diagnostics(ref, eps * eps)["total"]
+            spent += time.perf_counter() - t0
+            rows.append({"n": sizes[k], "steps": steps[k], "integrator": integ[k],
+                         "device_drift": drift[k], "cpu_drift": (e1 - e0) / abs(e0)})
+        cpu_drift = {"requests": rows, "cores": cpu.threads, "seconds": spent,
+                     "reference": "oracle FP64 direct sum and integrator, same initial state"}
+    return {
+        "workload": f"{count} requests, Poisson arrivals at {rate}/s, N log-uniform in [1e3, 2e6], "
+                    f"70/30 Plummer / two-disc collision, 100-1000 steps, Yoshida-4 or velocity "
+                    f"Verlet, eps {eps}, dt {dt}; direct below {below} particles, Barnes-Hut "
+                    f"(theta 0.5, leaf 16) above",
+        "requests": count, "arrival_rate_per_s": rate, "seed": seed,
+        "particles_total": int(sum(sizes)), "particle_steps": int(particle_steps),
+        "makespan_s": makespan, "requests_per_s": count / makespan,
+        "particle_steps_per_s": particle_steps / makespan,
+        "latency_s": {"p50": float(np.percentile(latency, 50)), "p99": float(np.percentile(latency, 99)),
+                      "max": float(latency.max()), "min": float(latency.min())},
+        "queue_wait_s": {"p50": float(np.percentile(queued, 50)), "max": float(queued.max())},
+        "rounds": stats["rounds"], "batch_builds": stats["batch_builds"],
+        "busy_s": stats["busy_s"], "peak_resident_bytes": stats["peak_resident_bytes"],
+        "memory_budget": stats["memory_budget"],
+        "energy_drift": {"max_abs_relative": float(np.max(np.abs(drift))),
+                         "median_abs_relative": float(np.median(np.abs(drift)))},
+        "cpu_reference_drift": cpu_drift,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,6 +448,10 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--alone-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-direct", action="store_true", help="skip the headline direct leg")
     ap.add_argument("--probes-only", action="store_true",
                     help="run only the machine-ceiling probes (copy probe: counter calibration)")
@@ -452,6 +545,9 @@ def main() -> int:
     if not args.no_batched:
         result["batched"] = bench_batched(E, W, ctx, args.batch_systems, args.batch_steps,
                                           args.alone_steps, cpu)
+    if not args.no_serving:
+        result["served"] = bench_serving(E, W, ctx, args.serve_requests, args.serve_rate,
+                                         args.serve_seed, cpu)
     log("[bench] ceilings")
     result["ceilings"] = measure_ceilings(ctx)
     ctx.close()

@@ -156,6 +156,7 @@ typedef struct {
     uint64_t leaf_count;
     uint32_t depth;
     uint32_t reserved;
+    uint64_t wave_steps;     /* walk iterations summed over waves (64 lanes each) */
 } orr_tree_stats;
 ORR_API int orr_tree_eval(orr_ctx* ctx, const orr_tree_args* args, orr_tree_stats* stats);
 

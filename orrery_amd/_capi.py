@@ -57,7 +57,7 @@ class OrrTreeStats(ctypes.Structure):
     _fields_ = [("particle_particle", ctypes.c_uint64), ("particle_cell", ctypes.c_uint64),
                 ("node_visits", ctypes.c_uint64), ("node_count", ctypes.c_uint64),
                 ("leaf_count", ctypes.c_uint64), ("depth", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("reserved", ctypes.c_uint32), ("wave_steps", ctypes.c_uint64)]
 
 
 class OrrTimings(ctypes.Structure):

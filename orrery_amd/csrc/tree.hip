@@ -414,10 +414,26 @@ __global__ __launch_bounds__(kBlock) void internal_moments(
     }
 }
 
+// Minimum over the 64 lanes of a wave (all lanes active). Within each row of
+// 16 lanes, four DPP row shifts make lane 15 of the row hold the row minimum
+// (shifted-in lanes outside the row read the identity ~0); the four row minima
+// are then combined in scalar registers.
+template<int kCtrl> __device__ __forceinline__ std::uint32_t dpp_min(std::uint32_t v) {
+    const std::uint32_t moved = static_cast<std::uint32_t>(
+        __builtin_amdgcn_update_dpp(-1, static_cast<int>(v), kCtrl, 0xf, 0xf, false));
+    return min(v, moved);
+}
+
 __device__ inline std::uint32_t wave_min_u32(std::uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), off, 64)));
-    return v;
+    v = dpp_min<0x111>(v);  // row_shr:1
+    v = dpp_min<0x112>(v);  // row_shr:2
+    v = dpp_min<0x114>(v);  // row_shr:4
+    v = dpp_min<0x118>(v);  // row_shr:8
+    const std::uint32_t r0 = __builtin_amdgcn_readlane(static_cast<int>(v), 15);
+    const std::uint32_t r1 = __builtin_amdgcn_readlane(static_cast<int>(v), 31);
+    const std::uint32_t r2 = __builtin_amdgcn_readlane(static_cast<int>(v), 47);
+    const std::uint32_t r3 = __builtin_amdgcn_readlane(static_cast<int>(v), 63);
+    return min(min(r0, r1), min(r2, r3));
 }
 
 __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
@@ -464,9 +480,14 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     std::uint32_t mine = valid ? root : kDone;
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     float px = 0.f, py = 0.f, pz = 0.f;
-    unsigned long long pp = 0, pc = 0, visits = 0, steps = 0;
+    // Per lane, pp < N, pc <= M and visits <= M: 32 bits suffice.
+    std::uint32_t pp = 0, pc = 0, visits = 0;
+    unsigned long long steps = 0;
+    // Each iteration advances at least one lane, so a walk takes at most
+    // 64 M iterations; the cap only turns a corrupted tree into a wrong result.
+    const unsigned long long cap = 64ull * M + 64;
     std::uint32_t cur = __builtin_amdgcn_readfirstlane(wave_min_u32(mine));
-    while (cur != kDone) {
+    while (cur != kDone && steps < cap) {
         ++steps;
         const WalkNode nd = nodes[cur];
         // An escape index always lies past its node; the guard only makes a
@@ -541,8 +562,8 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
         }
         if (mine >= node_end) mine = kDone;
         const std::uint32_t step = cur + 1;
-        cur = (__ballot(mine == step) != 0) ? step : wave_min_u32(mine);
-        cur = __builtin_amdgcn_readfirstlane(cur);
+        cur = (__ballot(mine == step) != 0) ? step
+                                            : __builtin_amdgcn_readfirstlane(wave_min_u32(mine));
     }
     kahan(px, sx, kx);
     kahan(py, sy, ky);
@@ -556,22 +577,22 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     // Counts per system: one atomic per wave when the wave holds one system.
     const std::uint32_t seg0 = __builtin_amdgcn_readfirstlane(seg);
     if (__ballot(valid && seg != seg0) == 0) {
-        pp = wave_sum_u64(pp);
-        pc = wave_sum_u64(pc);
-        visits = wave_sum_u64(visits);
+        const unsigned long long wpp = wave_sum_u64(pp);
+        const unsigned long long wpc = wave_sum_u64(pc);
+        const unsigned long long wvisits = wave_sum_u64(visits);
         if ((threadIdx.x & 63) == 0) {
             unsigned long long* st = stats + 4 * static_cast<std::size_t>(seg0);
-            atomicAdd(&st[0], pp);
-            atomicAdd(&st[1], pc);
-            atomicAdd(&st[2], visits);
+            atomicAdd(&st[0], wpp);
+            atomicAdd(&st[1], wpc);
+            atomicAdd(&st[2], wvisits);
             atomicAdd(&st[3], steps);
         }
     } else {
         if (valid) {
             unsigned long long* st = stats + 4 * static_cast<std::size_t>(seg);
-            atomicAdd(&st[0], pp);
-            atomicAdd(&st[1], pc);
-            atomicAdd(&st[2], visits);
+            atomicAdd(&st[0], static_cast<unsigned long long>(pp));
+            atomicAdd(&st[1], static_cast<unsigned long long>(pc));
+            atomicAdd(&st[2], static_cast<unsigned long long>(visits));
         }
         if ((threadIdx.x & 63) == 0) atomicAdd(&stats[4 * static_cast<std::size_t>(seg0) + 3], steps);
     }
@@ -754,6 +775,7 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
             stats->particle_particle += c64[4 * sgm];
             stats->particle_cell += c64[4 * sgm + 1];
             stats->node_visits += c64[4 * sgm + 2];
+            stats->wave_steps += c64[4 * sgm + 3];
         }
         ctx->last_tree = *stats;
     }

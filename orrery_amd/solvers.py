@@ -309,6 +309,7 @@ class HipTreeSolver:
         self.G = float(gravitational_constant)
         self._count = InteractionCount()
         self._visits = 0
+        self._wave_steps = 0
         self._stats = C.OrrTreeStats()
         self._staging = _Staging(self.ctx)
 
@@ -337,9 +338,14 @@ class HipTreeSolver:
     def reset_interaction_count(self) -> None:
         self._count = InteractionCount()
         self._visits = 0
+        self._wave_steps = 0
 
     def node_visits(self) -> int:
         return self._visits
+
+    def walk_lane_efficiency(self) -> float:
+        """Node visits per lane-iteration of the walk (1.0: no lane ever idles)."""
+        return self._visits / (64.0 * self._wave_steps) if self._wave_steps else 0.0
 
     def tree_summary(self) -> dict:
         s = self._stats
@@ -386,6 +392,7 @@ class HipTreeSolver:
         self._count.particle_particle += int(self._stats.particle_particle)
         self._count.particle_cell += int(self._stats.particle_cell)
         self._visits += int(self._stats.node_visits)
+        self._wave_steps += int(self._stats.wave_steps)
 
     def nodes(self) -> dict:
         """Nodes of the last evaluation in DFS order (TreeNode fields)."""
