@@ -109,12 +109,12 @@ typedef struct {
 } orr_direct_args;
 ORR_API int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* args);
 
-/* Launch configuration of the direct kernel, chosen per context by the
- * autotuner (no reference counterpart: the CUDA solver fixes its block size,
- * cuda_direct_solver.cpp:46-63). targets_per_lane in {1, 2, 4}; refine = 1
- * applies a Newton step to the hardware reciprocal square root (needed for
- * <= 1e-6 relative force error); block_target is the number of thread blocks
- * the source range is split to reach. Zero fields keep the current value. */
+/* Launch configuration of the direct kernel (no reference counterpart: the
+ * CUDA solver fixes its block size, cuda_direct_solver.cpp:46-63).
+ * targets_per_lane in {1, 2, 4}; refine = 1 applies a Newton step to the
+ * hardware reciprocal square root (off by default: v_rsq_f32 already keeps the
+ * force within 1e-6 relative); block_target is the number of thread blocks the
+ * source range is split to reach. Zero fields keep the current value. */
 typedef struct {
     int32_t targets_per_lane;
     int32_t refine;
@@ -211,7 +211,9 @@ ORR_API int orr_ctx_reset_timings(orr_ctx* ctx);
 /* ---- device-resident systems: state stays in HBM across steps ----------- */
 
 /* Integrators: 0 velocity Verlet (integrators/velocity_verlet.cpp:20-72),
- * 1 Yoshida-4 (integrators/yoshida4.cpp:24-59). Solvers: 0 direct, 1 Barnes-Hut.
+ * 1 Yoshida-4 (integrators/yoshida4.cpp:24-59), 2 classical Runge-Kutta 4
+ * (integrators/runge_kutta4.cpp:64-139; four evaluations per step, twelve more
+ * FP64 arrays of stage state). Solvers: 0 direct, 1 Barnes-Hut.
  * FP64 kick and drift on the device; the field is the FP32-term engine above. */
 typedef struct {
     int32_t integrator;

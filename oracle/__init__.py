@@ -292,7 +292,7 @@ def barnes_hut_fp32_counts(x, y, z, m, theta, leaf_capacity, threads=0):
 def make_field(integrator="vv", solver="direct", eps2=0.0, G=1.0, theta=0.5, leaf_capacity=32,
                threads=0, kernel=1):
     f = OrcField()
-    f.integrator = {"vv": 0, "yoshida4": 1}[integrator]
+    f.integrator = {"vv": 0, "yoshida4": 1, "rk4": 2}[integrator]
     f.solver = {"direct": 2 if kernel == 1 else 0, "barnes-hut": 1}[solver]
     f.softening_squared = eps2
     f.G = G

@@ -19,7 +19,7 @@
 //                              library bitwise (structure and moments)
 //   barnes_hut              -- theta = 0 equals direct with pp = N(N-1); reference
 //                              library bitwise (accelerations and counts)
-//   integrators             -- kepler-double.otj bitwise; measured orders 2 and 4
+//   integrators             -- kepler-double.otj bitwise; measured orders 2, 4 and 4 (RK4)
 //   diagnostics             -- agreement.csv energies; Plummer and uniform-sphere
 //                              closed forms
 //   disc_collision          -- this project's own generator (no reference
@@ -650,6 +650,50 @@ void vv_step(std::size_t n, double* x, double* y, double* z, double* vx, double*
     for (std::size_t i = 0; i < n; ++i) vz[i] += half * az[i];
 }
 
+// runge_kutta4.cpp:64-139: classical RK4 on (x, v) with stage sums; four
+// evaluations per step, the last restoring a(x) at the final positions.
+void rk4_step(std::size_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+              double* ax, double* ay, double* az, const double* m, const orc_field& f, double dt) {
+    double* pos[3] = {x, y, z};
+    double* vel[3] = {vx, vy, vz};
+    double* acc[3] = {ax, ay, az};
+    std::vector<double> p0[3], v0[3], sp[3], sv[3];
+    for (int c = 0; c < 3; ++c) {
+        p0[c].assign(pos[c], pos[c] + n);
+        v0[c].assign(vel[c], vel[c] + n);
+        sp[c].assign(vel[c], vel[c] + n);
+        sv[c].assign(acc[c], acc[c] + n);
+    }
+    auto stage = [&](double scale, double weight) {
+        for (int c = 0; c < 3; ++c) {
+            for (std::size_t i = 0; i < n; ++i) {
+                const double stage_velocity = vel[c][i];
+                pos[c][i] = p0[c][i] + (scale * stage_velocity);
+                vel[c][i] = v0[c][i] + (scale * acc[c][i]);
+            }
+        }
+        field_eval(n, x, y, z, m, f, ax, ay, az);
+        for (int c = 0; c < 3; ++c) {
+            for (std::size_t i = 0; i < n; ++i) sp[c][i] += weight * vel[c][i];
+        }
+        for (int c = 0; c < 3; ++c) {
+            for (std::size_t i = 0; i < n; ++i) sv[c][i] += weight * acc[c][i];
+        }
+    };
+    const double half = dt / 2;
+    stage(half, 2);
+    stage(half, 2);
+    stage(dt, 1);
+    const double sixth = dt / 6;
+    for (int c = 0; c < 3; ++c) {
+        for (std::size_t i = 0; i < n; ++i) pos[c][i] = p0[c][i] + (sixth * sp[c][i]);
+    }
+    for (int c = 0; c < 3; ++c) {
+        for (std::size_t i = 0; i < n; ++i) vel[c][i] = v0[c][i] + (sixth * sv[c][i]);
+    }
+    field_eval(n, x, y, z, m, f, ax, ay, az);
+}
+
 // yoshida4.cpp:24-41
 constexpr double kCubeRootOfTwo = 1.2599210498948731647672106;
 constexpr double kOuter = 1.0 / (2 - kCubeRootOfTwo);
@@ -1031,7 +1075,9 @@ int orc_steps(uint64_t n, double* x, double* y, double* z, double* vx, double* v
               double dt, uint64_t steps) {
     if (f == nullptr) return 1;
     for (uint64_t s = 0; s < steps; ++s) {
-        if (f->integrator == 1) {
+        if (f->integrator == 2) {
+            rk4_step(n, x, y, z, vx, vy, vz, ax, ay, az, m, *f, dt);
+        } else if (f->integrator == 1) {
             vv_step(n, x, y, z, vx, vy, vz, ax, ay, az, m, *f, kOuter * dt);
             vv_step(n, x, y, z, vx, vy, vz, ax, ay, az, m, *f, kInner * dt);
             vv_step(n, x, y, z, vx, vy, vz, ax, ay, az, m, *f, kOuter * dt);

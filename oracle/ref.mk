@@ -18,7 +18,8 @@ SOURCES := core/random core/diagnostics core/particle_data core/vec3_array \
            backend/static_executor backend/cpu_features backend/cpu_topology \
            initial_conditions/plummer initial_conditions/uniform_sphere \
            initial_conditions/centre_of_mass_frame integrators/velocity_verlet \
-           integrators/yoshida4 solvers/direct_kernel solvers/direct_kernel_scalar \
+           integrators/yoshida4 integrators/runge_kutta4 solvers/direct_kernel \
+           solvers/direct_kernel_scalar \
            solvers/direct_solver solvers/reference_kernel solvers/morton solvers/octree \
            solvers/tree_walk solvers/barnes_hut_solver
 

@@ -22,6 +22,7 @@
 #include "orrery/initial_conditions/plummer.hpp"
 #include "orrery/initial_conditions/uniform_sphere.hpp"
 #include "orrery/integrators/integrator.hpp"
+#include "orrery/integrators/runge_kutta4.hpp"
 #include "orrery/integrators/velocity_verlet.hpp"
 #include "orrery/integrators/yoshida4.hpp"
 #include "orrery/solvers/barnes_hut_solver.hpp"
@@ -275,7 +276,9 @@ int ref_steps(std::uint64_t n, double* x, double* y, double* z, double* vx, doub
         solver = std::move(ds);
     }
     std::unique_ptr<integrators::Integrator> step;
-    if (integrator == 1) {
+    if (integrator == 2) {
+        step = std::make_unique<integrators::RungeKutta4>();
+    } else if (integrator == 1) {
         step = std::make_unique<integrators::Yoshida4>();
     } else {
         step = std::make_unique<integrators::VelocityVerlet>();

@@ -28,8 +28,10 @@ void orr::set_last_error(const char* what) { g_last_error = what; }
 struct orr_system {
     orr_ctx* ctx = nullptr;
     std::uint64_t count = 0;
-    orr::Buffer state;  // x y z vx vy vz ax ay az m, FP64, count each
+    orr::Buffer state;   // x y z vx vy vz ax ay az m, FP64, count each
+    orr::Buffer stages;  // RK4 only: x0 (3), v0 (3), position sums (3), velocity sums (3)
     double* array(int k) const { return state.as<double>() + static_cast<std::size_t>(k) * count; }
+    double* stage(int k) const { return stages.as<double>() + static_cast<std::size_t>(k) * count; }
 };
 
 namespace {
@@ -89,9 +91,39 @@ void vv_substep(orr_system* s, const orr_step_config& c, double dt) {
 using orr::kInnerWeight;
 using orr::kOuterWeight;
 
+// runge_kutta4.cpp:64-139: three stages (dt/2 weight 2, dt/2 weight 2, dt weight
+// 1) after the first, combined with dt/6; a fourth evaluation restores a(x).
+void rk4_step(orr_system* s, const orr_step_config& c, double dt) {
+    const std::uint64_t n = s->count;
+    s->stages.reserve(sizeof(double) * 12 * n);
+    orr::Rk4Arrays r{};
+    for (int k = 0; k < 3; ++k) {
+        r.x[k] = s->array(k);
+        r.v[k] = s->array(3 + k);
+        r.a[k] = s->array(6 + k);
+        r.x0[k] = s->stage(k);
+        r.v0[k] = s->stage(3 + k);
+        r.sx[k] = s->stage(6 + k);
+        r.sv[k] = s->stage(9 + k);
+    }
+    orr::rk4(s->ctx, n, r, 0, 0.0);
+    const double half_step = dt / 2;
+    const double scales[3] = {half_step, half_step, dt};
+    const double weights[3] = {2, 2, 1};
+    for (int k = 0; k < 3; ++k) {
+        orr::rk4(s->ctx, n, r, 1, scales[k]);
+        field_eval(s, c);
+        orr::rk4(s->ctx, n, r, 2, weights[k]);
+    }
+    const double sixth_step = dt / 6;
+    orr::rk4(s->ctx, n, r, 3, sixth_step);
+    field_eval(s, c);
+}
+
 void check_config(const orr_step_config* c) {
     require(c != nullptr, "step config is null");
-    require(c->integrator == 0 || c->integrator == 1, "integrator must be 0 (VV) or 1 (Yoshida-4)");
+    require(c->integrator >= 0 && c->integrator <= 2,
+            "integrator must be 0 (VV), 1 (Yoshida-4) or 2 (RK4)");
     require(c->solver == 0 || c->solver == 1, "solver must be 0 (direct) or 1 (Barnes-Hut)");
     require(c->softening_squared >= 0, "softening squared must be >= 0");
 }
@@ -347,6 +379,7 @@ int orr_system_destroy(orr_system* s) {
         use_device(s->ctx);
         s->ctx->sync();
         s->state.release();
+        s->stages.release();
         delete s;
     });
 }
@@ -398,7 +431,9 @@ int orr_system_step(orr_system* s, const orr_step_config* c, double dt, uint64_t
         use_device(s->ctx);
         if (s->count == 0) return;
         for (uint64_t k = 0; k < steps; ++k) {
-            if (c->integrator == 1) {
+            if (c->integrator == 2) {
+                rk4_step(s, *c, dt);
+            } else if (c->integrator == 1) {
                 vv_substep(s, *c, kOuterWeight * dt);
                 vv_substep(s, *c, kInnerWeight * dt);
                 vv_substep(s, *c, kOuterWeight * dt);

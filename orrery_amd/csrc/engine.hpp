@@ -244,6 +244,18 @@ void kick(orr_ctx* ctx, std::uint64_t n, double* vx, double* vy, double* vz, con
           const double* ay, const double* az, double scale);
 void drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, const double* vx,
            const double* vy, const double* vz, double scale);
+// The arrays of one RK4 step, per component: state, step start, stage sums.
+struct Rk4Arrays {
+    double* x[3];
+    double* v[3];
+    double* a[3];
+    double* x0[3];
+    double* v0[3];
+    double* sx[3];
+    double* sv[3];
+};
+// op 0 begin, 1 move to a stage (scale), 2 accumulate (weight), 3 combine (scale).
+void rk4(orr_ctx* ctx, std::uint64_t n, const Rk4Arrays& arrays, int op, double scale);
 void diagnostics(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
                  const double* z, const double* vx, const double* vy, const double* vz,
                  const double* m, double softening_squared, double G, double out[8]);

@@ -3,6 +3,8 @@
 // kick / drift follow src/integrators/velocity_verlet.cpp:20-55 (v += s a,
 // x += s v, one multiply and one add each, never fused: the file disables
 // contraction so the update rounds as the reference's portable build does).
+// RK4 follows src/integrators/runge_kutta4.cpp:19-139 with the same unfused
+// multiply-then-add updates.
 // The diagnostics follow src/core/diagnostics.cpp:85-200: kinetic energy,
 // momentum and angular momentum as Neumaier-compensated sums, potential
 // energy as rows i of plain sums over j > i, the rows then compensated.
@@ -31,6 +33,34 @@ __global__ __launch_bounds__(256) void kick_kernel(std::uint64_t n, double* __re
     vx[i] = vx[i] + s * ax[i];
     vy[i] = vy[i] + s * ay[i];
     vz[i] = vz[i] + s * az[i];
+}
+
+// runge_kutta4.cpp:19-56, one kernel for its four element-wise loops:
+// op 0 begins a step (x0 = x, v0 = v, Sx = v, Sv = a); op 1 moves to a stage
+// (x = x0 + s v, v = v0 + s a, reading v first); op 2 accumulates a stage's
+// derivatives (Sx += s v, Sv += s a); op 3 combines (x = x0 + s Sx, v = v0 + s Sv).
+__global__ __launch_bounds__(256) void rk4_kernel(std::uint64_t n, Rk4Arrays r, int op, double s) {
+    const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (op == 0) {
+            r.x0[c][i] = r.x[c][i];
+            r.v0[c][i] = r.v[c][i];
+            r.sx[c][i] = r.v[c][i];
+            r.sv[c][i] = r.a[c][i];
+        } else if (op == 1) {
+            const double stage_velocity = r.v[c][i];
+            r.x[c][i] = r.x0[c][i] + (s * stage_velocity);
+            r.v[c][i] = r.v0[c][i] + (s * r.a[c][i]);
+        } else if (op == 2) {
+            r.sx[c][i] = r.sx[c][i] + (s * r.v[c][i]);
+            r.sv[c][i] = r.sv[c][i] + (s * r.a[c][i]);
+        } else {
+            r.x[c][i] = r.x0[c][i] + (s * r.sx[c][i]);
+            r.v[c][i] = r.v0[c][i] + (s * r.sv[c][i]);
+        }
+    }
 }
 
 // Neumaier step (core/diagnostics.cpp:36-59).
@@ -118,6 +148,14 @@ void kick(orr_ctx* ctx, std::uint64_t n, double* vx, double* vy, double* vz, con
     hipLaunchKernelGGL(kick_kernel, dim3(static_cast<std::uint32_t>((n + 255) / 256)), dim3(256),
                        0, ctx->stream, n, vx, vy, vz, ax, ay, az, s);
     ORR_LAUNCHED("kick");
+}
+
+void rk4(orr_ctx* ctx, std::uint64_t n, const Rk4Arrays& arrays, int op, double scale) {
+    if (n == 0) return;
+    PhaseTimer t(ctx, kIntegrate);
+    hipLaunchKernelGGL(rk4_kernel, dim3(static_cast<std::uint32_t>((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, n, arrays, op, scale);
+    ORR_LAUNCHED("rk4");
 }
 
 void drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, const double* vx,

@@ -206,25 +206,29 @@ __global__ __launch_bounds__(kBlock) void tree_nodes(
         std::uint32_t end = k + 1;
         for (int l = static_cast<int>(lleaf); l >= static_cast<int>(dk); --l) {
             end = prefix_end(codes, seg_end, end, key, static_cast<unsigned>(l));
-            double size = cube[3];
-            double c[3] = {cube[0] + size / 2, cube[1] + size / 2, cube[2] + size / 2};
-            for (int s = 1; s <= l; ++s) {
-                const unsigned oct = static_cast<unsigned>((key >> (3u * (kLevels - s))) & 7u);
+            const std::uint32_t node = base + (static_cast<unsigned>(l) - dk);
+            node_begin[node] = k;
+            node_count[node] = end - k;
+            node_level[node] = static_cast<std::uint8_t>(l);
+            node_next[node] = end >= n ? M : first_node[end];
+            if (static_cast<unsigned>(l) == lleaf) {
+                for (std::uint32_t j = k; j < end; ++j) leaf_of[j] = node;
+            }
+        }
+        // From the root downwards: one halving per level gives every chain
+        // node's centre with the same operations as a descent from the root.
+        double size = cube[3];
+        double c[3] = {cube[0] + size / 2, cube[1] + size / 2, cube[2] + size / 2};
+        for (unsigned l = 0; l <= lleaf; ++l) {
+            if (l > 0) {
+                const unsigned oct = static_cast<unsigned>((key >> (3u * (kLevels - l))) & 7u);
                 const double q = size / 4;
                 c[0] = c[0] + ((oct & 4u) != 0 ? q : -q);
                 c[1] = c[1] + ((oct & 2u) != 0 ? q : -q);
                 c[2] = c[2] + ((oct & 1u) != 0 ? q : -q);
                 size = size / 2;
             }
-            const std::uint32_t node = base + (static_cast<unsigned>(l) - dk);
-            node_begin[node] = k;
-            node_count[node] = end - k;
-            node_level[node] = static_cast<std::uint8_t>(l);
-            node_next[node] = end >= n ? M : first_node[end];
-            node_geo[node] = make_double4(c[0], c[1], c[2], size);
-            if (static_cast<unsigned>(l) == lleaf) {
-                for (std::uint32_t j = k; j < end; ++j) leaf_of[j] = node;
-            }
+            if (l >= dk) node_geo[base + (l - dk)] = make_double4(c[0], c[1], c[2], size);
         }
         depth = lleaf;
     }
@@ -490,6 +494,17 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     while (cur != kDone && steps < cap) {
         ++steps;
         const WalkNode nd = nodes[cur];
+        QuadF Q{};
+        if constexpr (kQuad) Q = quads[cur];
+        // Every scalar load of the iteration is issued here and waited for once;
+        // left to itself the compiler sinks them into the branches below, one
+        // dependent wait each.
+        asm volatile("" ::"s"(nd.a.x), "s"(nd.a.y), "s"(nd.a.z), "s"(nd.a.w), "s"(nd.b.x),
+                     "s"(nd.b.y), "s"(nd.b.z), "s"(nd.b.w));
+        if constexpr (kQuad) {
+            asm volatile("" ::"s"(Q.a.x), "s"(Q.a.y), "s"(Q.a.z), "s"(Q.a.w), "s"(Q.b.x),
+                         "s"(Q.b.y));
+        }
         // An escape index always lies past its node; the guard only makes a
         // corrupted tree end the walk instead of looping.
         const std::uint32_t next = nd.b.y > cur ? nd.b.y : node_end;
@@ -505,24 +520,26 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
                 float r2e = r2 + eps2;
                 if constexpr (kSafe) r2e = r2e > 0.f ? r2e : 1.f;
                 const float inv = __builtin_amdgcn_rsqf(r2e);
-                const float f = (nd.a.w * inv) * (inv * inv);
-                px = __builtin_fmaf(f, dx, px);
-                py = __builtin_fmaf(f, dy, py);
-                pz = __builtin_fmaf(f, dz, pz);
-                if constexpr (kQuad) {
-                    // tree_walk.hpp:156-181: -Q d inv^5 + 2.5 (d.Q d) d inv^7.
-                    const QuadF Q = quads[cur];
+                if constexpr (!kQuad) {
+                    const float f = (nd.a.w * inv) * (inv * inv);
+                    px = __builtin_fmaf(f, dx, px);
+                    py = __builtin_fmaf(f, dy, py);
+                    pz = __builtin_fmaf(f, dz, pz);
+                } else {
+                    // Monopole plus tree_walk.hpp:156-181 in one update:
+                    // d (m inv^3 + 2.5 (d.Q d) inv^7) - (Q d) inv^5.
                     const float i2 = inv * inv;
-                    const float i5 = (i2 * i2) * inv;
+                    const float i3 = i2 * inv;
+                    const float i5 = i3 * i2;
                     const float i7 = i5 * i2;
                     const float qx = __builtin_fmaf(Q.a.z, dz, __builtin_fmaf(Q.a.y, dy, Q.a.x * dx));
                     const float qy = __builtin_fmaf(Q.b.x, dz, __builtin_fmaf(Q.a.w, dy, Q.a.y * dx));
                     const float qz = __builtin_fmaf(Q.b.y, dz, __builtin_fmaf(Q.b.x, dy, Q.a.z * dx));
                     const float quadratic = __builtin_fmaf(dz, qz, __builtin_fmaf(dy, qy, dx * qx));
-                    const float s = (2.5f * quadratic) * i7;
-                    px = __builtin_fmaf(dx, s, __builtin_fmaf(qx, -i5, px));
-                    py = __builtin_fmaf(dy, s, __builtin_fmaf(qy, -i5, py));
-                    pz = __builtin_fmaf(dz, s, __builtin_fmaf(qz, -i5, pz));
+                    const float g = __builtin_fmaf(2.5f * quadratic, i7, nd.a.w * i3);
+                    px = __builtin_fmaf(dx, g, __builtin_fmaf(qx, -i5, px));
+                    py = __builtin_fmaf(dy, g, __builtin_fmaf(qy, -i5, py));
+                    pz = __builtin_fmaf(dz, g, __builtin_fmaf(qz, -i5, pz));
                 }
                 ++pc;
                 mine = next;
@@ -541,8 +558,7 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
             const float tz = oi.z - static_cast<float>(cb.z - ca.z);
             const std::uint32_t b = nd.b.z, e = nd.b.z + nd.b.w;
             if (leaf_pp) {
-                for (std::uint32_t j = b; j < e; ++j) {
-                    const float4 q = leaf_rel[j];
+                auto pair = [&](const float4 q) {
                     const float dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
                     float r2 = __builtin_fmaf(dx, dx, eps2);
                     r2 = __builtin_fmaf(dy, dy, r2);
@@ -553,7 +569,18 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
                     px = __builtin_fmaf(f, dx, px);
                     py = __builtin_fmaf(f, dy, py);
                     pz = __builtin_fmaf(f, dz, pz);
+                };
+                // Four sources per round so their scalar loads share one wait.
+                std::uint32_t j = b;
+                for (; j + 4 <= e; j += 4) {
+                    const float4 q0 = leaf_rel[j], q1 = leaf_rel[j + 1];
+                    const float4 q2 = leaf_rel[j + 2], q3 = leaf_rel[j + 3];
+                    pair(q0);
+                    pair(q1);
+                    pair(q2);
+                    pair(q3);
                 }
+                for (; j < e; ++j) pair(leaf_rel[j]);
                 kahan(px, sx, kx);
                 kahan(py, sy, ky);
                 kahan(pz, sz, kz);

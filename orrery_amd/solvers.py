@@ -431,7 +431,7 @@ def morton_order(ctx: Context, x: np.ndarray, y: np.ndarray, z: np.ndarray):
     return out
 
 
-INTEGRATORS = {"velocity-verlet": 0, "vv": 0, "yoshida4": 1}
+INTEGRATORS = {"velocity-verlet": 0, "vv": 0, "yoshida4": 1, "runge-kutta4": 2, "rk4": 2}
 SOLVERS = {"direct": 0, "hip-direct": 0, "barnes-hut": 1, "hip-tree": 1}
 
 

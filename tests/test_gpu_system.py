@@ -116,12 +116,13 @@ def test_device_diagnostics_match_oracle(ctx):
     s.close()
 
 
-@pytest.mark.parametrize("integrator", ["velocity-verlet", "yoshida4"])
+@pytest.mark.parametrize("integrator", ["velocity-verlet", "yoshida4", "runge-kutta4"])
 def test_steps_match_cpu_reference_path(ctx, integrator):
     # Same seed, same steps: device state against the oracle's FP64 integrator.
     p = oracle.plummer(2048, 11)
     eps, dt, steps = 0.01, 1e-3, 10
-    f = oracle.make_field(integrator="vv" if integrator == "velocity-verlet" else "yoshida4",
+    f = oracle.make_field(integrator={"velocity-verlet": "vv", "yoshida4": "yoshida4",
+                                      "runge-kutta4": "rk4"}[integrator],
                           solver="direct", eps2=eps * eps)
     ref = oracle.Particles(*(a.copy() for a in p.arrays()))
     acc = oracle.refresh(ref, f)
@@ -135,3 +136,40 @@ def test_steps_match_cpu_reference_path(ctx, integrator):
     assert np.max(np.abs(st["x"] - ref.x)) < 1e-9
     assert np.max(np.abs(st["vx"] - ref.vx)) < 1e-7
     s.close()
+
+
+def test_rk4_measured_order_and_tree_field(ctx):
+    # Circular two-body orbit over one period (tests/integrators/convergence_test.cpp):
+    # RK4's closure error falls by 2^4 when the step halves.
+    def closure(steps):
+        s = E.DeviceSystem(ctx, 2)
+        z = np.zeros(2)
+        s.upload(np.array([-0.5, 0.5]), z, z, z, np.array([-np.sqrt(0.5), np.sqrt(0.5)]), z,
+                 np.array([1.0, 1.0]))
+        cfg = E.StepConfig(integrator="runge-kutta4", solver="direct", softening=0.0)
+        s.refresh(cfg)
+        s.step(cfg, 2 * np.pi * np.sqrt(0.5) / steps, steps)
+        st = s.download()
+        s.close()
+        return np.hypot(st["x"][0] + 0.5, st["y"][0])
+    order = np.log2(closure(100) / closure(200))
+    assert abs(order - 4.0) < 0.2, order
+    # RK4 over the Barnes-Hut field against the oracle's RK4 over its FP64 walk.
+    p = oracle.plummer(3000, 5)
+    eps, dt, steps = 0.02, 1e-3, 3
+    s = E.DeviceSystem(ctx, p.n)
+    s.upload(*p.arrays())
+    cfg = E.StepConfig(integrator="runge-kutta4", solver="barnes-hut", softening=eps,
+                       opening_angle=0.5, leaf_capacity=16)
+    s.refresh(cfg)
+    s.step(cfg, dt, steps)
+    st = s.download()
+    s.close()
+    f = oracle.make_field(integrator="rk4", solver="barnes-hut", eps2=eps * eps, theta=0.5,
+                          leaf_capacity=16)
+    ref = oracle.Particles(*(a.copy() for a in p.arrays()))
+    acc = oracle.refresh(ref, f)
+    oracle.steps(ref, acc, f, dt, steps)
+    # FP32 cell terms vs the FP64 walk: positions agree to what 3 steps of
+    # 1e-3 relative force differences move them.
+    assert np.max(np.abs(st["x"] - ref.x)) < 1e-7

@@ -265,7 +265,7 @@ def test_bh_error_monotone_in_theta():
 
 # ---- integrators: measured order (tests/integrators/convergence_test.cpp) -----
 
-@pytest.mark.parametrize("integrator,order", [("vv", 2.0), ("yoshida4", 4.0)])
+@pytest.mark.parametrize("integrator,order", [("vv", 2.0), ("yoshida4", 4.0), ("rk4", 4.0)])
 def test_measured_order(integrator, order):
     # Circular two-body orbit, one period; error vs exact closure at dt and dt/2.
     def final_error(steps):

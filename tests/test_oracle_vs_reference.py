@@ -101,11 +101,11 @@ def test_barnes_hut_bitwise(quad):
     assert same((ax, ay, az), r) and (pp, pc) == (int(c[0]), int(c[1]))
 
 
-@pytest.mark.parametrize("integ,solver", [(0, 2), (1, 2), (0, 1), (1, 0)])
+@pytest.mark.parametrize("integ,solver", [(0, 2), (1, 2), (0, 1), (1, 0), (2, 2), (2, 1)])
 def test_integrators_bitwise(integ, solver):
     eps = 0.05
     s1, s2 = O.plummer(400, 3), O.plummer(400, 3)
-    f = O.make_field(integrator=["vv", "yoshida4"][integ],
+    f = O.make_field(integrator=["vv", "yoshida4", "rk4"][integ],
                      solver="barnes-hut" if solver == 1 else "direct", eps2=eps * eps, theta=0.5,
                      leaf_capacity=16, kernel=1 if solver == 2 else 0)
     acc = O.refresh(s1, f)
