@@ -228,6 +228,8 @@ typedef struct {
 
 ORR_API int orr_system_create(orr_ctx* ctx, uint64_t count, orr_system** out);
 ORR_API int orr_system_destroy(orr_system* sys);
+/* Number of particles (ParticleData::size, core/particle_data.hpp). */
+ORR_API int orr_system_count(orr_system* sys, uint64_t* count);
 /* Host arrays of `count` FP64 values each. */
 ORR_API int orr_system_upload(orr_system* sys, const double* x, const double* y, const double* z,
                       const double* vx, const double* vy, const double* vz, const double* m);
@@ -241,6 +243,38 @@ ORR_API int orr_system_step(orr_system* sys, const orr_step_config* config, doub
  * out = {kinetic, potential, px, py, pz, lx, ly, lz}. */
 ORR_API int orr_system_diagnostics(orr_system* sys, double softening_squared,
                            double gravitational_constant, double out[8]);
+
+/* ---- checkpoint and trajectory files from device-resident state ----------
+ *
+ * The reference's byte formats (docs/formats/checkpoint.md and
+ * docs/formats/trajectory.md), so `orrery resume`, `orrery inspect` and the web
+ * client read what the engine writes. */
+
+/* sim/checkpoint.cpp:86-131: magic ORRERYCK, version 1, flags 0 (FP64), step,
+ * N, configuration text (u32 length + bytes, stored as given), then masses and
+ * x y z, vx vy vz, ax ay az, then the FNV-1a-64 checksum of all of it; written
+ * to "<path>.partial" and renamed over `path`. */
+ORR_API int orr_checkpoint_write(orr_system* sys, const char* path, uint64_t step,
+                                 const char* configuration, uint64_t configuration_length);
+/* sim/checkpoint.cpp:133-194: refuses a wrong magic, version, reserved or
+ * single-precision flag, short file or checksum mismatch (ORR_ERR_INVALID_ARGUMENT);
+ * else creates a system on ctx holding the stored state, accelerations
+ * included, so a resumed run continues bit for bit. The configuration text is
+ * copied NUL-terminated into `configuration` (capacity bytes, may be null);
+ * parsing it is the caller's. */
+ORR_API int orr_checkpoint_read(orr_ctx* ctx, const char* path, orr_system** out, uint64_t* step,
+                                char* configuration, uint64_t capacity,
+                                uint64_t* configuration_length);
+
+/* sim/trajectory.cpp:38-101: the header (magic ORRERYTJ, version 1, flags with
+ * bit 1 = frames carry velocities, N, timestep, masses, checksum) is written at
+ * open; each append writes one frame (step, step x timestep, x y z [vx vy vz],
+ * checksum) from the system's current device state. */
+typedef struct orr_trajectory orr_trajectory;
+ORR_API int orr_trajectory_open(orr_system* sys, const char* path, double timestep,
+                                int32_t with_velocities, orr_trajectory** out);
+ORR_API int orr_trajectory_append(orr_trajectory* trajectory, orr_system* sys, uint64_t step);
+ORR_API int orr_trajectory_close(orr_trajectory* trajectory);
 
 /* ---- batches: many independent systems of uneven size, stepped together --
  *

@@ -340,3 +340,70 @@ def relative_errors(ax, ay, az, rx, ry, rz):
 def error_summary(rel):
     """(worst, rms) with rms over all particles, as the reference's tests define it."""
     return float(rel.max()) if rel.size else 0.0, float(np.sqrt(np.sum(rel * rel) / max(1, rel.size)))
+
+
+# ---------------------------------------------------------------------------
+# The reference's own checkpoint and trajectory code (oracle/_ref), for format
+# parity. Each returns None when the reference library is not built.
+
+
+def ref_checkpoint_read(path, capacity):
+    R = load_reference()
+    if R is None:
+        return None
+    step, count = ctypes.c_uint64(), ctypes.c_uint64()
+    arrays = [np.zeros(capacity) for _ in range(10)]
+    ptrs = (ctypes.c_void_p * 10)(*[a.ctypes.data for a in arrays])
+    text = ctypes.create_string_buffer(1 << 16)
+    R.ref_checkpoint_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_uint64]
+    rc = R.ref_checkpoint_read(os.fsencode(path), ctypes.byref(step), ctypes.byref(count), capacity,
+                               ptrs, text, len(text))
+    if rc != 0:
+        return {"refused": rc}
+    n = count.value
+    names = ("m", "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az")
+    out = {k: a[:n].copy() for k, a in zip(names, arrays)}
+    out.update(step=step.value, count=n, configuration=text.value.decode())
+    return out
+
+
+def ref_checkpoint_write(path, step, arrays, text=""):
+    """arrays in the order m, x, y, z, vx, vy, vz, ax, ay, az."""
+    R = load_reference()
+    if R is None:
+        return None
+    arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    ptrs = (ctypes.c_void_p * 10)(*[a.ctypes.data for a in arrays])
+    R.ref_checkpoint_write.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64,
+                                       ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p]
+    return R.ref_checkpoint_write(os.fsencode(path), int(step), len(arrays[0]), ptrs, text.encode())
+
+
+def ref_trajectory_read(path, capacity, max_frames):
+    R = load_reference()
+    if R is None:
+        return None
+    count, dt, vel = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_int()
+    masses = np.zeros(capacity)
+    steps = np.zeros(max_frames, dtype=np.uint64)
+    times = np.zeros(max_frames)
+    values = np.zeros(max_frames * 6 * capacity)
+    R.ref_trajectory_read.restype = ctypes.c_long
+    R.ref_trajectory_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                                      ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    frames = R.ref_trajectory_read(os.fsencode(path), ctypes.byref(count), ctypes.byref(dt),
+                                   ctypes.byref(vel), masses.ctypes.data, capacity, max_frames,
+                                   steps.ctypes.data, times.ctypes.data, values.ctypes.data)
+    if frames < 0:
+        return {"refused": True}
+    n = count.value
+    out = {"count": n, "timestep": dt.value, "velocities": bool(vel.value), "masses": masses[:n].copy(),
+           "frames": []}
+    for f in range(frames):
+        v = values[f * 6 * n: (f + 1) * 6 * n].reshape(6, n)  # frames packed at 6 n
+        out["frames"].append({"step": int(steps[f]), "time": float(times[f]), "values": v.copy()})
+    return out

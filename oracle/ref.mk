@@ -21,7 +21,8 @@ SOURCES := core/random core/diagnostics core/particle_data core/vec3_array \
            integrators/yoshida4 integrators/runge_kutta4 solvers/direct_kernel \
            solvers/direct_kernel_scalar \
            solvers/direct_solver solvers/reference_kernel solvers/morton solvers/octree \
-           solvers/tree_walk solvers/barnes_hut_solver
+           solvers/tree_walk solvers/barnes_hut_solver sim/binary_stream sim/config_file \
+           sim/configuration sim/checkpoint sim/trajectory
 
 OBJECTS := $(addprefix $(OUT)/,$(addsuffix .o,$(subst /,_,$(SOURCES)))) $(OUT)/avx2.o $(OUT)/ref_shim.o
 

@@ -11,6 +11,8 @@
 #include <cstring>
 #include <memory>
 #include <span>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "orrery/backend/work_stealing_executor.hpp"
@@ -22,6 +24,9 @@
 #include "orrery/initial_conditions/plummer.hpp"
 #include "orrery/initial_conditions/uniform_sphere.hpp"
 #include "orrery/integrators/integrator.hpp"
+#include "orrery/sim/checkpoint.hpp"
+#include "orrery/sim/config_file.hpp"
+#include "orrery/sim/trajectory.hpp"
 #include "orrery/integrators/runge_kutta4.hpp"
 #include "orrery/integrators/velocity_verlet.hpp"
 #include "orrery/integrators/yoshida4.hpp"
@@ -313,4 +318,102 @@ int ref_diagnostics(std::uint64_t n, const double* x, const double* y, const dou
     return 0;
 }
 
+
+// sim/checkpoint.cpp through the reference's own reader: step, count, the ten
+// arrays (m, x, y, z, vx, vy, vz, ax, ay, az, each `capacity` long) and the
+// configuration as the reference writes it back. 0 on success, 1 on refusal.
+int ref_checkpoint_read(const char* path, std::uint64_t* step, std::uint64_t* count,
+                        std::uint64_t capacity, double* const* arrays, char* text,
+                        std::uint64_t text_capacity) {
+    try {
+        const sim::Checkpoint c = sim::read_checkpoint(path);
+        *step = c.step;
+        *count = c.particles.size();
+        if (*count > capacity) return 2;
+        const auto& d = c.particles;
+        const std::span<const double> src[10] = {
+            d.masses(), d.positions().x, d.positions().y, d.positions().z,
+            d.velocities().x, d.velocities().y, d.velocities().z,
+            d.accelerations().x, d.accelerations().y, d.accelerations().z};
+        for (int k = 0; k < 10; ++k) {
+            for (std::size_t i = 0; i < *count; ++i) arrays[k][i] = src[k][i];
+        }
+        std::ostringstream out;
+        sim::write_configuration(out, c.configuration);
+        const std::string t = out.str();
+        const std::size_t n = std::min<std::size_t>(t.size(), text_capacity - 1);
+        std::memcpy(text, t.data(), n);
+        text[n] = '\0';
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
+// A checkpoint written by the reference: configuration parsed from `text`.
+int ref_checkpoint_write(const char* path, std::uint64_t step, std::uint64_t n,
+                         const double* const* arrays, const char* text) {
+    try {
+        sim::Checkpoint c;
+        std::istringstream in(text);
+        c.configuration = sim::parse_configuration(in, "shim");
+        c.step = step;
+        c.particles.resize(n);
+        auto& d = c.particles;
+        const std::span<double> dst[10] = {
+            d.masses(), d.positions().x, d.positions().y, d.positions().z,
+            d.velocities().x, d.velocities().y, d.velocities().z,
+            d.accelerations().x, d.accelerations().y, d.accelerations().z};
+        for (int k = 0; k < 10; ++k) {
+            for (std::size_t i = 0; i < n; ++i) dst[k][i] = arrays[k][i];
+        }
+        sim::write_checkpoint(path, c);
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
+// Frames of a trajectory through the reference's TrajectoryReader: the header
+// fields, then up to max_frames frames of step, time and 6 x n values
+// (x y z, then vx vy vz when present). Returns the frame count, or -1.
+long ref_trajectory_read(const char* path, std::uint64_t* count, double* timestep,
+                         int* has_velocities, double* masses, std::uint64_t capacity,
+                         std::uint64_t max_frames, std::uint64_t* steps, double* times,
+                         double* values) {
+    try {
+        sim::TrajectoryReader r(path);
+        *count = r.info().particle_count;
+        *timestep = r.info().timestep;
+        *has_velocities = r.info().has_velocities ? 1 : 0;
+        const std::size_t n = *count;
+        if (n > capacity) return -1;
+        for (std::size_t i = 0; i < n; ++i) masses[i] = r.masses()[i];
+        sim::TrajectoryFrame f;
+        long frames = 0;
+        while (static_cast<std::uint64_t>(frames) < max_frames && r.read_frame(f)) {
+            steps[frames] = f.step;
+            times[frames] = f.time;
+            double* v = values + static_cast<std::size_t>(frames) * 6 * n;
+            const auto p = f.positions.view();
+            for (std::size_t i = 0; i < n; ++i) {
+                v[i] = p.x[i];
+                v[n + i] = p.y[i];
+                v[2 * n + i] = p.z[i];
+            }
+            if (*has_velocities != 0) {
+                const auto u = f.velocities.view();
+                for (std::size_t i = 0; i < n; ++i) {
+                    v[3 * n + i] = u.x[i];
+                    v[4 * n + i] = u.y[i];
+                    v[5 * n + i] = u.z[i];
+                }
+            }
+            ++frames;
+        }
+        return frames;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
 }  // extern "C"

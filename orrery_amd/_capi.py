@@ -149,11 +149,18 @@ SIGNATURES = {
     "orr_ctx_reset_timings": [_vp],
     "orr_system_create": [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)],
     "orr_system_destroy": [_vp],
+    "orr_system_count": [_vp, ctypes.POINTER(ctypes.c_uint64)],
     "orr_system_upload": [_vp] + [_vp] * 7,
     "orr_system_download": [_vp] + [_vp] * 9,
     "orr_system_refresh": [_vp, ctypes.POINTER(OrrStepConfig)],
     "orr_system_step": [_vp, ctypes.POINTER(OrrStepConfig), ctypes.c_double, ctypes.c_uint64],
     "orr_system_diagnostics": [_vp, ctypes.c_double, ctypes.c_double, _d],
+    "orr_checkpoint_write": [_vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p, ctypes.c_uint64],
+    "orr_checkpoint_read": [_vp, ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_uint64),
+                            ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)],
+    "orr_trajectory_open": [_vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(_vp)],
+    "orr_trajectory_append": [_vp, _vp, ctypes.c_uint64],
+    "orr_trajectory_close": [_vp],
     "orr_batch_create": [_vp, ctypes.POINTER(OrrBatchSystem), ctypes.c_uint32,
                          ctypes.POINTER(OrrBatchConfig), ctypes.POINTER(_vp)],
     "orr_batch_destroy": [_vp],

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBRARY = os.path.join(HERE, "liborrery_hip.so")
-SOURCES = ["context.hip", "capi.hip", "direct.hip", "morton.hip", "scan.hip", "tree.hip", "batch.hip", "serve.hip",
+SOURCES = ["context.hip", "capi.hip", "direct.hip", "morton.hip", "scan.hip", "tree.hip", "batch.hip", "serve.hip", "io.hip",
            "integrate.hip", "probes.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

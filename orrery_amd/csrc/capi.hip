@@ -34,6 +34,24 @@ struct orr_system {
     double* stage(int k) const { return stages.as<double>() + static_cast<std::size_t>(k) * count; }
 };
 
+std::uint64_t orr::system_count(const orr_system* s) { return s->count; }
+
+void orr::system_download(orr_system* s, int first, int arrays, double* host) {
+    ORR_HIP(hipSetDevice(s->ctx->device));
+    if (s->count == 0) return;
+    ORR_HIP(hipMemcpyAsync(host, s->array(first), sizeof(double) * arrays * s->count,
+                           hipMemcpyDeviceToHost, s->ctx->stream));
+    s->ctx->sync();
+}
+
+void orr::system_upload(orr_system* s, int first, int arrays, const double* host) {
+    ORR_HIP(hipSetDevice(s->ctx->device));
+    if (s->count == 0) return;
+    ORR_HIP(hipMemcpyAsync(s->array(first), host, sizeof(double) * arrays * s->count,
+                           hipMemcpyHostToDevice, s->ctx->stream));
+    s->ctx->sync();
+}
+
 namespace {
 
 void field_eval(orr_system* s, const orr_step_config& c) {
@@ -370,6 +388,13 @@ int orr_system_create(orr_ctx* ctx, uint64_t count, orr_system** out) {
         ORR_HIP(hipMemsetAsync(s->state.ptr, 0, s->state.bytes, ctx->stream));
         ctx->sync();
         *out = s.release();
+    });
+}
+
+int orr_system_count(orr_system* s, uint64_t* count) {
+    return guarded([&] {
+        require(s != nullptr && count != nullptr, "null argument");
+        *count = s->count;
     });
 }
 

@@ -229,6 +229,12 @@ void tree_copy_nodes(orr_ctx* ctx, double* mass, double* cx, double* cy, double*
                      double* acc, std::uint64_t* next, std::uint64_t* first,
                      std::uint64_t* count);
 
+// capi.hip: a device-resident system's size and contiguous arrays (k = 0..9:
+// x y z vx vy vz ax ay az m), copied `arrays` at a time from array `first`.
+std::uint64_t system_count(const orr_system* s);
+void system_download(orr_system* s, int first, int arrays, double* host);
+void system_upload(orr_system* s, int first, int arrays, const double* host);
+
 // batch.hip, for the serving loop: enqueue steps of a batch on its own streams
 // without waiting (batches run concurrently), then wait for them.
 void batch_step_enqueue(orr_batch* b, std::uint64_t steps);

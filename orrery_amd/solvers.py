@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -501,6 +502,28 @@ class DeviceSystem:
         c = config.native()
         C.check(C.load().orr_system_step(self.handle, ctypes.byref(c), float(timestep), int(steps)))
 
+    def checkpoint(self, path: str, step: int, configuration: str = "") -> None:
+        """Write the reference's checkpoint format (sim/checkpoint.cpp) from device state."""
+        text = configuration.encode()
+        C.check(C.load().orr_checkpoint_write(self.handle, os.fsencode(path), int(step), text, len(text)))
+
+    @classmethod
+    def from_checkpoint(cls, ctx: Context, path: str):
+        """(system, step, configuration text) from a checkpoint file."""
+        h = ctypes.c_void_p()
+        step = ctypes.c_uint64()
+        length = ctypes.c_uint64()
+        buf = ctypes.create_string_buffer(1 << 20)
+        C.check(C.load().orr_checkpoint_read(ctx.handle, os.fsencode(path), ctypes.byref(h),
+                                             ctypes.byref(step), buf, len(buf), ctypes.byref(length)))
+        s = cls.__new__(cls)
+        s.ctx = ctx
+        s.handle = h
+        count = ctypes.c_uint64()
+        C.check(C.load().orr_system_count(h, ctypes.byref(count)))
+        s.count = count.value
+        return s, step.value, buf.value.decode()
+
     def diagnostics(self, softening: float, gravitational_constant: float = 1.0) -> dict:
         out = np.zeros(8)
         C.check(C.load().orr_system_diagnostics(self.handle, float(softening) * float(softening),
@@ -705,4 +728,28 @@ class Server:
 
     def drain(self) -> None:
         while self.round() > 0:
+            pass
+
+
+class Trajectory:
+    """The reference's trajectory format (sim/trajectory.cpp) written from device state."""
+
+    def __init__(self, system: DeviceSystem, path: str, timestep: float, velocities: bool = False):
+        h = ctypes.c_void_p()
+        C.check(C.load().orr_trajectory_open(system.handle, os.fsencode(path), float(timestep),
+                                             1 if velocities else 0, ctypes.byref(h)))
+        self.handle = h
+
+    def append(self, system: DeviceSystem, step: int) -> None:
+        C.check(C.load().orr_trajectory_append(self.handle, system.handle, int(step)))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            C.check(C.load().orr_trajectory_close(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
             pass
