@@ -456,6 +456,27 @@ void batch_step_enqueue(orr_batch* b, std::uint64_t steps) { step_enqueue(b, ste
 void batch_wait(orr_batch* b) { sync_all(b); }
 hipStream_t batch_stream(orr_batch* b) { return b->dctx->stream; }
 
+void batch_refresh_enqueue(orr_batch* b) { evaluate(b); }
+
+void batch_diagnostics_enqueue(orr_batch* b, std::uint32_t system, double* out) {
+    const std::uint32_t s = internal_of(b, system);
+    const orr_batch_system& sys = b->systems[system];
+    const std::uint64_t begin = b->first[s], count = b->first[s + 1] - begin;
+    diagnostics_enqueue(b->dctx, count, b->array(0) + begin, b->array(1) + begin,
+                        b->array(2) + begin, b->array(3) + begin, b->array(4) + begin,
+                        b->array(5) + begin, b->array(9) + begin, sys.softening_squared,
+                        sys.gravitational_constant, out);
+}
+
+void batch_download_enqueue(orr_batch* b, std::uint32_t system, double* host) {
+    const std::uint32_t s = internal_of(b, system);
+    const std::uint64_t begin = b->first[s], count = b->first[s + 1] - begin;
+    for (int k = 0; k < 6 && count > 0; ++k) {
+        ORR_HIP(hipMemcpyAsync(host + static_cast<std::size_t>(k) * count, b->array(k) + begin,
+                               count * sizeof(double), hipMemcpyDeviceToHost, b->dctx->stream));
+    }
+}
+
 }  // namespace orr
 
 using orr::guarded;
@@ -639,12 +660,9 @@ int orr_batch_diagnostics(orr_batch* b, uint32_t system, double out[8]) {
         const std::uint32_t s = orr::internal_of(b, system);
         orr::use_device(b);
         orr::sync_all(b);
-        const orr_batch_system& sys = b->systems[system];
-        const std::uint64_t begin = b->first[s], count = b->first[s + 1] - begin;
-        orr::diagnostics(b->dctx, count, b->array(0) + begin, b->array(1) + begin,
-                         b->array(2) + begin, b->array(3) + begin, b->array(4) + begin,
-                         b->array(5) + begin, b->array(9) + begin, sys.softening_squared,
-                         sys.gravitational_constant, out);
+        (void)s;
+        orr::batch_diagnostics_enqueue(b, system, out);
+        orr::sync_all(b);
     });
 }
 

@@ -66,7 +66,8 @@ void free_context(orr_ctx* ctx) noexcept {
                       &ctx->order_codes, &ctx->order_index, &ctx->order_codes_alt,
                       &ctx->order_index_alt, &ctx->order_hist, &ctx->order_bbox,
                       &ctx->order_flags, &ctx->tree_nodes, &ctx->tree_aux, &ctx->tree_work,
-                      &ctx->tree_counts, &ctx->tree_sorted, &ctx->tree_leafref}) {
+                      &ctx->tree_counts, &ctx->tree_sorted, &ctx->tree_leafref,
+                      &ctx->diag_scratch}) {
         b->release();
     }
     for (auto& p : ctx->pending) {

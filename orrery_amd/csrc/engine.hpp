@@ -104,6 +104,7 @@ struct orr_ctx {
     orr::Buffer order_codes, order_index, order_codes_alt, order_index_alt, order_hist,
         order_bbox, order_flags;
     orr::Buffer tree_nodes, tree_aux, tree_work, tree_counts, tree_sorted, tree_leafref;
+    orr::Buffer diag_scratch;
 
     // Direct-kernel configuration (orr_ctx_set_direct_config).
     int direct_targets_per_lane = 1;
@@ -241,6 +242,12 @@ void batch_step_enqueue(orr_batch* b, std::uint64_t steps);
 void batch_wait(orr_batch* b);
 // The stream every step of the batch starts and ends on.
 hipStream_t batch_stream(orr_batch* b);
+// Without waiting, on the batch's stream: accelerations for the current
+// positions; a system's diagnostics into `out` (8 doubles); its x y z vx vy vz
+// into `host` (6 x count doubles). Pinned host memory keeps the copies async.
+void batch_refresh_enqueue(orr_batch* b);
+void batch_diagnostics_enqueue(orr_batch* b, std::uint32_t system, double* out);
+void batch_download_enqueue(orr_batch* b, std::uint32_t system, double* host);
 
 // probes.hip: machine-ceiling probes; kind 0 fma, 1 rsq, 2 read, 3 copy.
 void probe(orr_ctx* ctx, int kind, std::uint64_t size, double* rate, double* ms);
@@ -265,5 +272,10 @@ void rk4(orr_ctx* ctx, std::uint64_t n, const Rk4Arrays& arrays, int op, double 
 void diagnostics(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
                  const double* z, const double* vx, const double* vy, const double* vz,
                  const double* m, double softening_squared, double G, double out[8]);
+// Enqueued on ctx->stream without waiting; `out` (8 doubles, best pinned host
+// memory) holds {kinetic, potential, p, L} once the stream reaches it.
+void diagnostics_enqueue(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
+                         const double* z, const double* vx, const double* vy, const double* vz,
+                         const double* m, double softening_squared, double G, double* out);
 
 }  // namespace orr

@@ -127,6 +127,9 @@ __global__ __launch_bounds__(64) void chunk_sums(std::uint64_t n, const double* 
     out[2 * id + 1] = s.correction;
 }
 
+// One lane per term: the chunk results summed in chunk order, then written in
+// the output order {kinetic, potential, p, L}, kinetic halved
+// (diagnostics.cpp:197).
 __global__ void final_sums(std::uint32_t chunks, const double* __restrict__ parts,
                            double* __restrict__ result) {
     const int term = threadIdx.x;
@@ -136,7 +139,14 @@ __global__ void final_sums(std::uint32_t chunks, const double* __restrict__ part
         s.add(parts[2 * (term * chunks + c)]);
         s.add(parts[2 * (term * chunks + c) + 1]);
     }
-    result[term] = s.value();
+    const double v = s.value();
+    if (term == 0) {
+        result[0] = v / 2;
+    } else if (term == 7) {
+        result[1] = v;
+    } else {
+        result[1 + term] = v;
+    }
 }
 
 }  // namespace
@@ -164,40 +174,35 @@ void drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, const
     kick(ctx, n, x, y, z, vx, vy, vz, s);
 }
 
+void diagnostics_enqueue(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
+                         const double* z, const double* vx, const double* vy, const double* vz,
+                         const double* m, double e2, double G, double* out) {
+    if (n == 0) {
+        for (int k = 0; k < 8; ++k) out[k] = 0.0;
+        return;
+    }
+    const std::uint32_t chunks = static_cast<std::uint32_t>(n < 1024 ? n : 1024);
+    ctx->diag_scratch.reserve(sizeof(double) * (kTerms * n + 2 * kTerms * chunks + kTerms));
+    double* terms = ctx->diag_scratch.as<double>();
+    double* parts = terms + kTerms * n;
+    double* result = parts + 2 * kTerms * chunks;
+    PhaseTimer t(ctx, kEnergy);
+    hipLaunchKernelGGL(diagnostic_terms, dim3(static_cast<std::uint32_t>((n + 255) / 256)),
+                       dim3(256), 0, ctx->stream, n, x, y, z, vx, vy, vz, m, e2, G, terms);
+    ORR_LAUNCHED("diagnostic_terms");
+    hipLaunchKernelGGL(chunk_sums, dim3((kTerms * chunks + 63) / 64), dim3(64), 0, ctx->stream, n,
+                       terms, chunks, parts);
+    ORR_LAUNCHED("chunk_sums");
+    hipLaunchKernelGGL(final_sums, dim3(1), dim3(64), 0, ctx->stream, chunks, parts, result);
+    ORR_LAUNCHED("final_sums");
+    ORR_HIP(hipMemcpyAsync(out, result, sizeof(double) * kTerms, hipMemcpyDeviceToHost, ctx->stream));
+}
+
 void diagnostics(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
                  const double* z, const double* vx, const double* vy, const double* vz,
                  const double* m, double e2, double G, double out[8]) {
-    for (int k = 0; k < 8; ++k) out[k] = 0.0;
-    if (n == 0) return;
-    const std::uint32_t chunks = static_cast<std::uint32_t>(n < 1024 ? n : 1024);
-    const std::size_t bytes = sizeof(double) * (kTerms * n + 2 * kTerms * chunks + kTerms);
-    Buffer scratch;
-    scratch.reserve(bytes);
-    double* terms = scratch.as<double>();
-    double* parts = terms + kTerms * n;
-    double* result = parts + 2 * kTerms * chunks;
-    try {
-        PhaseTimer t(ctx, kEnergy);
-        hipLaunchKernelGGL(diagnostic_terms, dim3(static_cast<std::uint32_t>((n + 255) / 256)),
-                           dim3(256), 0, ctx->stream, n, x, y, z, vx, vy, vz, m, e2, G, terms);
-        ORR_LAUNCHED("diagnostic_terms");
-        hipLaunchKernelGGL(chunk_sums, dim3((kTerms * chunks + 63) / 64), dim3(64), 0,
-                           ctx->stream, n, terms, chunks, parts);
-        ORR_LAUNCHED("chunk_sums");
-        hipLaunchKernelGGL(final_sums, dim3(1), dim3(64), 0, ctx->stream, chunks, parts, result);
-        ORR_LAUNCHED("final_sums");
-    } catch (...) {
-        scratch.release();
-        throw;
-    }
-    double host[kTerms];
-    ORR_HIP(hipMemcpyAsync(host, result, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+    diagnostics_enqueue(ctx, n, x, y, z, vx, vy, vz, m, e2, G, out);
     ctx->sync();
-    scratch.release();
-    // {kinetic, potential, p, L}: kinetic is half the summed m v^2 (diagnostics.cpp:197).
-    out[0] = host[0] / 2;
-    out[1] = host[7];
-    for (int k = 0; k < 6; ++k) out[2 + k] = host[1 + k];
 }
 
 }  // namespace orr

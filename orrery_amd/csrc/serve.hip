@@ -11,13 +11,14 @@
 //      integrator (and one softening and G) share a batch, requests of
 //      `alone_from` particles or more get their own; a continuing request's
 //      state moves device to device, a new one is uploaded and its starting
-//      diagnostics taken;
+//      diagnostics and the refresh are enqueued on the batch's stream;
 //   3. enqueue a quantum of steps on every batch -- each batch on its own
 //      streams, so the batches run concurrently -- sized so that the round
 //      lasts about `round_ms` by the batch's last measured step time, and never
 //      past the nearest finish of its members; wait for all;
-//   4. retire finished requests: final diagnostics, state back to the host,
-//      memory returned to the budget.
+//   4. retire the requests whose last quantum ran: their end diagnostics and
+//      final state were enqueued behind that quantum on their batch's stream,
+//      so they overlap the other batches' work; memory returns to the budget.
 //
 // A request's arithmetic does not depend on its batch (see batch.hip), nor on
 // how its steps are split into quanta, so its result is the same alone and in
@@ -48,15 +49,23 @@ enum : int { kQueued = 0, kRunning = 1, kDone = 2 };
 
 struct Request {
     orr_request spec{};
-    std::vector<double> host[7];  // initial state while queued; final x..vz when done
-    bool host_valid = true;
+    std::vector<double> host[7];  // initial state while queued
     int state = kQueued;
-    bool fresh = true;  // not yet on the device
+    bool fresh = true;      // not yet on the device
+    bool finishing = false;  // its last quantum, end diagnostics and copy are enqueued
     std::uint64_t steps_done = 0;
     double submitted = 0, admitted = 0, completed = 0;
-    double diag0[8]{}, diag1[8]{};
+    double* diag = nullptr;    // pinned: start diagnostics [0, 8), end [8, 16)
+    double* result = nullptr;  // pinned: final x y z vx vy vz once done, until taken
+    bool taken = false;
     std::uint64_t bytes = 0;
 };
+
+void release(Request& r) noexcept {
+    if (r.diag != nullptr) (void)hipHostFree(r.diag);
+    if (r.result != nullptr) (void)hipHostFree(r.result);
+    r.diag = r.result = nullptr;
+}
 
 struct GroupKey {
     int integrator;
@@ -147,17 +156,17 @@ void rebuild(orr_server* s, const GroupKey& key, Group& g) {
                     call(orr_batch_upload(nb, k, r.host[0].data(), r.host[1].data(), r.host[2].data(),
                                           r.host[3].data(), r.host[4].data(), r.host[5].data(),
                                           r.host[6].data()));
-                    call(orr_batch_diagnostics(nb, k, r.diag0));
                     for (auto& v : r.host) std::vector<double>().swap(v);
-                    r.host_valid = false;
                     r.fresh = false;
+                    // Start diagnostics on the batch's stream, ahead of its steps.
+                    orr::batch_diagnostics_enqueue(nb, k, r.diag);
                 } else {
                     call(orr_batch_copy_system(nb, k, old, g.slot.at(id)));
                 }
             }
-            // Accelerations for the current positions; a continuing request's
-            // come out as the bits it already had.
-            call(orr_batch_refresh(nb));
+            // Accelerations for the current positions (a continuing request's
+            // come out as the bits it already had), enqueued before the steps.
+            orr::batch_refresh_enqueue(nb);
             g.slot = std::move(slot);
         } catch (...) {
             orr_batch_destroy(nb);
@@ -185,6 +194,12 @@ void admit(orr_server* s) {
             ++it;
             continue;
         }
+        // Pinned host memory for the diagnostics and the final state, taken at
+        // admission so that nothing allocates while batches run.
+        if (r.diag == nullptr) ORR_HIP(hipHostMalloc(reinterpret_cast<void**>(&r.diag), 16 * sizeof(double)));
+        if (r.result == nullptr) {
+            ORR_HIP(hipHostMalloc(reinterpret_cast<void**>(&r.result), 6 * r.spec.count * sizeof(double)));
+        }
         r.state = kRunning;
         r.admitted = s->now();
         s->resident += r.bytes;
@@ -196,21 +211,17 @@ void admit(orr_server* s) {
     }
 }
 
+// After a round: requests whose last quantum ran (their end diagnostics and
+// final state were enqueued behind it) are done.
 void retire(orr_server* s, Group& g) {
     std::vector<std::uint64_t> keep;
     for (std::uint64_t id : g.members) {
         Request& r = s->requests.at(id);
-        if (r.steps_done < r.spec.steps) {
+        if (!r.finishing) {
             keep.push_back(id);
             continue;
         }
-        const std::uint32_t k = g.slot.at(id);
-        call(orr_batch_diagnostics(g.batch, k, r.diag1));
-        for (int a = 0; a < 6; ++a) r.host[a].assign(r.spec.count, 0.0);
-        call(orr_batch_download(g.batch, k, r.host[0].data(), r.host[1].data(), r.host[2].data(),
-                                r.host[3].data(), r.host[4].data(), r.host[5].data(), nullptr,
-                                nullptr, nullptr));
-        r.host_valid = true;
+        r.finishing = false;
         r.state = kDone;
         r.completed = s->now();
         s->resident -= r.bytes;
@@ -245,6 +256,15 @@ void round(orr_server* s) {
         ORR_HIP(hipEventRecord(g.start, orr::batch_stream(g.batch)));
         orr::batch_step_enqueue(g.batch, g.quantum);
         ORR_HIP(hipEventRecord(g.stop, orr::batch_stream(g.batch)));
+        // Members this quantum finishes: end diagnostics and final state behind it.
+        for (std::uint64_t id : g.members) {
+            Request& r = s->requests.at(id);
+            if (r.spec.steps - r.steps_done != g.quantum) continue;
+            const std::uint32_t k = g.slot.at(id);
+            orr::batch_diagnostics_enqueue(g.batch, k, r.diag + 8);
+            orr::batch_download_enqueue(g.batch, k, r.result);
+            r.finishing = true;
+        }
         running.push_back(&g);
     }
     for (Group* g : running) orr::batch_wait(g->batch);
@@ -279,6 +299,7 @@ void destroy(orr_server* s) noexcept {
         if (g.batch != nullptr) orr_batch_destroy(g.batch);
         release_events(g);
     }
+    for (auto& [id, r] : s->requests) release(r);
     delete s;
 }
 
@@ -380,8 +401,10 @@ int orr_server_status(orr_server* s, uint64_t id, orr_request_status* out) {
         out->submitted_s = r.submitted;
         out->admitted_s = r.admitted;
         out->completed_s = r.completed;
-        std::memcpy(out->diagnostics_start, r.diag0, sizeof(r.diag0));
-        std::memcpy(out->diagnostics_end, r.diag1, sizeof(r.diag1));
+        // Start diagnostics exist once a round has run since admission, end
+        // diagnostics once the request is done.
+        if (r.diag != nullptr && r.state != kQueued) std::memcpy(out->diagnostics_start, r.diag, 8 * sizeof(double));
+        if (r.diag != nullptr && r.state == kDone) std::memcpy(out->diagnostics_end, r.diag + 8, 8 * sizeof(double));
     });
 }
 
@@ -391,13 +414,17 @@ int orr_server_result(orr_server* s, uint64_t id, double* x, double* y, double* 
         require(s != nullptr, "null server");
         Request& r = find(s, id);
         require(r.state == kDone, "request not finished");
-        require(r.host_valid, "result already taken");
+        require(!r.taken, "result already taken");
         double* dst[6] = {x, y, z, vx, vy, vz};
         for (int a = 0; a < 6; ++a) {
-            if (dst[a] != nullptr) std::memcpy(dst[a], r.host[a].data(), r.spec.count * sizeof(double));
+            if (dst[a] != nullptr) {
+                std::memcpy(dst[a], r.result + static_cast<std::size_t>(a) * r.spec.count,
+                            r.spec.count * sizeof(double));
+            }
         }
-        for (auto& v : r.host) std::vector<double>().swap(v);
-        r.host_valid = false;
+        (void)hipHostFree(r.result);
+        r.result = nullptr;
+        r.taken = true;
     });
 }
 

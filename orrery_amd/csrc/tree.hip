@@ -268,6 +268,13 @@ __global__ __launch_bounds__(kBlock) void tree_gather(
                               static_cast<float>(pz - g.z), static_cast<float>(pm));
 }
 
+// The walk's escape index: the DFS successor, or "done" when the node's cell
+// ends its system (the successor then belongs to the next system or past M).
+__device__ inline std::uint32_t walk_escape(std::uint32_t next, std::uint32_t cell_end,
+                                            std::uint32_t system_end) {
+    return cell_end >= system_end ? 0xffffffffu : next;
+}
+
 __device__ inline void finish_node(std::uint32_t node, double mass, double cx, double cy, double cz,
                                    const double4 geo, double theta, const double* __restrict__ cube,
                                    bool leaf, std::uint32_t begin, std::uint32_t count,
@@ -324,13 +331,16 @@ __global__ __launch_bounds__(kBlock) void leaf_moments(
     const std::uint32_t* __restrict__ node_count, const std::uint32_t* __restrict__ node_next,
     const double4* __restrict__ node_geo, const double4* __restrict__ sorted, double theta,
     const double* __restrict__ cubes, const std::uint32_t* __restrict__ seg_of,
+    const std::uint32_t* __restrict__ seg_start, std::uint32_t n,
     double4* __restrict__ moment, WalkNode* __restrict__ walk,
     double* __restrict__ quad64, QuadF* __restrict__ quadf) {
     const std::uint32_t M = counters[0];
     for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
         const std::uint32_t b = node_begin[node];
         if (node_level[node] != leaf_level_of_first[b]) continue;  // internal
-        const double* cube = cubes + (seg_of != nullptr ? 4 * static_cast<std::size_t>(seg_of[b]) : 0);
+        const std::uint32_t seg = seg_of != nullptr ? seg_of[b] : 0u;
+        const double* cube = cubes + 4 * static_cast<std::size_t>(seg);
+        const std::uint32_t system_end = seg_start != nullptr ? seg_start[seg + 1] : n;
         const std::uint32_t e = b + node_count[node];
         double mass = 0, wx = 0, wy = 0, wz = 0;
         for (std::uint32_t i = b; i < e; ++i) {
@@ -351,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void leaf_moments(
             cz = sorted[b].z;
         }
         finish_node(node, mass, cx, cy, cz, node_geo[node], theta, cube, true, b, e - b,
-                    node_next[node], moment, walk);
+                    walk_escape(node_next[node], e, system_end), moment, walk);
         if (quad64 != nullptr) {
             double q[6] = {0, 0, 0, 0, 0, 0};
             for (std::uint32_t i = b; i < e; ++i) {
@@ -368,16 +378,20 @@ __global__ __launch_bounds__(kBlock) void leaf_moments(
 __global__ __launch_bounds__(kBlock) void internal_moments(
     unsigned level, const std::uint32_t* __restrict__ counters,
     const std::uint8_t* __restrict__ node_level, const std::uint8_t* __restrict__ leaf_level_of_first,
-    const std::uint32_t* __restrict__ node_begin, const std::uint32_t* __restrict__ node_next,
+    const std::uint32_t* __restrict__ node_begin, const std::uint32_t* __restrict__ node_count,
+    const std::uint32_t* __restrict__ node_next,
     const double4* __restrict__ node_geo, double theta, const double* __restrict__ cubes,
-    const std::uint32_t* __restrict__ seg_of, double4* __restrict__ moment,
+    const std::uint32_t* __restrict__ seg_of, const std::uint32_t* __restrict__ seg_start,
+    std::uint32_t n, double4* __restrict__ moment,
     WalkNode* __restrict__ walk, double* __restrict__ quad64, QuadF* __restrict__ quadf) {
     const std::uint32_t M = counters[0];
     for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
         if (node_level[node] != level) continue;
         const std::uint32_t b = node_begin[node];
         if (leaf_level_of_first[b] == level) continue;  // a leaf
-        const double* cube = cubes + (seg_of != nullptr ? 4 * static_cast<std::size_t>(seg_of[b]) : 0);
+        const std::uint32_t seg = seg_of != nullptr ? seg_of[b] : 0u;
+        const double* cube = cubes + 4 * static_cast<std::size_t>(seg);
+        const std::uint32_t system_end = seg_start != nullptr ? seg_start[seg + 1] : n;
         const std::uint32_t end = node_next[node];
         double mass = 0, wx = 0, wy = 0, wz = 0, mx = 0, my = 0, mz = 0;
         std::uint32_t children = 0;
@@ -404,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void internal_moments(
             cz = mz / k;
         }
         finish_node(node, mass, cx, cy, cz, node_geo[node], theta, cube, false, b, 0,
-                    node_next[node], moment, walk);
+                    walk_escape(node_next[node], b + node_count[node], system_end), moment, walk);
         if (quad64 != nullptr) {
             double q[6] = {0, 0, 0, 0, 0, 0};
             for (std::uint32_t c = node + 1; c < end; c = max(node_next[c], c + 1)) {
@@ -473,14 +487,10 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     const float4 oi = leaf_rel[ic];
     const std::uint32_t own = leaf_of[ic];
     const double4 ca = node_geo[own];
-    // This lane's system: its nodes are [root, node_end) in DFS order.
+    // This lane's system starts at its root; the walk records' escape indices
+    // end the walk at the system's last subtree.
     const std::uint32_t seg = seg_of != nullptr ? seg_of[ic] : 0u;
-    std::uint32_t root = 0, node_end = M;
-    if (seg_start != nullptr) {
-        root = first_node[seg_start[seg]];
-        const std::uint32_t e = seg_start[seg + 1];
-        node_end = e >= n ? M : first_node[e];
-    }
+    const std::uint32_t root = seg_start != nullptr ? first_node[seg_start[seg]] : 0u;
     std::uint32_t mine = valid ? root : kDone;
     float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     float px = 0.f, py = 0.f, pz = 0.f;
@@ -505,9 +515,9 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
             asm volatile("" ::"s"(Q.a.x), "s"(Q.a.y), "s"(Q.a.z), "s"(Q.a.w), "s"(Q.b.x),
                          "s"(Q.b.y));
         }
-        // An escape index always lies past its node; the guard only makes a
-        // corrupted tree end the walk instead of looping.
-        const std::uint32_t next = nd.b.y > cur ? nd.b.y : node_end;
+        // An escape index always lies past its node (or is kDone); the guard only
+        // makes a corrupted tree end the walk instead of looping.
+        const std::uint32_t next = nd.b.y > cur ? nd.b.y : kDone;
         bool leaf_pp = false;
         if (mine == cur) {
             ++visits;
@@ -587,7 +597,6 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
                 px = py = pz = 0.f;
             }
         }
-        if (mine >= node_end) mine = kDone;
         const std::uint32_t step = cur + 1;
         cur = (__ballot(mine == step) != 0) ? step
                                             : __builtin_amdgcn_readfirstlane(wave_min_u32(mine));
@@ -765,13 +774,13 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
         PhaseTimer t(ctx, kMoments);
         hipLaunchKernelGGL(leaf_moments, dim3(node_blocks), dim3(kBlock), 0, ctx->stream, L.c32,
                            L.level, L.leaf_level, L.begin, L.count, L.next, L.geo, L.sorted, theta,
-                           cube, seg_of, L.moment, L.walk, L.quad64, L.quadf);
+                           cube, seg_of, seg_start, n32, L.moment, L.walk, L.quad64, L.quadf);
         ORR_LAUNCHED("leaf_moments");
         for (int level = static_cast<int>(kLevels) - 1; level >= 0; --level) {
             hipLaunchKernelGGL(internal_moments, dim3(node_blocks), dim3(kBlock), 0, ctx->stream,
                                static_cast<unsigned>(level), L.c32, L.level, L.leaf_level, L.begin,
-                               L.next, L.geo, theta, cube, seg_of, L.moment, L.walk, L.quad64,
-                               L.quadf);
+                               L.count, L.next, L.geo, theta, cube, seg_of, seg_start, n32,
+                               L.moment, L.walk, L.quad64, L.quadf);
             ORR_LAUNCHED("internal_moments");
         }
     }
