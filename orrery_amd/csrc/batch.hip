@@ -543,6 +543,9 @@ int orr_batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t cou
         ORR_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
         ORR_HIP(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
         b->state.reserve(sizeof(double) * 10 * std::max<std::uint64_t>(total, 1));
+        std::uint64_t largest = 1;
+        for (std::uint32_t u = 0; u < count; ++u) largest = std::max<std::uint64_t>(largest, systems[u].count);
+        b->dctx->diag_scratch.reserve(orr::diagnostics_scratch_bytes(largest));
         ORR_HIP(hipMemsetAsync(b->state.ptr, 0, b->state.bytes, b->dctx->stream));
         orr::plan(b.get());
         *out = b.release();

@@ -273,7 +273,11 @@ void diagnostics(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y
                  const double* z, const double* vx, const double* vy, const double* vz,
                  const double* m, double softening_squared, double G, double out[8]);
 // Enqueued on ctx->stream without waiting; `out` (8 doubles, best pinned host
-// memory) holds {kinetic, potential, p, L} once the stream reaches it.
+// memory) holds {kinetic, potential, p, L} once the stream reaches it. The
+// scratch (ctx->diag_scratch) grows only here: callers that enqueue several
+// reserve diagnostics_scratch_bytes(largest n) first, so no growth frees memory
+// that queued kernels still use.
+std::size_t diagnostics_scratch_bytes(std::uint64_t n);
 void diagnostics_enqueue(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
                          const double* z, const double* vx, const double* vy, const double* vz,
                          const double* m, double softening_squared, double G, double* out);

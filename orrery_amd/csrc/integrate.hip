@@ -174,6 +174,11 @@ void drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, const
     kick(ctx, n, x, y, z, vx, vy, vz, s);
 }
 
+std::size_t diagnostics_scratch_bytes(std::uint64_t n) {
+    const std::uint64_t chunks = n < 1024 ? n : 1024;
+    return sizeof(double) * (kTerms * n + 2 * kTerms * chunks + kTerms);
+}
+
 void diagnostics_enqueue(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
                          const double* z, const double* vx, const double* vy, const double* vz,
                          const double* m, double e2, double G, double* out) {
@@ -182,7 +187,7 @@ void diagnostics_enqueue(orr_ctx* ctx, std::uint64_t n, const double* x, const d
         return;
     }
     const std::uint32_t chunks = static_cast<std::uint32_t>(n < 1024 ? n : 1024);
-    ctx->diag_scratch.reserve(sizeof(double) * (kTerms * n + 2 * kTerms * chunks + kTerms));
+    ctx->diag_scratch.reserve(diagnostics_scratch_bytes(n));
     double* terms = ctx->diag_scratch.as<double>();
     double* parts = terms + kTerms * n;
     double* result = parts + 2 * kTerms * chunks;
