@@ -213,3 +213,42 @@ def test_invalid_batches_are_rejected(ctx):
     b.step(2)
     assert b.info()["particles"] == 0
     b.close()
+
+
+def test_edge_systems_in_one_batch(ctx):
+    # Zero softening (masked self terms), tiny systems forced onto the tree,
+    # systems smaller than a leaf, and one direct system alone in its branch.
+    ns = [1, 2, 7, 40, 3000]
+    systems = [E.BatchSystem(1, 0.0, 1e-3, solver=0),      # single direct particle, eps = 0
+               E.BatchSystem(2, 0.01, 1e-3, solver=1),     # two bodies on the tree
+               E.BatchSystem(7, 0.01, 1e-3, solver=1),     # fewer than a leaf
+               E.BatchSystem(40, 0.01, 1e-3, solver=1),
+               E.BatchSystem(3000, 0.01, 1e-3, solver=1)]
+    b = E.DeviceBatch(ctx, systems, E.BatchConfig(opening_angle=0.5, leaf_capacity=16))
+    info = b.info()
+    assert info["direct_systems"] == 1 and info["tree_systems"] == 4
+    states = []
+    for k, n in enumerate(ns):
+        p = oracle.plummer(n, 50 + k) if n > 1 else oracle.Particles(*(np.array([v]) for v in (0.3, -0.2, 0.1, 0, 0, 0, 1.0)))
+        b.upload(k, *p.arrays())
+        states.append(p)
+    b.refresh()
+    got = b.download(0)
+    assert got["ax"][0] == 0 and got["ay"][0] == 0 and got["az"][0] == 0
+    for k in (1, 2):
+        # Below a leaf's capacity the tree is one leaf: every pair directly.
+        p = states[k]
+        ref = oracle.reference_acceleration(p.x, p.y, p.z, p.m, 1e-4)
+        g = b.download(k)
+        worst, rms = oracle.error_summary(oracle.relative_errors(g["ax"], g["ay"], g["az"], *ref))
+        assert worst < 1e-5, (ns[k], worst)
+    solver = E.HipTreeSolver(E.TreeParameters(0.5, 16), E.Softening(0.01), context=ctx)
+    for k in (3, 4):
+        p = states[k]
+        acc = (np.zeros(p.n), np.zeros(p.n), np.zeros(p.n))
+        solver.evaluate((p.x, p.y, p.z), p.m, acc)
+        g = b.download(k)
+        assert np.array_equal(g["ax"], acc[0]) and np.array_equal(g["az"], acc[2]), ns[k]
+    b.step(3)
+    assert np.all(np.isfinite(b.download(2)["x"]))
+    b.close()
