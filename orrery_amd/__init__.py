@@ -6,8 +6,10 @@ fallback and fails loudly when the library is not built.
 """
 from .solvers import (BatchConfig, BatchSystem, Context, DeviceArray, DeviceBatch, DeviceSystem,
                       HipDirectSolver, HipTreeSolver, InteractionCount, Server, ServerConfig,
-                      Softening, StepConfig, TreeParameters, describe, device_count, morton_order)
+                      Softening, StepConfig, Trajectory, TreeParameters, describe, device_count,
+                      morton_order)
 
 __all__ = ["BatchConfig", "BatchSystem", "Context", "DeviceArray", "DeviceBatch", "DeviceSystem",
            "HipDirectSolver", "HipTreeSolver", "InteractionCount", "Server", "ServerConfig",
-           "Softening", "StepConfig", "TreeParameters", "describe", "device_count", "morton_order"]
+           "Softening", "StepConfig", "Trajectory", "TreeParameters", "describe", "device_count",
+           "morton_order"]
