@@ -140,20 +140,39 @@ def test_steps_match_cpu_reference_path(ctx, integrator):
 
 def test_rk4_measured_order_and_tree_field(ctx):
     # Circular two-body orbit over one period (tests/integrators/convergence_test.cpp):
-    # RK4's closure error falls by 2^4 when the step halves.
+    # RK4's closure error falls by ~2^4 when the step halves. At 48 and 96 steps
+    # a period the error (4e-5, 2e-6) stands well above the FP32 force floor
+    # (~1e-7), where the FP64 oracle measures 4.35 (still pre-asymptotic); the
+    # device must reproduce the oracle's errors and so its order.
+    x0 = np.array([-0.5, 0.5])
+    v0 = np.array([-np.sqrt(0.5), np.sqrt(0.5)])
+    period = 2 * np.pi * np.sqrt(0.5)
+
     def closure(steps):
         s = E.DeviceSystem(ctx, 2)
         z = np.zeros(2)
-        s.upload(np.array([-0.5, 0.5]), z, z, z, np.array([-np.sqrt(0.5), np.sqrt(0.5)]), z,
-                 np.array([1.0, 1.0]))
+        s.upload(x0, z, z, z, v0, z, np.array([1.0, 1.0]))
         cfg = E.StepConfig(integrator="runge-kutta4", solver="direct", softening=0.0)
         s.refresh(cfg)
-        s.step(cfg, 2 * np.pi * np.sqrt(0.5) / steps, steps)
+        s.step(cfg, period / steps, steps)
         st = s.download()
         s.close()
         return np.hypot(st["x"][0] + 0.5, st["y"][0])
-    order = np.log2(closure(100) / closure(200))
-    assert abs(order - 4.0) < 0.2, order
+
+    def closure_oracle(steps):
+        st = oracle.Particles(x0.copy(), np.zeros(2), np.zeros(2), np.zeros(2), v0.copy(),
+                              np.zeros(2), np.array([1.0, 1.0]))
+        f = oracle.make_field("rk4", "direct", eps2=0.0, kernel=0)
+        acc = oracle.refresh(st, f)
+        oracle.steps(st, acc, f, period / steps, steps)
+        return np.hypot(st.x[0] + 0.5, st.y[0])
+
+    dev = [closure(48), closure(96)]
+    ref = [closure_oracle(48), closure_oracle(96)]
+    for d, r in zip(dev, ref):
+        assert abs(d - r) < 2e-7, (d, r)
+    order, order_ref = np.log2(dev[0] / dev[1]), np.log2(ref[0] / ref[1])
+    assert abs(order - order_ref) < 0.1 and abs(order - 4.0) < 0.5, (order, order_ref)
     # RK4 over the Barnes-Hut field against the oracle's RK4 over its FP64 walk.
     p = oracle.plummer(3000, 5)
     eps, dt, steps = 0.02, 1e-3, 3
