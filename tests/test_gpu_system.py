@@ -140,10 +140,10 @@ def test_steps_match_cpu_reference_path(ctx, integrator):
 
 def test_rk4_measured_order_and_tree_field(ctx):
     # Circular two-body orbit over one period (tests/integrators/convergence_test.cpp):
-    # RK4's closure error falls by ~2^4 when the step halves. At 48 and 96 steps
-    # a period the error (4e-5, 2e-6) stands well above the FP32 force floor
-    # (~1e-7), where the FP64 oracle measures 4.35 (still pre-asymptotic); the
-    # device must reproduce the oracle's errors and so its order.
+    # RK4's closure error falls by ~2^4 when the step halves. At 24 and 48 steps
+    # a period the error (8e-4, 4e-5) stands well above what FP32 forces add over
+    # a period (~3e-7 at 96 steps), where the FP64 oracle measures 4.5 (still
+    # pre-asymptotic); the device must reproduce the oracle's errors and order.
     x0 = np.array([-0.5, 0.5])
     v0 = np.array([-np.sqrt(0.5), np.sqrt(0.5)])
     period = 2 * np.pi * np.sqrt(0.5)
@@ -167,12 +167,12 @@ def test_rk4_measured_order_and_tree_field(ctx):
         oracle.steps(st, acc, f, period / steps, steps)
         return np.hypot(st.x[0] + 0.5, st.y[0])
 
-    dev = [closure(48), closure(96)]
-    ref = [closure_oracle(48), closure_oracle(96)]
+    dev = [closure(24), closure(48)]
+    ref = [closure_oracle(24), closure_oracle(48)]
     for d, r in zip(dev, ref):
-        assert abs(d - r) < 2e-7, (d, r)
+        assert abs(d - r) < 1e-6, (d, r)
     order, order_ref = np.log2(dev[0] / dev[1]), np.log2(ref[0] / ref[1])
-    assert abs(order - order_ref) < 0.1 and abs(order - 4.0) < 0.5, (order, order_ref)
+    assert abs(order - order_ref) < 0.05 and 3.8 < order < 4.7, (order, order_ref)
     # RK4 over the Barnes-Hut field against the oracle's RK4 over its FP64 walk.
     p = oracle.plummer(3000, 5)
     eps, dt, steps = 0.02, 1e-3, 3
