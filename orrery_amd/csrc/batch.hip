@@ -16,7 +16,7 @@
 //
 // A step is kick (all particles, each with its own system's scale), drift,
 // then two branches on two streams: the direct systems (segmented Morton order,
-// tile packing, one persistent work-queue kernel, fixed-order reduction) and
+// tile packing, one persistent work-item kernel, fixed-order reduction) and
 // the Barnes-Hut systems (segmented Morton order, octree build, moments, walk),
 // joined, then the closing kick. With use_graph the step is captured once as a
 // hipGraph with the two branches in parallel, and replayed.
@@ -99,7 +99,6 @@ struct orr_batch {
     uint4* ditem = nullptr;
     float* deps2 = nullptr;
     double* dG = nullptr;
-    unsigned* queue = nullptr;
     std::uint32_t *dstart = nullptr, *dchunks = nullptr, *dchunk_first = nullptr;
     std::uint32_t *tstart = nullptr, *tchunks = nullptr, *tchunk_first = nullptr;
     std::uint32_t dchunk_count = 0, tchunk_count = 0;
@@ -202,7 +201,7 @@ void plan(orr_batch* b) {
             slots += splits;
         }
     }
-    // Longest items first: the queue hands them out before the short ones.
+    // Longest items first: with the strided schedule every wave starts on a long one.
     std::stable_sort(ditem.begin(), ditem.end(),
                      [](const uint4& a, const uint4& c) { return (a.z - a.y) > (c.z - c.y); });
     b->tiles = static_cast<std::uint32_t>(dtile.size());
@@ -214,7 +213,7 @@ void plan(orr_batch* b) {
     const std::size_t bytes =
         padded(4 * seg_of.size()) + padded(8 * scale.size()) + padded(16 * dsys.size()) +
         padded(16 * dtile.size()) + padded(16 * ditem.size()) + padded(4 * deps2.size()) +
-        padded(8 * dG.size()) + padded(4) + padded(4 * dstart.size()) + padded(4 * dchunks.size()) +
+        padded(8 * dG.size()) + padded(4 * dstart.size()) + padded(4 * dchunks.size()) +
         padded(4 * dchunk_first.size()) + padded(4 * tstart.size()) + padded(4 * tchunks.size()) +
         padded(4 * tchunk_first.size()) + 256;
     b->tables.reserve(bytes);
@@ -226,7 +225,6 @@ void plan(orr_batch* b) {
     b->ditem = carve<uint4>(p, ditem.size());
     b->deps2 = carve<float>(p, deps2.size());
     b->dG = carve<double>(p, dG.size());
-    b->queue = carve<unsigned>(p, 1);
     b->dstart = carve<std::uint32_t>(p, dstart.size());
     b->dchunks = carve<std::uint32_t>(p, dchunks.size());
     b->dchunk_first = carve<std::uint32_t>(p, dchunk_first.size());
@@ -245,7 +243,6 @@ void plan(orr_batch* b) {
     put(b->ditem, ditem);
     put(b->deps2, deps2);
     put(b->dG, dG);
-    ORR_HIP(hipMemsetAsync(b->queue, 0, sizeof(unsigned), st));
     put(b->dstart, dstart);
     put(b->dchunks, dchunks);
     put(b->dchunk_first, dchunk_first);
@@ -281,7 +278,6 @@ void evaluate_direct(orr_batch* b) {
     v.seg_of = b->seg_of;
     v.eps2 = b->deps2;
     v.G = b->dG;
-    v.queue = b->queue;
     v.safe = b->safe;
     direct_batch_enqueue(ctx, b->array(0), b->array(1), b->array(2), b->array(9),
                          ctx->order_index.as<std::uint32_t>(), v, b->array(6), b->array(7),

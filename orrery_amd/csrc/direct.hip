@@ -326,10 +326,11 @@ __global__ __launch_bounds__(64) void batch_prepare(const double* __restrict__ x
               packed_lo, centres, extents);
 }
 
-// Persistent: every wave takes work items from a queue (one counter, zero on
-// entry) until the list is exhausted, so long items started first (the list is
-// sorted by length) keep all CUs busy to the end. Each item writes its own
-// partial slot: the result does not depend on which wave ran it.
+// Persistent: the grid's W waves take the item list (sorted longest first) in
+// strides -- wave w runs items w, w + W, w + 2W, ... -- so every wave starts on a
+// long item and finishes on short ones, with no shared counter to contend on.
+// Each item writes its own partial slot: the result does not depend on which
+// wave ran it.
 constexpr int kItemUnroll = 8;
 
 // Eight waves per SIMD: without the hint the scheduler spends registers on
@@ -343,14 +344,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                                                        const uint4* __restrict__ item,
                                                        std::uint32_t items,
                                                        const float* __restrict__ eps2_of,
-                                                       unsigned* __restrict__ queue,
                                                        float* __restrict__ partial) {
     const unsigned lane = threadIdx.x & (kTile - 1);
-    for (;;) {
-        unsigned taken = 0;
-        if (lane == 0) taken = atomicAdd(queue, 1u);
-        const std::uint32_t next = __builtin_amdgcn_readfirstlane(taken);
-        if (next >= items) break;
+    const std::uint32_t waves = gridDim.x * (kBlock / kTile);
+    const std::uint32_t first = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kTile) + threadIdx.x / kTile);
+    for (std::uint32_t next = first; next < items; next += waves) {
         const uint4 it = item[next];
         const float eps2 = eps2_of[tile[it.x].z];
         const std::uint32_t k = it.x * kTile + lane;
@@ -379,8 +377,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
 
 // One thread per particle of the batch (sorted position k): fold its tile's
 // slots in split order in FP64, multiply by the system's G, scatter through
-// `order`. sys[s] = {first particle, count, first tile, splits}. Leaves the
-// work queue at zero for the next launch.
+// `order`. sys[s] = {first particle, count, first tile, splits}.
 __global__ __launch_bounds__(256) void batch_reduce(const float* __restrict__ partial,
                                                     const uint4* __restrict__ sys,
                                                     const uint4* __restrict__ tile,
@@ -388,12 +385,10 @@ __global__ __launch_bounds__(256) void batch_reduce(const float* __restrict__ pa
                                                     const double* __restrict__ G_of,
                                                     std::uint64_t n,
                                                     const std::uint32_t* __restrict__ order,
-                                                    unsigned* __restrict__ queue,
                                                     double* __restrict__ ax,
                                                     double* __restrict__ ay,
                                                     double* __restrict__ az) {
     const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k == 0) *queue = 0;
     if (k >= n) return;
     const std::uint32_t s = seg_of[k];
     const uint4 S = sys[s];
@@ -515,14 +510,14 @@ void direct_batch_enqueue(orr_ctx* ctx, const double* x, const double* y, const 
         auto kernel = b.safe ? (refine ? direct_items<true, true> : direct_items<true, false>)
                              : (refine ? direct_items<false, true> : direct_items<false, false>);
         hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, ctx->stream, packed, packed_lo,
-                           centres, extents, b.tile, b.item, b.items, b.eps2, b.queue, partials);
+                           centres, extents, b.tile, b.item, b.items, b.eps2, partials);
         ORR_LAUNCHED("direct_items");
     }
     {
         PhaseTimer t(ctx, kDirectReduce);
         const std::uint32_t blocks = static_cast<std::uint32_t>((b.count + 255) / 256);
         hipLaunchKernelGGL(batch_reduce, dim3(blocks), dim3(256), 0, ctx->stream, partials, b.sys,
-                           b.tile, b.seg_of, b.G, b.count, order, b.queue, ax, ay, az);
+                           b.tile, b.seg_of, b.G, b.count, order, ax, ay, az);
         ORR_LAUNCHED("batch_reduce");
     }
 }

@@ -178,7 +178,6 @@ struct DirectBatchView {
     const std::uint32_t* seg_of = nullptr;   // [count] system of each sorted position
     const float* eps2 = nullptr;             // [systems] softening squared
     const double* G = nullptr;               // [systems] gravitational constant
-    unsigned* queue = nullptr;               // work counter: zero on entry, left zero
     bool safe = false;                       // some system has zero softening
 };
 void direct_batch_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,

@@ -180,68 +180,111 @@ __device__ std::uint32_t prefix_end(const std::uint64_t* __restrict__ codes, std
     return bad;
 }
 
-// Geometric centre of every node of the chain (octree.cpp:41-47 from the root
-// centre of morton.hpp:93-95), its range, level and escape index.
-__global__ __launch_bounds__(kBlock) void tree_nodes(
-    const std::uint64_t* __restrict__ codes, const std::uint8_t* __restrict__ d,
-    const std::uint8_t* __restrict__ leaf_level, const std::uint32_t* __restrict__ chain,
-    const std::uint32_t* __restrict__ first_node, std::uint32_t n, std::uint32_t cap,
-    const std::uint32_t* __restrict__ seg_of, const std::uint32_t* __restrict__ seg_start,
-    const double* __restrict__ cubes, std::uint32_t* __restrict__ counters,
-    std::uint32_t* __restrict__ node_begin, std::uint32_t* __restrict__ node_count,
-    std::uint32_t* __restrict__ node_next, std::uint8_t* __restrict__ node_level,
-    double4* __restrict__ node_geo, std::uint32_t* __restrict__ leaf_of) {
+// Each chain's nodes: start particle and level (one thread per chain start; no
+// searches here, so no thread waits on another's).
+__global__ __launch_bounds__(kBlock) void tree_chains(
+    const std::uint8_t* __restrict__ d, const std::uint8_t* __restrict__ leaf_level,
+    const std::uint32_t* __restrict__ chain, const std::uint32_t* __restrict__ first_node,
+    std::uint32_t n, uint2* __restrict__ block_counts, std::uint32_t* __restrict__ node_begin,
+    std::uint8_t* __restrict__ node_level) {
+    __shared__ unsigned s_leaves[kBlock / 64], s_depth[kBlock / 64];
     const std::uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     const bool owns = k < n && chain[k] != 0;
     unsigned depth = 0;
     if (owns) {
-        const std::uint32_t M = counters[0];
         const unsigned dk = d[k], lleaf = leaf_level[k];
         const std::uint32_t base = first_node[k];
-        const std::uint64_t key = codes[k];
-        const std::uint32_t seg = seg_of != nullptr ? seg_of[k] : 0u;
-        const std::uint32_t seg_end = seg_start != nullptr ? seg_start[seg + 1] : n;
-        const double* cube = cubes + 4 * static_cast<std::size_t>(seg);
-        // From the leaf upwards: each cell's end bounds the next search from below.
-        std::uint32_t end = k + 1;
-        for (int l = static_cast<int>(lleaf); l >= static_cast<int>(dk); --l) {
-            end = prefix_end(codes, seg_end, end, key, static_cast<unsigned>(l));
-            const std::uint32_t node = base + (static_cast<unsigned>(l) - dk);
-            node_begin[node] = k;
-            node_count[node] = end - k;
-            node_level[node] = static_cast<std::uint8_t>(l);
-            node_next[node] = end >= n ? M : first_node[end];
-            if (static_cast<unsigned>(l) == lleaf) {
-                for (std::uint32_t j = k; j < end; ++j) leaf_of[j] = node;
-            }
-        }
-        // From the root downwards: one halving per level gives every chain
-        // node's centre with the same operations as a descent from the root.
-        double size = cube[3];
-        double c[3] = {cube[0] + size / 2, cube[1] + size / 2, cube[2] + size / 2};
-        for (unsigned l = 0; l <= lleaf; ++l) {
-            if (l > 0) {
-                const unsigned oct = static_cast<unsigned>((key >> (3u * (kLevels - l))) & 7u);
-                const double q = size / 4;
-                c[0] = c[0] + ((oct & 4u) != 0 ? q : -q);
-                c[1] = c[1] + ((oct & 2u) != 0 ? q : -q);
-                c[2] = c[2] + ((oct & 1u) != 0 ? q : -q);
-                size = size / 2;
-            }
-            if (l >= dk) node_geo[base + (l - dk)] = make_double4(c[0], c[1], c[2], size);
+        for (unsigned l = dk; l <= lleaf; ++l) {
+            node_begin[base + (l - dk)] = k;
+            node_level[base + (l - dk)] = static_cast<std::uint8_t>(l);
         }
         depth = lleaf;
     }
-    // One leaf per chain: count them and their deepest level once per wave.
+    // One leaf per chain: each block writes its leaf count and deepest level to
+    // its own slot (atomics on one address would serialise every wave).
     const unsigned long long leaves = __ballot(owns);
     unsigned deepest = depth;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) deepest = max(deepest, static_cast<unsigned>(__shfl_xor(static_cast<int>(deepest), off, 64)));
-    if ((threadIdx.x & 63) == 0 && leaves != 0) {
-        atomicAdd(&counters[1], static_cast<std::uint32_t>(__popcll(leaves)));
-        atomicMax(&counters[2], deepest);
+    if ((threadIdx.x & 63) == 0) {
+        s_leaves[threadIdx.x / 64] = static_cast<unsigned>(__popcll(leaves));
+        s_depth[threadIdx.x / 64] = deepest;
     }
-    (void)cap;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned total = 0, deep = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            total += s_leaves[w];
+            deep = max(deep, s_depth[w]);
+        }
+        block_counts[blockIdx.x] = make_uint2(total, deep);
+    }
+}
+
+// One block: leaf count and depth from the per-block slots.
+__global__ __launch_bounds__(kBlock) void tree_counts(const uint2* __restrict__ block_counts,
+                                                      std::uint32_t blocks,
+                                                      std::uint32_t* __restrict__ counters) {
+    __shared__ unsigned s_total[kBlock], s_deep[kBlock];
+    unsigned total = 0, deep = 0;
+    for (std::uint32_t b = threadIdx.x; b < blocks; b += kBlock) {
+        total += block_counts[b].x;
+        deep = max(deep, block_counts[b].y);
+    }
+    s_total[threadIdx.x] = total;
+    s_deep[threadIdx.x] = deep;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            s_total[threadIdx.x] += s_total[threadIdx.x + w];
+            s_deep[threadIdx.x] = max(s_deep[threadIdx.x], s_deep[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        counters[1] = s_total[0];
+        counters[2] = s_deep[0];
+    }
+}
+
+// One thread per node: its cell's end (a prefix search from its start, so a
+// node never waits on another), escape index, geometric centre (octree.cpp:41-47:
+// one halving per level from the root centre of morton.hpp:93-95) and, for a
+// leaf, the leaf id of its particles.
+__global__ __launch_bounds__(kBlock) void tree_node_fill(
+    const std::uint64_t* __restrict__ codes, const std::uint8_t* __restrict__ leaf_level,
+    const std::uint32_t* __restrict__ first_node, std::uint32_t n,
+    const std::uint32_t* __restrict__ seg_of, const std::uint32_t* __restrict__ seg_start,
+    const double* __restrict__ cubes, const std::uint32_t* __restrict__ counters,
+    const std::uint32_t* __restrict__ node_begin, const std::uint8_t* __restrict__ node_level,
+    std::uint32_t* __restrict__ node_count, std::uint32_t* __restrict__ node_next,
+    double4* __restrict__ node_geo, std::uint32_t* __restrict__ leaf_of) {
+    const std::uint32_t M = counters[0];
+    for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
+        const std::uint32_t k = node_begin[node];
+        const unsigned l = node_level[node];
+        const std::uint64_t key = codes[k];
+        const std::uint32_t seg = seg_of != nullptr ? seg_of[k] : 0u;
+        const std::uint32_t seg_end = seg_start != nullptr ? seg_start[seg + 1] : n;
+        const double* cube = cubes + 4 * static_cast<std::size_t>(seg);
+        const std::uint32_t end = prefix_end(codes, seg_end, k + 1, key, l);
+        node_count[node] = end - k;
+        node_next[node] = end >= n ? M : first_node[end];
+        double size = cube[3];
+        double c[3] = {cube[0] + size / 2, cube[1] + size / 2, cube[2] + size / 2};
+        for (unsigned s = 1; s <= l; ++s) {
+            const unsigned oct = static_cast<unsigned>((key >> (3u * (kLevels - s))) & 7u);
+            const double q = size / 4;
+            c[0] = c[0] + ((oct & 4u) != 0 ? q : -q);
+            c[1] = c[1] + ((oct & 2u) != 0 ? q : -q);
+            c[2] = c[2] + ((oct & 1u) != 0 ? q : -q);
+            size = size / 2;
+        }
+        node_geo[node] = make_double4(c[0], c[1], c[2], size);
+        if (l == leaf_level[k]) {
+            for (std::uint32_t j = k; j < end; ++j) leaf_of[j] = node;
+        }
+    }
 }
 
 // Sorted FP64 particles for the moments; FP32 walk data: position relative to
@@ -477,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     const std::uint32_t* __restrict__ seg_start, const std::uint32_t* __restrict__ first_node,
     std::uint32_t n, float eps2, double G,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    unsigned long long* __restrict__ stats) {
+    unsigned long long* __restrict__ stats, unsigned long long* __restrict__ wave_stats) {
     constexpr std::uint32_t kDone = 0xffffffffu;
     const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < n;
@@ -610,18 +653,21 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
         ay[dst] = G * (static_cast<double>(sy) - static_cast<double>(ky));
         az[dst] = G * (static_cast<double>(sz) - static_cast<double>(kz));
     }
-    // Counts per system: one atomic per wave when the wave holds one system.
+    // Counts: a wave inside one system writes its sums to its own slot, summed
+    // per system by walk_counts (atomics on one address would serialise every
+    // wave); a wave spanning two systems adds per lane.
     const std::uint32_t seg0 = __builtin_amdgcn_readfirstlane(seg);
+    unsigned long long* mine_slot =
+        wave_stats + 4 * static_cast<std::size_t>(blockIdx.x * (kBlock / 64) + threadIdx.x / 64);
     if (__ballot(valid && seg != seg0) == 0) {
         const unsigned long long wpp = wave_sum_u64(pp);
         const unsigned long long wpc = wave_sum_u64(pc);
         const unsigned long long wvisits = wave_sum_u64(visits);
         if ((threadIdx.x & 63) == 0) {
-            unsigned long long* st = stats + 4 * static_cast<std::size_t>(seg0);
-            atomicAdd(&st[0], wpp);
-            atomicAdd(&st[1], wpc);
-            atomicAdd(&st[2], wvisits);
-            atomicAdd(&st[3], steps);
+            mine_slot[0] = wpp;
+            mine_slot[1] = wpc;
+            mine_slot[2] = wvisits;
+            mine_slot[3] = steps;
         }
     } else {
         if (valid) {
@@ -630,8 +676,39 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
             atomicAdd(&st[1], static_cast<unsigned long long>(pc));
             atomicAdd(&st[2], static_cast<unsigned long long>(visits));
         }
-        if ((threadIdx.x & 63) == 0) atomicAdd(&stats[4 * static_cast<std::size_t>(seg0) + 3], steps);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&stats[4 * static_cast<std::size_t>(seg0) + 3], steps);
+            mine_slot[0] = mine_slot[1] = mine_slot[2] = mine_slot[3] = 0;
+        }
     }
+}
+
+// One block per system: the wave slots over its particle range, summed (a wave
+// spanning two systems left zeros and added its counts directly).
+__global__ __launch_bounds__(kBlock) void walk_counts(const unsigned long long* __restrict__ wave_stats,
+                                                      std::uint32_t waves,
+                                                      const std::uint32_t* __restrict__ seg_start,
+                                                      unsigned long long* __restrict__ stats) {
+    __shared__ unsigned long long s[4][kBlock];
+    const std::uint32_t seg = blockIdx.x;
+    std::uint32_t w0 = 0, w1 = waves;
+    if (seg_start != nullptr) {
+        w0 = seg_start[seg] / 64;
+        w1 = min(waves, (seg_start[seg + 1] + 63) / 64);
+    }
+    unsigned long long acc[4] = {0, 0, 0, 0};
+    for (std::uint32_t w = w0 + threadIdx.x; w < w1; w += kBlock) {
+        for (int c = 0; c < 4; ++c) acc[c] += wave_stats[4 * static_cast<std::size_t>(w) + c];
+    }
+    for (int c = 0; c < 4; ++c) s[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) {
+            for (int c = 0; c < 4; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) atomicAdd(&stats[4 * static_cast<std::size_t>(seg) + threadIdx.x], s[threadIdx.x][0]);
 }
 
 template<typename T> T* carve(char*& p, std::size_t count) {
@@ -664,6 +741,8 @@ struct Layout {
     // Counters.
     std::uint32_t* c32;
     unsigned long long* c64;
+    uint2* block_counts;               // per block of tree_chains: {leaves, depth}
+    unsigned long long* wave_stats;    // per walk wave: {pp, pc, visits, steps}
 };
 
 std::size_t node_bound(std::uint64_t n, std::uint64_t cap) {
@@ -681,7 +760,9 @@ Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole,
                                      4 * scan_scratch_words(n);
     const std::size_t per_quad = quadrupole ? 6 * sizeof(double) + sizeof(QuadF) : 0;
     const std::size_t per_node = nodes * (4 * 3 + 1 + 32 + 32 + sizeof(WalkNode) + per_quad);
-    const std::size_t counters = 16 + 8 * 4 * static_cast<std::size_t>(segments);
+    const std::size_t counters = 16 + 8 * 4 * static_cast<std::size_t>(segments) +
+                                 8 * ((n + kBlock - 1) / kBlock) +
+                                 32 * (kBlock / 64) * ((n + kBlock - 1) / kBlock) + 512;
     const std::size_t bytes = per_particle + per_node + counters + 21 * 256 + 64;
     if (allocate) ctx->tree_work.reserve(bytes);
     char* p = static_cast<char*>(ctx->tree_work.ptr);
@@ -708,6 +789,8 @@ Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole,
     }
     L.c32 = carve<std::uint32_t>(p, 4);
     L.c64 = carve<unsigned long long>(p, 4 * static_cast<std::size_t>(segments));
+    L.block_counts = carve<uint2>(p, (n + kBlock - 1) / kBlock);
+    L.wave_stats = carve<unsigned long long>(p, 4 * (kBlock / 64) * ((n + kBlock - 1) / kBlock));
     return L;
 }
 
@@ -758,10 +841,17 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
                            L.chain, L.leaf_level);
         ORR_LAUNCHED("tree_chain");
         exclusive_scan_u32(ctx, L.chain, L.first_node, n, L.scan_scratch, &L.c32[0]);
-        hipLaunchKernelGGL(tree_nodes, dim3(blocks), dim3(kBlock), 0, ctx->stream, codes, L.d,
-                           L.leaf_level, L.chain, L.first_node, n32, cap, seg_of, seg_start, cube,
-                           L.c32, L.begin, L.count, L.next, L.level, L.geo, L.leaf_of);
-        ORR_LAUNCHED("tree_nodes");
+        hipLaunchKernelGGL(tree_chains, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.d,
+                           L.leaf_level, L.chain, L.first_node, n32, L.block_counts, L.begin,
+                           L.level);
+        ORR_LAUNCHED("tree_chains");
+        hipLaunchKernelGGL(tree_counts, dim3(1), dim3(kBlock), 0, ctx->stream, L.block_counts,
+                           blocks, L.c32);
+        ORR_LAUNCHED("tree_counts");
+        hipLaunchKernelGGL(tree_node_fill, dim3(node_blocks), dim3(kBlock), 0, ctx->stream, codes,
+                           L.leaf_level, L.first_node, n32, seg_of, seg_start, cube, L.c32,
+                           L.begin, L.level, L.count, L.next, L.geo, L.leaf_of);
+        ORR_LAUNCHED("tree_node_fill");
     }
     {
         PhaseTimer t(ctx, kGather);
@@ -794,8 +884,11 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
         hipLaunchKernelGGL(walk, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.walk, L.quadf, L.geo,
                            L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, seg_of, seg_start,
                            L.first_node, n32, eps2, a.gravitational_constant, a.ax, a.ay, a.az,
-                           counts64);
+                           counts64, L.wave_stats);
         ORR_LAUNCHED("tree_walk");
+        hipLaunchKernelGGL(walk_counts, dim3(segs.count), dim3(kBlock), 0, ctx->stream,
+                           L.wave_stats, blocks * (kBlock / 64), seg_start, counts64);
+        ORR_LAUNCHED("walk_counts");
     }
     if (stats != nullptr) {
         std::uint32_t c32[4];
