@@ -453,6 +453,8 @@ def main() -> int:
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-direct", action="store_true", help="skip the headline direct leg")
+    ap.add_argument("--walk-targets", type=int, default=0,
+                    help="targets per lane of the Barnes-Hut walk (1 or 2; 0: the engine's default)")
     ap.add_argument("--probes-only", action="store_true",
                     help="run only the machine-ceiling probes (copy probe: counter calibration)")
     args = ap.parse_args()
@@ -476,6 +478,8 @@ def main() -> int:
 
     cpu = None if args.no_cpu_baseline else CpuLeg()
     ctx = E.Context(0)
+    if args.walk_targets:
+        ctx.set_tree_config(targets_per_lane=args.walk_targets)
     dev = E.describe(0)
     log(f"[bench] device {dev['name']} {dev['arch']} CUs={dev['compute_units']}")
     n, eps, dt = args.n, 0.005, 5e-4

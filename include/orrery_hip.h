@@ -160,6 +160,18 @@ typedef struct {
 } orr_tree_stats;
 ORR_API int orr_tree_eval(orr_ctx* ctx, const orr_tree_args* args, orr_tree_stats* stats);
 
+/* Walk configuration (no reference counterpart; cuda_tree_solver.cpp fixes its
+ * traversal): targets_per_lane 1 or 2 -- how many targets each lane of a
+ * wave walks together -- or -1 to choose by size (2 for one system of 2^20 or
+ * more particles, else 1; the default). Results are bit-identical either way;
+ * only the speed differs. Zero keeps the current value. */
+typedef struct {
+    int32_t targets_per_lane;
+    int32_t reserved[3];
+} orr_tree_config;
+ORR_API int orr_ctx_set_tree_config(orr_ctx* ctx, const orr_tree_config* config);
+ORR_API int orr_ctx_get_tree_config(orr_ctx* ctx, orr_tree_config* config);
+
 /* Morton ordering on the device: MortonOrdering::build (solvers/morton.cpp:180-197).
  * Writes the bounding cube {origin x, y, z, size} to host cube_out and the
  * ascending (code, index) keys to device arrays of `count` entries. */

@@ -124,6 +124,10 @@ class OrrServerStats(ctypes.Structure):
                 ("queued", ctypes.c_uint32), ("busy_s", ctypes.c_double)]
 
 
+class OrrTreeConfig(ctypes.Structure):
+    _fields_ = [("targets_per_lane", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+
 # (name, argtypes) of every entry point the header declares.
 SIGNATURES = {
     "orr_device_count": [ctypes.POINTER(ctypes.c_int32)],
@@ -140,6 +144,8 @@ SIGNATURES = {
     "orr_ctx_set_direct_config": [_vp, ctypes.POINTER(OrrDirectConfig)],
     "orr_ctx_get_direct_config": [_vp, ctypes.POINTER(OrrDirectConfig)],
     "orr_tree_eval": [_vp, ctypes.POINTER(OrrTreeArgs), ctypes.POINTER(OrrTreeStats)],
+    "orr_ctx_set_tree_config": [_vp, ctypes.POINTER(OrrTreeConfig)],
+    "orr_ctx_get_tree_config": [_vp, ctypes.POINTER(OrrTreeConfig)],
     "orr_morton_order": [_vp, _vp, _vp, _vp, ctypes.c_uint64, _vp, _vp, _d],
     "orr_tree_nodes": [_vp] + [_vp] * 8,
     "orr_probe": [_vp, ctypes.c_int32, ctypes.c_uint64, _d, _d],

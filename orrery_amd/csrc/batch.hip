@@ -536,6 +536,9 @@ int orr_batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t cou
         b->leaf_capacity = b->tune_leaf ? 16u : config->leaf_capacity;
         b->dctx = orr::make_context(ctx->device);
         b->tctx = orr::make_context(ctx->device);
+        // The owner's kernel settings carry over to the batch's private contexts.
+        b->tctx->walk_targets_per_lane = ctx->walk_targets_per_lane;
+        b->dctx->direct_refine = ctx->direct_refine;
         ORR_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
         ORR_HIP(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
         b->state.reserve(sizeof(double) * 10 * std::max<std::uint64_t>(total, 1));

@@ -274,6 +274,24 @@ int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* c) {
     });
 }
 
+int orr_ctx_set_tree_config(orr_ctx* ctx, const orr_tree_config* config) {
+    return guarded([&] {
+        require(ctx != nullptr && config != nullptr, "null argument");
+        const int r = config->targets_per_lane;
+        require(r >= -1 && r <= 2, "targets_per_lane must be 1, 2, 0 (keep) or -1 (by size)");
+        if (r == -1) ctx->walk_targets_per_lane = 0;
+        if (r > 0) ctx->walk_targets_per_lane = r;
+    });
+}
+
+int orr_ctx_get_tree_config(orr_ctx* ctx, orr_tree_config* config) {
+    return guarded([&] {
+        require(ctx != nullptr && config != nullptr, "null argument");
+        *config = orr_tree_config{};
+        config->targets_per_lane = ctx->walk_targets_per_lane;
+    });
+}
+
 int orr_ctx_get_direct_config(orr_ctx* ctx, orr_direct_config* c) {
     return guarded([&] {
         require(ctx != nullptr && c != nullptr, "null argument");

@@ -110,6 +110,8 @@ struct orr_ctx {
     int direct_targets_per_lane = 1;
     int direct_refine = 0;
     std::uint32_t direct_block_target = 8192;
+    // Barnes-Hut walk configuration (orr_ctx_set_tree_config); 0 chooses by size.
+    int walk_targets_per_lane = 0;
 
     // Host mirror of the last tree summary and the shape of its device layout.
     orr_tree_stats last_tree{};

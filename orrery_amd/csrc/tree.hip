@@ -510,7 +510,115 @@ __device__ __forceinline__ void kahan(float p, float& s, float& c) {
     s = t;
 }
 
+// One target's walk state: FP32 position relative to the root centre and
+// offset from its leaf centre, its leaf, the next node it needs, Kahan sums.
+struct WalkTarget {
+    float4 xi, oi;
+    std::uint32_t own, mine;
+    float sx, sy, sz, kx, ky, kz, px, py, pz;
+    // Per target, pp < N, pc <= M and visits <= M: 32 bits suffice.
+    std::uint32_t pp, pc, visits;
+};
+
+// The target's step at node `cur` if it is the node it needs (tree_walk.cpp:20-83):
+// accept (monopole, optionally quadrupole, then skip the subtree), mark a leaf
+// for particle-particle terms (returns true), or open (descend to cur + 1).
 template<bool kSafe, bool kQuad>
+__device__ __forceinline__ bool visit(WalkTarget& t, std::uint32_t cur, const WalkNode& nd,
+                                      const QuadF& Q, std::uint32_t next, float eps2) {
+    if (t.mine != cur) return false;
+    ++t.visits;
+    const float dx = nd.a.x - t.xi.x, dy = nd.a.y - t.xi.y, dz = nd.a.z - t.xi.z;
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    float acc;
+    std::memcpy(&acc, &nd.b.x, sizeof(acc));
+    if (r2 >= acc) {
+        // tree_walk.hpp:129-139 monopole.
+        float r2e = r2 + eps2;
+        if constexpr (kSafe) r2e = r2e > 0.f ? r2e : 1.f;
+        const float inv = __builtin_amdgcn_rsqf(r2e);
+        if constexpr (!kQuad) {
+            const float f = (nd.a.w * inv) * (inv * inv);
+            t.px = __builtin_fmaf(f, dx, t.px);
+            t.py = __builtin_fmaf(f, dy, t.py);
+            t.pz = __builtin_fmaf(f, dz, t.pz);
+        } else {
+            // Monopole plus tree_walk.hpp:156-181 in one update:
+            // d (m inv^3 + 2.5 (d.Q d) inv^7) - (Q d) inv^5.
+            const float i2 = inv * inv;
+            const float i3 = i2 * inv;
+            const float i5 = i3 * i2;
+            const float i7 = i5 * i2;
+            const float qx = __builtin_fmaf(Q.a.z, dz, __builtin_fmaf(Q.a.y, dy, Q.a.x * dx));
+            const float qy = __builtin_fmaf(Q.b.x, dz, __builtin_fmaf(Q.a.w, dy, Q.a.y * dx));
+            const float qz = __builtin_fmaf(Q.b.y, dz, __builtin_fmaf(Q.b.x, dy, Q.a.z * dx));
+            const float quadratic = __builtin_fmaf(dz, qz, __builtin_fmaf(dy, qy, dx * qx));
+            const float g = __builtin_fmaf(2.5f * quadratic, i7, nd.a.w * i3);
+            t.px = __builtin_fmaf(dx, g, __builtin_fmaf(qx, -i5, t.px));
+            t.py = __builtin_fmaf(dy, g, __builtin_fmaf(qy, -i5, t.py));
+            t.pz = __builtin_fmaf(dz, g, __builtin_fmaf(qz, -i5, t.pz));
+        }
+        ++t.pc;
+        t.mine = next;
+        return false;
+    }
+    if (nd.b.w > 0) {
+        t.pp += nd.b.w - (t.own == cur ? 1u : 0u);
+        t.mine = next;
+        return true;
+    }
+    t.mine = cur + 1;
+    return false;
+}
+
+// Particle-particle terms of leaf [b, e) for one target, then the partial is
+// Kahan-added. The target is re-expressed against the leaf's centre cb in FP64.
+template<bool kSafe>
+__device__ __forceinline__ void leaf_pairs(WalkTarget& t, const double4& cb,
+                                           const double4* __restrict__ node_geo,
+                                           const float4* __restrict__ leaf_rel, std::uint32_t b,
+                                           std::uint32_t e, float eps2) {
+    // The target's own leaf centre, read here rather than held through the walk
+    // (a leaf visit is about one iteration in a hundred).
+    const double4 ca = node_geo[t.own];
+    const float tx = t.oi.x - static_cast<float>(cb.x - ca.x);
+    const float ty = t.oi.y - static_cast<float>(cb.y - ca.y);
+    const float tz = t.oi.z - static_cast<float>(cb.z - ca.z);
+    auto pair = [&](const float4 q) {
+        const float dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+        float r2 = __builtin_fmaf(dx, dx, eps2);
+        r2 = __builtin_fmaf(dy, dy, r2);
+        r2 = __builtin_fmaf(dz, dz, r2);
+        if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        const float f = (q.w * inv) * (inv * inv);
+        t.px = __builtin_fmaf(f, dx, t.px);
+        t.py = __builtin_fmaf(f, dy, t.py);
+        t.pz = __builtin_fmaf(f, dz, t.pz);
+    };
+    // Four sources per round so their scalar loads share one wait.
+    std::uint32_t j = b;
+    for (; j + 4 <= e; j += 4) {
+        const float4 q0 = leaf_rel[j], q1 = leaf_rel[j + 1];
+        const float4 q2 = leaf_rel[j + 2], q3 = leaf_rel[j + 3];
+        pair(q0);
+        pair(q1);
+        pair(q2);
+        pair(q3);
+    }
+    for (; j < e; ++j) pair(leaf_rel[j]);
+    kahan(t.px, t.sx, t.kx);
+    kahan(t.py, t.sy, t.ky);
+    kahan(t.pz, t.sz, t.kz);
+    t.px = t.py = t.pz = 0.f;
+}
+
+// Wave-coherent walk. Each lane owns kR targets (wave w owns the 64 kR sorted
+// particles from 64 kR w, target r of lane l being 64 r + l); the wave visits
+// the smallest node any of its targets needs, loads it once, and every target
+// that needs it takes its step. A target therefore sees exactly the node
+// sequence and arithmetic of a walk of its own, whatever the other targets are.
+template<int kR, bool kSafe, bool kQuad>
 __global__ __launch_bounds__(kBlock) void tree_walk(
     const WalkNode* __restrict__ nodes, const QuadF* __restrict__ quads,
     const double4* __restrict__ node_geo,
@@ -522,28 +630,37 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     unsigned long long* __restrict__ stats, unsigned long long* __restrict__ wave_stats) {
     constexpr std::uint32_t kDone = 0xffffffffu;
-    const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = i < n;
+    const std::uint32_t wave = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    const std::uint32_t lane = threadIdx.x & 63;
     const std::uint32_t M = counters[0];
-    const std::uint32_t ic = valid ? i : n - 1;
-    const float4 xi = root_rel[ic];
-    const float4 oi = leaf_rel[ic];
-    const std::uint32_t own = leaf_of[ic];
-    const double4 ca = node_geo[own];
-    // This lane's system starts at its root; the walk records' escape indices
-    // end the walk at the system's last subtree.
-    const std::uint32_t seg = seg_of != nullptr ? seg_of[ic] : 0u;
-    const std::uint32_t root = seg_start != nullptr ? first_node[seg_start[seg]] : 0u;
-    std::uint32_t mine = valid ? root : kDone;
-    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
-    float px = 0.f, py = 0.f, pz = 0.f;
-    // Per lane, pp < N, pc <= M and visits <= M: 32 bits suffice.
-    std::uint32_t pp = 0, pc = 0, visits = 0;
+    WalkTarget t[kR];
+    std::uint32_t seg[kR];
+    bool valid[kR];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+        const std::uint32_t i = wave * (64u * kR) + 64u * r + lane;
+        valid[r] = i < n;
+        const std::uint32_t ic = valid[r] ? i : n - 1;
+        t[r].xi = root_rel[ic];
+        t[r].oi = leaf_rel[ic];
+        t[r].own = leaf_of[ic];
+        // The target's system starts at its root; the walk records' escape
+        // indices end the walk at the system's last subtree.
+        seg[r] = seg_of != nullptr ? seg_of[ic] : 0u;
+        const std::uint32_t root = seg_start != nullptr ? first_node[seg_start[seg[r]]] : 0u;
+        t[r].mine = valid[r] ? root : kDone;
+        t[r].sx = t[r].sy = t[r].sz = t[r].kx = t[r].ky = t[r].kz = 0.f;
+        t[r].px = t[r].py = t[r].pz = 0.f;
+        t[r].pp = t[r].pc = t[r].visits = 0;
+    }
     unsigned long long steps = 0;
-    // Each iteration advances at least one lane, so a walk takes at most
-    // 64 M iterations; the cap only turns a corrupted tree into a wrong result.
-    const unsigned long long cap = 64ull * M + 64;
-    std::uint32_t cur = __builtin_amdgcn_readfirstlane(wave_min_u32(mine));
+    // Each iteration advances at least one target, so a walk takes at most
+    // 64 kR M iterations; the cap only turns a corrupted tree into a wrong result.
+    const unsigned long long cap = 64ull * kR * M + 64;
+    std::uint32_t least = t[0].mine;
+#pragma unroll
+    for (int r = 1; r < kR; ++r) least = min(least, t[r].mine);
+    std::uint32_t cur = __builtin_amdgcn_readfirstlane(wave_min_u32(least));
     while (cur != kDone && steps < cap) {
         ++steps;
         const WalkNode nd = nodes[cur];
@@ -561,123 +678,76 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
         // An escape index always lies past its node (or is kDone); the guard only
         // makes a corrupted tree end the walk instead of looping.
         const std::uint32_t next = nd.b.y > cur ? nd.b.y : kDone;
-        bool leaf_pp = false;
-        if (mine == cur) {
-            ++visits;
-            const float dx = nd.a.x - xi.x, dy = nd.a.y - xi.y, dz = nd.a.z - xi.z;
-            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-            float acc;
-            std::memcpy(&acc, &nd.b.x, sizeof(acc));
-            if (r2 >= acc) {
-                // tree_walk.hpp:129-139 monopole.
-                float r2e = r2 + eps2;
-                if constexpr (kSafe) r2e = r2e > 0.f ? r2e : 1.f;
-                const float inv = __builtin_amdgcn_rsqf(r2e);
-                if constexpr (!kQuad) {
-                    const float f = (nd.a.w * inv) * (inv * inv);
-                    px = __builtin_fmaf(f, dx, px);
-                    py = __builtin_fmaf(f, dy, py);
-                    pz = __builtin_fmaf(f, dz, pz);
-                } else {
-                    // Monopole plus tree_walk.hpp:156-181 in one update:
-                    // d (m inv^3 + 2.5 (d.Q d) inv^7) - (Q d) inv^5.
-                    const float i2 = inv * inv;
-                    const float i3 = i2 * inv;
-                    const float i5 = i3 * i2;
-                    const float i7 = i5 * i2;
-                    const float qx = __builtin_fmaf(Q.a.z, dz, __builtin_fmaf(Q.a.y, dy, Q.a.x * dx));
-                    const float qy = __builtin_fmaf(Q.b.x, dz, __builtin_fmaf(Q.a.w, dy, Q.a.y * dx));
-                    const float qz = __builtin_fmaf(Q.b.y, dz, __builtin_fmaf(Q.b.x, dy, Q.a.z * dx));
-                    const float quadratic = __builtin_fmaf(dz, qz, __builtin_fmaf(dy, qy, dx * qx));
-                    const float g = __builtin_fmaf(2.5f * quadratic, i7, nd.a.w * i3);
-                    px = __builtin_fmaf(dx, g, __builtin_fmaf(qx, -i5, px));
-                    py = __builtin_fmaf(dy, g, __builtin_fmaf(qy, -i5, py));
-                    pz = __builtin_fmaf(dz, g, __builtin_fmaf(qz, -i5, pz));
-                }
-                ++pc;
-                mine = next;
-            } else if (nd.b.w > 0) {
-                leaf_pp = true;
-                pp += nd.b.w - (own == cur ? 1u : 0u);
-                mine = next;
-            } else {
-                mine = cur + 1;
-            }
+        bool leaf[kR];
+        bool any_leaf = false;
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            leaf[r] = visit<kSafe, kQuad>(t[r], cur, nd, Q, next, eps2);
+            any_leaf = any_leaf || leaf[r];
         }
-        if (__ballot(leaf_pp) != 0) {
+        if (__ballot(any_leaf) != 0) {
             const double4 cb = node_geo[cur];
-            const float tx = oi.x - static_cast<float>(cb.x - ca.x);
-            const float ty = oi.y - static_cast<float>(cb.y - ca.y);
-            const float tz = oi.z - static_cast<float>(cb.z - ca.z);
             const std::uint32_t b = nd.b.z, e = nd.b.z + nd.b.w;
-            if (leaf_pp) {
-                auto pair = [&](const float4 q) {
-                    const float dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
-                    float r2 = __builtin_fmaf(dx, dx, eps2);
-                    r2 = __builtin_fmaf(dy, dy, r2);
-                    r2 = __builtin_fmaf(dz, dz, r2);
-                    if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    const float f = (q.w * inv) * (inv * inv);
-                    px = __builtin_fmaf(f, dx, px);
-                    py = __builtin_fmaf(f, dy, py);
-                    pz = __builtin_fmaf(f, dz, pz);
-                };
-                // Four sources per round so their scalar loads share one wait.
-                std::uint32_t j = b;
-                for (; j + 4 <= e; j += 4) {
-                    const float4 q0 = leaf_rel[j], q1 = leaf_rel[j + 1];
-                    const float4 q2 = leaf_rel[j + 2], q3 = leaf_rel[j + 3];
-                    pair(q0);
-                    pair(q1);
-                    pair(q2);
-                    pair(q3);
-                }
-                for (; j < e; ++j) pair(leaf_rel[j]);
-                kahan(px, sx, kx);
-                kahan(py, sy, ky);
-                kahan(pz, sz, kz);
-                px = py = pz = 0.f;
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                if (leaf[r]) leaf_pairs<kSafe>(t[r], cb, node_geo, leaf_rel, b, e, eps2);
             }
         }
         const std::uint32_t step = cur + 1;
-        cur = (__ballot(mine == step) != 0) ? step
-                                            : __builtin_amdgcn_readfirstlane(wave_min_u32(mine));
+        bool descends = false;
+        least = t[0].mine;
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            descends = descends || t[r].mine == step;
+            least = min(least, t[r].mine);
+        }
+        cur = (__ballot(descends) != 0) ? step : __builtin_amdgcn_readfirstlane(wave_min_u32(least));
     }
-    kahan(px, sx, kx);
-    kahan(py, sy, ky);
-    kahan(pz, sz, kz);
-    if (valid) {
-        const std::uint32_t dst = index[i];
-        ax[dst] = G * (static_cast<double>(sx) - static_cast<double>(kx));
-        ay[dst] = G * (static_cast<double>(sy) - static_cast<double>(ky));
-        az[dst] = G * (static_cast<double>(sz) - static_cast<double>(kz));
+    std::uint32_t pp = 0, pc = 0, visits = 0;
+    bool mixed = false;
+    const std::uint32_t seg0 = __builtin_amdgcn_readfirstlane(seg[0]);
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+        kahan(t[r].px, t[r].sx, t[r].kx);
+        kahan(t[r].py, t[r].sy, t[r].ky);
+        kahan(t[r].pz, t[r].sz, t[r].kz);
+        if (valid[r]) {
+            const std::uint32_t dst = index[wave * (64u * kR) + 64u * r + lane];
+            ax[dst] = G * (static_cast<double>(t[r].sx) - static_cast<double>(t[r].kx));
+            ay[dst] = G * (static_cast<double>(t[r].sy) - static_cast<double>(t[r].ky));
+            az[dst] = G * (static_cast<double>(t[r].sz) - static_cast<double>(t[r].kz));
+            pp += t[r].pp;
+            pc += t[r].pc;
+            visits += t[r].visits;
+        }
+        mixed = mixed || (valid[r] && seg[r] != seg0);
     }
     // Counts: a wave inside one system writes its sums to its own slot, summed
     // per system by walk_counts (atomics on one address would serialise every
-    // wave); a wave spanning two systems adds per lane.
-    const std::uint32_t seg0 = __builtin_amdgcn_readfirstlane(seg);
-    unsigned long long* mine_slot =
-        wave_stats + 4 * static_cast<std::size_t>(blockIdx.x * (kBlock / 64) + threadIdx.x / 64);
-    if (__ballot(valid && seg != seg0) == 0) {
+    // wave); a wave spanning two systems adds per target.
+    unsigned long long* mine_slot = wave_stats + 4 * static_cast<std::size_t>(wave);
+    if (__ballot(mixed) == 0) {
         const unsigned long long wpp = wave_sum_u64(pp);
         const unsigned long long wpc = wave_sum_u64(pc);
         const unsigned long long wvisits = wave_sum_u64(visits);
-        if ((threadIdx.x & 63) == 0) {
+        if (lane == 0) {
             mine_slot[0] = wpp;
             mine_slot[1] = wpc;
             mine_slot[2] = wvisits;
-            mine_slot[3] = steps;
+            mine_slot[3] = steps * kR;  // lane-target slots: 64 per count
         }
     } else {
-        if (valid) {
-            unsigned long long* st = stats + 4 * static_cast<std::size_t>(seg);
-            atomicAdd(&st[0], static_cast<unsigned long long>(pp));
-            atomicAdd(&st[1], static_cast<unsigned long long>(pc));
-            atomicAdd(&st[2], static_cast<unsigned long long>(visits));
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            if (valid[r]) {
+                unsigned long long* st = stats + 4 * static_cast<std::size_t>(seg[r]);
+                atomicAdd(&st[0], static_cast<unsigned long long>(t[r].pp));
+                atomicAdd(&st[1], static_cast<unsigned long long>(t[r].pc));
+                atomicAdd(&st[2], static_cast<unsigned long long>(t[r].visits));
+            }
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&stats[4 * static_cast<std::size_t>(seg0) + 3], steps);
+        if (lane == 0) {
+            atomicAdd(&stats[4 * static_cast<std::size_t>(seg0) + 3], steps * kR);
             mine_slot[0] = mine_slot[1] = mine_slot[2] = mine_slot[3] = 0;
         }
     }
@@ -686,15 +756,15 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
 // One block per system: the wave slots over its particle range, summed (a wave
 // spanning two systems left zeros and added its counts directly).
 __global__ __launch_bounds__(kBlock) void walk_counts(const unsigned long long* __restrict__ wave_stats,
-                                                      std::uint32_t waves,
+                                                      std::uint32_t waves, std::uint32_t per_wave,
                                                       const std::uint32_t* __restrict__ seg_start,
                                                       unsigned long long* __restrict__ stats) {
     __shared__ unsigned long long s[4][kBlock];
     const std::uint32_t seg = blockIdx.x;
     std::uint32_t w0 = 0, w1 = waves;
     if (seg_start != nullptr) {
-        w0 = seg_start[seg] / 64;
-        w1 = min(waves, (seg_start[seg + 1] + 63) / 64);
+        w0 = seg_start[seg] / per_wave;
+        w1 = min(waves, (seg_start[seg + 1] + per_wave - 1) / per_wave);
     }
     unsigned long long acc[4] = {0, 0, 0, 0};
     for (std::uint32_t w = w0 + threadIdx.x; w < w1; w += kBlock) {
@@ -879,15 +949,28 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
         const float eps2 = static_cast<float>(a.softening_squared);
         const bool safe = !(a.softening_squared > 0);
         const bool quad = a.quadrupole != 0;
-        auto walk = safe ? (quad ? tree_walk<true, true> : tree_walk<true, false>)
-                         : (quad ? tree_walk<false, true> : tree_walk<false, false>);
-        hipLaunchKernelGGL(walk, dim3(blocks), dim3(kBlock), 0, ctx->stream, L.walk, L.quadf, L.geo,
-                           L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, seg_of, seg_start,
-                           L.first_node, n32, eps2, a.gravitational_constant, a.ax, a.ay, a.az,
-                           counts64, L.wave_stats);
+        // Two targets per lane pay off for one large system (measured: 4.5 % at
+        // N = 2^21); zero in the context means choose by that rule.
+        const int R = ctx->walk_targets_per_lane != 0 ? ctx->walk_targets_per_lane
+                      : (segs.count == 1 && n >= (std::uint64_t{1} << 20) ? 2 : 1);
+        using Walk = decltype(&tree_walk<1, true, true>);
+        Walk walk;
+        if (R == 2) {
+            walk = safe ? (quad ? tree_walk<2, true, true> : tree_walk<2, true, false>)
+                        : (quad ? tree_walk<2, false, true> : tree_walk<2, false, false>);
+        } else {
+            walk = safe ? (quad ? tree_walk<1, true, true> : tree_walk<1, true, false>)
+                        : (quad ? tree_walk<1, false, true> : tree_walk<1, false, false>);
+        }
+        const std::uint32_t per_block = kBlock * static_cast<std::uint32_t>(R);
+        const std::uint32_t walk_blocks = (n32 + per_block - 1) / per_block;
+        hipLaunchKernelGGL(walk, dim3(walk_blocks), dim3(kBlock), 0, ctx->stream, L.walk, L.quadf,
+                           L.geo, L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, seg_of,
+                           seg_start, L.first_node, n32, eps2, a.gravitational_constant, a.ax,
+                           a.ay, a.az, counts64, L.wave_stats);
         ORR_LAUNCHED("tree_walk");
         hipLaunchKernelGGL(walk_counts, dim3(segs.count), dim3(kBlock), 0, ctx->stream,
-                           L.wave_stats, blocks * (kBlock / 64), seg_start, counts64);
+                           L.wave_stats, walk_blocks * (kBlock / 64), 64u * R, seg_start, counts64);
         ORR_LAUNCHED("walk_counts");
     }
     if (stats != nullptr) {

@@ -200,3 +200,26 @@ def test_device_resident_tree_stepping(ctx):
     oracle.steps(ref, acc, f, dt, 5)
     assert np.max(np.abs(st["x"] - ref.x)) < 1e-8
     sysd.close()
+
+
+@pytest.mark.parametrize("quadrupole", [False, True])
+def test_two_targets_per_lane_is_bitwise(ctx, quadrupole):
+    # A wave walking 128 targets (two per lane) visits each target's nodes in the
+    # same DFS order with the same arithmetic as a wave of 64.
+    p = oracle.disc_collision(30000, 4)
+    results = []
+    try:
+        for r in (1, 2):
+            ctx.set_tree_config(targets_per_lane=r)
+            s = E.HipTreeSolver(E.TreeParameters(0.5, 16, quadrupole=quadrupole), E.Softening(0.01),
+                                context=ctx)
+            acc = (np.zeros(p.n), np.zeros(p.n), np.zeros(p.n))
+            s.evaluate((p.x, p.y, p.z), p.m, acc)
+            c = s.interaction_count()
+            results.append((acc, c.particle_particle, c.particle_cell, s.node_visits()))
+    finally:
+        ctx.set_tree_config(targets_per_lane=-1)
+    (a1, pp1, pc1, v1), (a2, pp2, pc2, v2) = results
+    for k in range(3):
+        assert np.array_equal(a1[k], a2[k])
+    assert (pp1, pc1, v1) == (pp2, pc2, v2)
