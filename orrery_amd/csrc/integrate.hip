@@ -7,7 +7,8 @@
 // multiply-then-add updates.
 // The diagnostics follow src/core/diagnostics.cpp:85-200: kinetic energy,
 // momentum and angular momentum as Neumaier-compensated sums, potential
-// energy as rows i of plain sums over j > i, the rows then compensated.
+// energy as rows i of plain sums over j > i (in column chunks folded in order),
+// the rows then compensated.
 // Device reductions are deterministic: fixed chunks, each summed in index
 // order by one lane, then the chunk results summed in order by one lane.
 #include <hip/hip_runtime.h>
@@ -80,6 +81,41 @@ struct Compensated {
 
 constexpr int kTerms = 8;  // ke, px, py, pz, lx, ly, lz, pe
 
+// Columns of the potential's rows are summed in at most kColumnChunks chunks
+// of at least kColumnMin columns, so a system of a few thousand particles
+// still spreads over many waves.
+constexpr std::uint64_t kColumnMin = 2048;
+constexpr std::uint64_t kColumnChunks = 16;
+
+__host__ __device__ inline std::uint64_t column_width(std::uint64_t n) {
+    const std::uint64_t even = (n + kColumnChunks - 1) / kColumnChunks;
+    return even > kColumnMin ? even : kColumnMin;
+}
+
+// Row i of the potential over column chunk blockIdx.y: plain FP64 sum over
+// j > i of m_j / sqrt(r^2 + e^2) in index order (diagnostics.cpp:129-158 sums
+// the whole row plainly; here the chunks' sums are folded in chunk order).
+__global__ __launch_bounds__(256) void potential_rows(std::uint64_t n, std::uint64_t width,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z,
+                                                      const double* __restrict__ m, double e2,
+                                                      double* __restrict__ rows) {
+    const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const std::uint64_t c0 = static_cast<std::uint64_t>(blockIdx.y) * width;
+    if (i >= n) return;
+    double inter = 0.0;
+    const std::uint64_t j1 = c0 + width < n ? c0 + width : n;
+    if (j1 > i + 1) {
+        const double xi = x[i], yi = y[i], zi = z[i];
+        for (std::uint64_t j = (i + 1 > c0 ? i + 1 : c0); j < j1; ++j) {
+            const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
+            inter += m[j] * (1.0 / sqrt(((dx * dx) + (dy * dy) + (dz * dz)) + e2));
+        }
+    }
+    rows[static_cast<std::uint64_t>(blockIdx.y) * n + i] = inter;
+}
+
 // Per-particle terms, term-major [kTerms][n].
 __global__ __launch_bounds__(256) void diagnostic_terms(std::uint64_t n,
                                                         const double* __restrict__ x,
@@ -88,8 +124,10 @@ __global__ __launch_bounds__(256) void diagnostic_terms(std::uint64_t n,
                                                         const double* __restrict__ vx,
                                                         const double* __restrict__ vy,
                                                         const double* __restrict__ vz,
-                                                        const double* __restrict__ m, double e2,
-                                                        double G, double* __restrict__ terms) {
+                                                        const double* __restrict__ m,
+                                                        const double* __restrict__ rows,
+                                                        std::uint32_t chunks, double G,
+                                                        double* __restrict__ terms) {
     const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double mi = m[i];
@@ -102,12 +140,8 @@ __global__ __launch_bounds__(256) void diagnostic_terms(std::uint64_t n,
     terms[4 * n + i] = ((yi * uz) - (zi * uy)) * mi;
     terms[5 * n + i] = ((zi * ux) - (xi * uz)) * mi;
     terms[6 * n + i] = ((xi * uy) - (yi * ux)) * mi;
-    // Row i of the potential: plain FP64 sum over j > i in index order.
-    double inter = 0.0;
-    for (std::uint64_t j = i + 1; j < n; ++j) {
-        const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
-        inter += m[j] * (1.0 / sqrt(((dx * dx) + (dy * dy) + (dz * dz)) + e2));
-    }
+    double inter = rows[i];
+    for (std::uint32_t c = 1; c < chunks; ++c) inter += rows[static_cast<std::uint64_t>(c) * n + i];
     terms[7 * n + i] = -G * mi * inter;
 }
 
@@ -127,25 +161,44 @@ __global__ __launch_bounds__(64) void chunk_sums(std::uint64_t n, const double* 
     out[2 * id + 1] = s.correction;
 }
 
-// One lane per term: the chunk results summed in chunk order, then written in
-// the output order {kinetic, potential, p, L}, kinetic halved
-// (diagnostics.cpp:197).
-__global__ void final_sums(std::uint32_t chunks, const double* __restrict__ parts,
-                           double* __restrict__ result) {
-    const int term = threadIdx.x;
-    if (term >= kTerms) return;
-    Compensated s;
-    for (std::uint32_t c = 0; c < chunks; ++c) {
-        s.add(parts[2 * (term * chunks + c)]);
-        s.add(parts[2 * (term * chunks + c) + 1]);
+// One block, 32 lanes per term: each lane folds a strided subset of the chunk
+// results (compensated), then the lanes' (total, correction) pairs are folded
+// in a fixed tree order; written in the output order {kinetic, potential, p,
+// L}, kinetic halved (diagnostics.cpp:197).
+__global__ __launch_bounds__(256) void final_sums(std::uint32_t chunks,
+                                                  const double* __restrict__ parts,
+                                                  double* __restrict__ result) {
+    __shared__ double s_total[256], s_corr[256];
+    const int term = threadIdx.x / 32, lane = threadIdx.x % 32;
+    Compensated acc;
+    for (std::uint32_t c = lane; c < chunks; c += 32) {
+        acc.add(parts[2 * (term * chunks + c)]);
+        acc.add(parts[2 * (term * chunks + c) + 1]);
     }
-    const double v = s.value();
-    if (term == 0) {
-        result[0] = v / 2;
-    } else if (term == 7) {
-        result[1] = v;
-    } else {
-        result[1 + term] = v;
+    s_total[threadIdx.x] = acc.total;
+    s_corr[threadIdx.x] = acc.correction;
+    __syncthreads();
+    for (int h = 16; h > 0; h >>= 1) {
+        if (lane < h) {
+            Compensated pair;
+            pair.add(s_total[threadIdx.x]);
+            pair.add(s_corr[threadIdx.x]);
+            pair.add(s_total[threadIdx.x + h]);
+            pair.add(s_corr[threadIdx.x + h]);
+            s_total[threadIdx.x] = pair.total;
+            s_corr[threadIdx.x] = pair.correction;
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        const double v = s_total[threadIdx.x] + s_corr[threadIdx.x];
+        if (term == 0) {
+            result[0] = v / 2;
+        } else if (term == 7) {
+            result[1] = v;
+        } else {
+            result[1 + term] = v;
+        }
     }
 }
 
@@ -176,7 +229,8 @@ void drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, const
 
 std::size_t diagnostics_scratch_bytes(std::uint64_t n) {
     const std::uint64_t chunks = n < 1024 ? n : 1024;
-    return sizeof(double) * (kTerms * n + 2 * kTerms * chunks + kTerms);
+    const std::uint64_t columns = (n + column_width(n) - 1) / column_width(n);
+    return sizeof(double) * (kTerms * n + columns * n + 2 * kTerms * chunks + kTerms);
 }
 
 void diagnostics_enqueue(orr_ctx* ctx, std::uint64_t n, const double* x, const double* y,
@@ -188,17 +242,24 @@ void diagnostics_enqueue(orr_ctx* ctx, std::uint64_t n, const double* x, const d
     }
     const std::uint32_t chunks = static_cast<std::uint32_t>(n < 1024 ? n : 1024);
     ctx->diag_scratch.reserve(diagnostics_scratch_bytes(n));
+    const std::uint64_t width = column_width(n);
+    const std::uint32_t columns = static_cast<std::uint32_t>((n + width - 1) / width);
     double* terms = ctx->diag_scratch.as<double>();
-    double* parts = terms + kTerms * n;
+    double* rows = terms + kTerms * n;
+    double* parts = rows + static_cast<std::size_t>(columns) * n;
     double* result = parts + 2 * kTerms * chunks;
     PhaseTimer t(ctx, kEnergy);
-    hipLaunchKernelGGL(diagnostic_terms, dim3(static_cast<std::uint32_t>((n + 255) / 256)),
-                       dim3(256), 0, ctx->stream, n, x, y, z, vx, vy, vz, m, e2, G, terms);
+    const std::uint32_t row_blocks = static_cast<std::uint32_t>((n + 255) / 256);
+    hipLaunchKernelGGL(potential_rows, dim3(row_blocks, columns), dim3(256), 0, ctx->stream, n,
+                       width, x, y, z, m, e2, rows);
+    ORR_LAUNCHED("potential_rows");
+    hipLaunchKernelGGL(diagnostic_terms, dim3(row_blocks), dim3(256), 0, ctx->stream, n, x, y, z,
+                       vx, vy, vz, m, rows, columns, G, terms);
     ORR_LAUNCHED("diagnostic_terms");
     hipLaunchKernelGGL(chunk_sums, dim3((kTerms * chunks + 63) / 64), dim3(64), 0, ctx->stream, n,
                        terms, chunks, parts);
     ORR_LAUNCHED("chunk_sums");
-    hipLaunchKernelGGL(final_sums, dim3(1), dim3(64), 0, ctx->stream, chunks, parts, result);
+    hipLaunchKernelGGL(final_sums, dim3(1), dim3(256), 0, ctx->stream, chunks, parts, result);
     ORR_LAUNCHED("final_sums");
     ORR_HIP(hipMemcpyAsync(out, result, sizeof(double) * kTerms, hipMemcpyDeviceToHost, ctx->stream));
 }

@@ -39,9 +39,10 @@
 
 namespace {
 
-// HBM held per admitted particle: the FP64 state (80 B) and the scratch of the
-// ordering, tree and direct tables (under 200 B), plus a fixed 1 MiB.
-constexpr std::uint64_t kBytesPerParticle = 280;
+// HBM held per admitted particle: the FP64 state (80 B), the scratch of the
+// ordering, tree and direct tables (under 200 B) and of the diagnostics (up to
+// 8 term and 16 column-chunk FP64 values, 192 B), plus a fixed 1 MiB.
+constexpr std::uint64_t kBytesPerParticle = 480;
 constexpr std::uint64_t kBytesPerRequest = 1u << 20;
 constexpr double kMaxBudget = 288e9;
 

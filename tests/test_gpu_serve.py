@@ -92,8 +92,8 @@ def test_matches_the_fp64_integrator_and_its_energy_drift(ctx):
 
 def test_memory_budget_bounds_admission(ctx):
     reqs = requests()
-    # Room for about two of these requests at a time (280 B per particle + 1 MiB each).
-    budget = 2 * (1 << 20) + 280 * 14000
+    # Room for about two of these requests at a time (480 B per particle + 1 MiB each).
+    budget = 2 * (1 << 20) + 480 * 14000
     s = E.Server(ctx, config(memory_budget=budget))
     ids = [s.submit(w, integ, EPS, 1e-3, steps) for w, integ, steps in reqs]
     s.round()
