@@ -168,6 +168,26 @@ def measure_ceilings(ctx) -> dict:
             "hbm_copy_GBs": copy / 1e9, "copy_bytes": 2 * (1 << 30), "copy_ms": copy_ms}
 
 
+def direct_accuracy(E, W, ctx, cpu, seeds=(1, 2, 3)) -> list:
+    """Config 2's accuracy statement: force error of one evaluation against the
+    FP64 direct sum at N = 65,536 (every target) and 262,144 (4096 targets), per seed."""
+    eps, rows = 0.005, []
+    rng = np.random.default_rng(77)
+    for n, sample in ((65536, 65536), (262144, 4096)):
+        for seed in seeds:
+            state = W.plummer(n, seed)
+            s = E.DeviceSystem(ctx, n)
+            s.upload(*state.arrays())
+            s.refresh(E.StepConfig(integrator="yoshida4", solver="direct", softening=eps))
+            acc = s.download()
+            s.close()
+            tgt = np.arange(n) if sample >= n else np.sort(rng.choice(n, size=sample, replace=False))
+            e = cpu.fp64_errors(state, eps, (acc["ax"], acc["ay"], acc["az"]), tgt)
+            rows.append({"n": n, "seed": seed, "rms_relative": e["rms_relative"],
+                         "max_relative": e["max_relative"], "targets": e["targets"]})
+    return rows
+
+
 def bench_barnes_hut(E, W, ctx, n: int, evals: int, error_targets: int, cpu) -> dict:
     """Barnes-Hut on the two-disc collision: N particles, theta 0.5, leaf 16."""
     eps, theta, cap = 0.01, 0.5, 16
@@ -435,7 +455,7 @@ def bench_serving(E, W, ctx, count: int, rate: float, seed: int, cpu) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--seed", type=int, default=1)
@@ -540,6 +560,8 @@ def main() -> int:
             tgt = np.sort(rng.choice(n, size=min(n, args.error_targets), replace=False))
             result["force_error"] = cpu.fp64_errors(state, eps, (acc["ax"], acc["ay"], acc["az"]), tgt)
             result["cpu_baseline"] = cpu.direct_baseline(state, eps)
+            log("[bench] direct accuracy at N = 65,536 and 262,144, seeds 1-3")
+            result["force_error_by_case"] = direct_accuracy(E, W, ctx, cpu)
         else:
             result["force_error"] = None
             result["cpu_baseline"] = None
