@@ -60,7 +60,8 @@ def build_host(verbose: bool = False) -> None:
     """C++20 ForceSolver hosts over the C ABI, and their C++ test driver."""
     tmp = HOST_LIBRARY + ".tmp"
     _run([CXX, "-std=c++20", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", *HOST_INCLUDES,
-          os.path.join(HERE, "host", "hip_solvers.cpp"), "-L", HERE, "-lorrery_hip",
+          os.path.join(HERE, "host", "hip_solvers.cpp"), os.path.join(HERE, "host", "hip_assembly.cpp"),
+          "-L", HERE, "-lorrery_hip",
           "-Wl,-rpath,$ORIGIN", "-o", tmp], "host library")
     os.replace(tmp, HOST_LIBRARY)
     tmp = HOST_TEST + ".tmp"

@@ -11,15 +11,20 @@
 #include <memory>
 #include <random>
 #include <span>
+#include <sstream>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
+#include "orrery_hip/hip_assembly.hpp"
 #include "orrery_hip/hip_solvers.hpp"
 
 using orrery::core::Real;
 using orrery::core::Softening;
 using orrery::core::Vec3Span;
 using orrery::solvers::HipDirectSolver;
+using orrery::solvers::HipSolverKind;
+using orrery::solvers::HipSolverSettings;
 using orrery::solvers::HipTreeSolver;
 using orrery::solvers::TreeParameters;
 
@@ -184,6 +189,66 @@ void tree_cases() {
     CHECK(corrected->parameters().opening_angle == 1 && corrected->parameters().leaf_capacity == 1);
 }
 
+// `[solver] kind` spellings and the assembly step (assembly.cpp:116-129, 246-272).
+void assembly_cases(bool device) {
+    using orrery::solvers::cpu_equivalent;
+    using orrery::solvers::parse_hip_solver_kind;
+    CHECK(parse_hip_solver_kind("hip-direct") == HipSolverKind::kHipDirect);
+    CHECK(parse_hip_solver_kind("hip-tree") == HipSolverKind::kHipTree);
+    CHECK(!parse_hip_solver_kind("HIP-tree").has_value());
+    CHECK(!parse_hip_solver_kind("hip-tree ").has_value());
+    CHECK(!parse_hip_solver_kind("sycl-tree").has_value());
+    CHECK(orrery::solvers::to_string(HipSolverKind::kHipTree) == "hip-tree");
+    CHECK(cpu_equivalent(HipSolverKind::kHipTree) == "barnes-hut");
+    CHECK(cpu_equivalent(HipSolverKind::kHipDirect) == "direct");
+    CHECK(orrery::solvers::is_tree_solver(HipSolverKind::kHipTree));
+    CHECK(!orrery::solvers::is_tree_solver(HipSolverKind::kHipDirect));
+
+    HipSolverSettings tree;
+    tree.kind = HipSolverKind::kHipTree;
+    tree.softening = 0.05;
+    tree.opening_angle = 1.7;  // corrected to 1 (octree.cpp:189-194)
+    tree.leaf_capacity = 0;    // corrected to 1
+    tree.quadrupole = true;
+    HipSolverSettings direct;
+    direct.kind = HipSolverKind::kHipDirect;
+    direct.softening = 0.1;
+    std::ostringstream report;
+    if (device) {
+        auto t = orrery::solvers::make_hip_solver(tree, report);
+        auto d = orrery::solvers::make_hip_solver(direct, report);
+        CHECK(t != nullptr && d != nullptr && report.str().empty());
+        if (t == nullptr || d == nullptr) return;
+        CHECK(t->name() == "hip-tree" && d->name() == "hip-direct");
+        CHECK(t->softening().squared() == 0.05 * 0.05 && d->softening().squared() == 0.1 * 0.1);
+        const auto* ht = dynamic_cast<const HipTreeSolver*>(t.get());
+        CHECK(ht != nullptr);
+        if (ht != nullptr) {
+            CHECK(ht->parameters().opening_angle == 1.0);
+            CHECK(ht->parameters().leaf_capacity == 1);
+            CHECK(ht->parameters().quadrupole);
+        }
+        return;
+    }
+    CHECK(orrery::solvers::try_make_hip_solver(tree) == nullptr);
+    // Fallback allowed: a null and the reference's note, naming the CPU solver.
+    CHECK(orrery::solvers::make_hip_solver(tree, report) == nullptr);
+    CHECK(report.str() ==
+          "note: hip-tree was asked for but no usable device was found, so this run uses "
+          "barnes-hut on the CPU\n");
+    // Fallback refused: the run does not start.
+    direct.allow_cpu_fallback = false;
+    bool refused = false;
+    try {
+        (void)orrery::solvers::make_hip_solver(direct, report);
+    } catch (const std::runtime_error& e) {
+        refused = std::string(e.what()) ==
+                  "solver.kind is hip-direct and no usable device was found. Set "
+                  "solver.allow_cpu_fallback = true to run on the CPU instead";
+    }
+    CHECK(refused);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -192,9 +257,11 @@ int main(int argc, char** argv) {
         // cuda_direct_solver_test.cpp:132-136: no device -> null, the caller falls back.
         CHECK(HipDirectSolver::try_create(Softening{0.1}) == nullptr);
         CHECK(HipTreeSolver::try_create() == nullptr);
+        assembly_cases(false);
     } else {
         direct_cases();
         tree_cases();
+        assembly_cases(true);
     }
     if (failures == 0) std::printf("all host-solver checks passed\n");
     return failures == 0 ? 0 : 1;

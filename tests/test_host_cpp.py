@@ -20,7 +20,8 @@ REFERENCE_INCLUDE = "/root/reference/include"
 def test_hosts_compile_against_reference_headers():
     cmd = ["g++", "-std=c++20", "-fsyntax-only", "-Wall", "-Werror", "-I", REFERENCE_INCLUDE,
            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "orrery_amd", "host", "include"),
-           os.path.join(ROOT, "orrery_amd", "host", "hip_solvers.cpp")]
+           os.path.join(ROOT, "orrery_amd", "host", "hip_solvers.cpp"),
+           os.path.join(ROOT, "orrery_amd", "host", "hip_assembly.cpp")]
     res = subprocess.run(cmd, capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
 
