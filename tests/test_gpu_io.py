@@ -32,8 +32,13 @@ def stepped(ctx, n=3000, seed=2, steps=5, integrator="yoshida4", solver="barnes-
     return s, p, cfg
 
 
-def test_resume_from_checkpoint_is_bitwise(ctx, tmp_path):
-    a, p, cfg = stepped(ctx)
+@pytest.mark.parametrize("integrator", ["velocity-verlet", "yoshida4", "runge-kutta4"])
+@pytest.mark.parametrize("solver", ["direct", "barnes-hut"])
+def test_resume_from_checkpoint_is_bitwise(ctx, tmp_path, integrator, solver):
+    # simulation_test.cpp:225-318: for every integrator and solver, a run resumed
+    # from its checkpoint continues bit for bit (the stored accelerations are
+    # current, so the resumed run skips the refresh).
+    a, p, cfg = stepped(ctx, integrator=integrator, solver=solver)
     path = str(tmp_path / "run.ock")
     a.checkpoint(path, 5, "[run]\ntimestep = 0.001\n")
     b, step, text = E.DeviceSystem.from_checkpoint(ctx, path)

@@ -46,6 +46,53 @@ def test_kepler_fixture_frames(ctx):
     s.close()
 
 
+def energy_history(ctx, integrator, orbits=60, steps_per_orbit=200, every=10):
+    # energy_behaviour_test.cpp:82-133: Kepler e = 0.5, a = 1, equal unit masses,
+    # no softening; the largest relative energy error over the first and the last
+    # twentieth of the run, and over all of it.
+    s = E.DeviceSystem(ctx, 2)
+    s.upload(*kepler_state())
+    cfg = E.StepConfig(integrator=integrator, solver="direct", softening=0.0)
+    e0 = s.diagnostics(0.0)["total"]
+    s.refresh(cfg)
+    dt = 2 * np.pi / np.sqrt(2.0) / steps_per_orbit
+    total = orbits * steps_per_orbit
+    window = total // 20
+    early = late = extreme = 0.0
+    for done in range(every, total + 1, every):
+        s.step(cfg, dt, every)
+        err = abs((s.diagnostics(0.0)["total"] - e0) / e0)
+        extreme = max(extreme, err)
+        if done <= window:
+            early = max(early, err)
+        if done > total - window:
+            late = max(late, err)
+    s.close()
+    return early, late, extreme
+
+
+def test_symplectic_energy_envelope(ctx):
+    # energy_behaviour_test.cpp:137-176: over 60 orbits the error of velocity
+    # Verlet and Yoshida-4 stays inside an envelope that does not grow; Yoshida's
+    # is the smaller. (FP64 oracle on this orbit: 2.7e-3 and 2.0e-5.)
+    vv = energy_history(ctx, "velocity-verlet")
+    y4 = energy_history(ctx, "yoshida4")
+    print(f"VV early/late/extreme {vv}, Y4 {y4}")
+    for early, late, extreme in (vv, y4):
+        assert extreme <= 0.005 and late <= 2 * early
+    assert y4[2] < vv[2]
+
+
+def test_rk4_energy_error_grows(ctx):
+    # energy_behaviour_test.cpp:178-206: RK4 has no conserved nearby energy; its
+    # error grows by more than ten times between the first and last twentieth,
+    # and ends above Yoshida-4's. (FP64 oracle: 3.5e-6 -> 5.6e-5.)
+    rk = energy_history(ctx, "runge-kutta4")
+    y4 = energy_history(ctx, "yoshida4")
+    print(f"RK4 early/late/extreme {rk}, Y4 {y4}")
+    assert rk[1] >= 10 * rk[0] and rk[1] > y4[1]
+
+
 def test_agreement_fixture_energy(ctx):
     t = otj.read("tests/golden/agreement.otj")
     rows = otj.read_csv("tests/golden/agreement.csv")

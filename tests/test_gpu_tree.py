@@ -94,6 +94,50 @@ def test_identical_points_stop_at_depth_21(ctx):
     assert np.all(ax == 0) and np.all(ay == 0) and np.all(az == 0)
 
 
+def test_zero_masses_give_no_nan(ctx):
+    # octree_test.cpp:434-451: a sized but unfilled container (zero masses at the
+    # origin) builds a tree with zero masses and finite centres and radii.
+    n = 16
+    z = np.zeros(n)
+    p = oracle.Particles(z, z.copy(), z.copy(), z.copy(), z.copy(), z.copy(), z.copy())
+    s = E.HipTreeSolver(E.TreeParameters(0.5, 4, quadrupole=True), E.Softening(0.1), context=ctx)
+    acc = [np.full(n, np.nan) for _ in range(3)]
+    s.evaluate((p.x, p.y, p.z), p.m, acc)
+    nodes = s.nodes()
+    assert np.all(nodes["mass"] == 0)
+    assert np.all(np.isfinite(nodes["com"])) and np.all(np.isfinite(nodes["acc_r2"]))
+    assert np.all(np.isfinite(s.quadrupoles()))
+    assert all(np.all(a == 0) for a in acc)
+
+
+@pytest.mark.parametrize("quadrupole", [False, True])
+def test_massless_particles_among_massive_ones(ctx, quadrupole):
+    # Zero-mass cells take the mean of their particles (octree.cpp:75-84, 120-135):
+    # every third particle massless, whole massless leaves included; moments
+    # bitwise against the oracle, forces finite and equal to the massive set's.
+    p = oracle.plummer(6000, 31)
+    m = p.m.copy()
+    m[::3] = 0.0
+    m[p.x > 1.5] = 0.0
+    q = oracle.Particles(p.x, p.y, p.z, p.vx, p.vy, p.vz, m)
+    s = E.HipTreeSolver(E.TreeParameters(0.5, 8, quadrupole=quadrupole), E.Softening(0.02), context=ctx)
+    acc = [np.zeros(q.n) for _ in range(3)]
+    s.evaluate((q.x, q.y, q.z), q.m, acc)
+    assert all(np.all(np.isfinite(a)) for a in acc)
+    codes, index, origin, size = oracle.morton_order(q.x, q.y, q.z)
+    i = index.astype(np.int64)
+    t = oracle.tree_build(q.x[i], q.y[i], q.z[i], q.m[i], codes, origin, size, 0.5, 8, quadrupole=quadrupole)
+    got = s.nodes()
+    assert np.any(t.mass == 0)
+    assert np.array_equal(got["mass"], t.mass) and np.array_equal(got["com"], t.com)
+    assert np.array_equal(got["acc_r2"], t.acc_r2)
+    if quadrupole:
+        assert np.array_equal(s.quadrupoles(), t.quad)
+    ref = oracle.reference_acceleration(q.x, q.y, q.z, q.m, 0.02 ** 2)
+    worst, rms = oracle.error_summary(oracle.relative_errors(*acc, *ref))
+    assert rms < 1e-2 and worst < 1e-1, (rms, worst)
+
+
 @pytest.mark.parametrize("gen,n,theta,cap", [("plummer", 20000, 0.5, 16), ("plummer", 20000, 0.8, 32),
                                              ("disc", 50000, 0.5, 16), ("disc", 50000, 0.6, 16)])
 def test_counts_equal_fp32_policy_mirror(ctx, gen, n, theta, cap):
