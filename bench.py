@@ -161,10 +161,12 @@ def measure_ceilings(ctx) -> dict:
     ctx.probe("fma", 256)  # warm-up
     fma, _ = ctx.probe("fma", 4096)
     rsq, _ = ctx.probe("rsq", 1024)
+    mix, _ = ctx.probe("mix", 1024)
     ctx.probe("read", 1 << 28)
     read, _ = ctx.probe("read", 1 << 31)
     copy, copy_ms = ctx.probe("copy", 1 << 30)
-    return {"fp32_fma_TFLOPs": fma / 1e12, "rsq_Gops": rsq / 1e9, "hbm_read_GBs": read / 1e9,
+    return {"fp32_fma_TFLOPs": fma / 1e12, "rsq_Gops": rsq / 1e9,
+            "interaction_mix_per_s": mix, "hbm_read_GBs": read / 1e9,
             "hbm_copy_GBs": copy / 1e9, "copy_bytes": 2 * (1 << 30), "copy_ms": copy_ms}
 
 
@@ -541,6 +543,9 @@ def main() -> int:
         eval_ms = sum(timings.get(p, (0.0, 0))[0] for p in
                       ("direct_prepare", "direct_kernel", "direct_reduce", "bbox", "encode", "sort")) / max(1, evals)
         achieved = FLOP_PER_INTERACTION * n * (n - 1) / (kernel_ms * 1e-3) / 1e12
+        # The issue-rate ceiling: one interaction's 12 VALU + 1 v_rsq_f32, no memory.
+        ctx.probe("mix", 256)
+        mix, _ = ctx.probe("mix", 2048)
         result.update({
             "value": value,
             "ms_per_step": elapsed * 1e3 / max(1, args.steps),
@@ -550,7 +555,12 @@ def main() -> int:
             "roofline": {"bound": "fp32-valu", "kernel": "direct_tiles", "achieved": achieved,
                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                          "traffic": measured_traffic(n),
+                         "issue_ceiling": {"interactions_per_s": mix,
+                                           "TFLOPs": FLOP_PER_INTERACTION * mix / 1e12,
+                                           "frac": achieved * 1e12 / FLOP_PER_INTERACTION / mix},
                          "note": "20 flop per interaction x N(N-1) per launch / mean launch time; "
+                                 "issue_ceiling = the engine's mix probe (12 VALU + 1 v_rsq_f32 per "
+                                 "interaction, no memory, packed FP32 off); "
                                  "traffic = calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per "
                                  "launch (profiles/direct_tiles_traffic.json)"},
         })

@@ -196,11 +196,14 @@ ORR_API int orr_tree_quadrupoles(orr_ctx* ctx, double* quadrupoles);
  * fma_probe, div_sqrt_probe, read_probe). kind: 0 FP32 FMA chains (size =
  * iterations; rate in flop/s), 1 v_rsq_f32 chains (iterations; rsq/s),
  * 2 HBM read (size = bytes; bytes/s), 3 HBM copy (bytes copied; bytes read +
- * written per second). */
+ * written per second), 4 one softened-gravity interaction's instruction mix
+ * without memory (eleven FMAs, one add, one v_rsq_f32; size = iterations of
+ * four; rate in interactions/s): the issue-rate ceiling of the direct kernels. */
 #define ORR_PROBE_FMA 0
 #define ORR_PROBE_RSQ 1
 #define ORR_PROBE_READ 2
 #define ORR_PROBE_COPY 3
+#define ORR_PROBE_MIX 4
 ORR_API int orr_probe(orr_ctx* ctx, int32_t kind, uint64_t size, double* rate_out,
                       double* milliseconds_out);
 

@@ -356,10 +356,11 @@ int orr_tree_quadrupoles(orr_ctx* ctx, double* out) {
 int orr_probe(orr_ctx* ctx, int32_t kind, uint64_t size, double* rate, double* ms) {
     return guarded([&] {
         require(ctx != nullptr && rate != nullptr && ms != nullptr, "null argument");
-        require(kind >= 0 && kind <= 3, "probe kind must be 0..3");
+        require(kind >= 0 && kind <= 4, "probe kind must be 0..4");
         require(size > 0, "probe size must be > 0");
-        require(kind >= 2 || size <= (uint64_t{1} << 24), "probe iterations must be <= 2^24");
-        require(kind < 2 || size <= (uint64_t{1} << 34), "probe buffer must be <= 16 GiB");
+        const bool iterations = kind <= 1 || kind == 4;
+        require(!iterations || size <= (uint64_t{1} << 24), "probe iterations must be <= 2^24");
+        require(iterations || size <= (uint64_t{1} << 34), "probe buffer must be <= 16 GiB");
         use_device(ctx);
         ctx->sync();
         orr::probe(ctx, kind, size, rate, ms);

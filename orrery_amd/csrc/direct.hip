@@ -239,7 +239,11 @@ __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict_
         const std::uint32_t k = min(base + r * kBlock, npad - 1);
         const float4 o = packed[k];
         const float4 l = packed_lo[k];
-        const double4 c = centres[k / kTile];
+        // A wave's 64 targets are one whole tile (npad is a multiple of kTile,
+        // and a wave past the end clamps every lane to the last tile), so the
+        // tile's centre and extents are wave-uniform: scalar loads, SGPRs.
+        const std::uint32_t tile = __builtin_amdgcn_readfirstlane(k / kTile);
+        const double4 c = centres[tile];
         ox[r] = o.x;
         oy[r] = o.y;
         oz[r] = o.z;
@@ -249,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict_
         cx[r] = c.x;
         cy[r] = c.y;
         cz[r] = c.z;
-        ce[r] = extents[k / kTile];
+        ce[r] = extents[tile];
         sx[r] = sy[r] = sz[r] = 0.f;
         kx[r] = ky[r] = kz[r] = 0.f;
     }

@@ -109,7 +109,7 @@ struct orr_ctx {
     // Direct-kernel configuration (orr_ctx_set_direct_config).
     int direct_targets_per_lane = 1;
     int direct_refine = 0;
-    std::uint32_t direct_block_target = 8192;
+    std::uint32_t direct_block_target = 32768;
     // Barnes-Hut walk configuration (orr_ctx_set_tree_config); 0 chooses by size.
     int walk_targets_per_lane = 0;
 

@@ -136,10 +136,11 @@ class Context:
         return dict(targets_per_lane=c.targets_per_lane)
 
     # -- machine ceilings (benchmarks/cuda_probes.cu equivalents) -----------
-    PROBES = {"fma": 0, "rsq": 1, "read": 2, "copy": 3}
+    PROBES = {"fma": 0, "rsq": 1, "read": 2, "copy": 3, "mix": 4}
 
     def probe(self, kind: str, size: int) -> tuple[float, float]:
-        """(rate, milliseconds): flop/s (fma), rsq/s (rsq) or bytes/s (read, copy)."""
+        """(rate, milliseconds): flop/s (fma), rsq/s (rsq), interactions/s (mix: one
+        interaction's 12 VALU + 1 v_rsq_f32 without memory) or bytes/s (read, copy)."""
         rate, ms = ctypes.c_double(), ctypes.c_double()
         C.check(self._lib.orr_probe(self.handle, self.PROBES[kind], int(size), ctypes.byref(rate),
                                     ctypes.byref(ms)))
