@@ -22,7 +22,8 @@ Further legs in the same JSON line:
 * ``served``     -- Poisson arrivals of run requests (N log-uniform in [1e3, 2e6],
   70/30 Plummer / two-disc collision, 100-1000 Yoshida-4 or velocity Verlet
   steps) through the continuous-batching server: p50/p99 time to completion,
-  throughput, per-request energy drift against the CPU reference;
+  throughput, per-request energy drift against the CPU reference, and beside
+  them the same trace served one request at a time on the single-system path;
 * ``cpu_baseline`` and the ``force_error`` figures -- the CPU leg: the
   reference's threaded AVX2 FP64 kernel (or the oracle's restatement) timed on
   this host's cores over a bounded sample of the same input, and the FP64
@@ -408,6 +409,39 @@ def bench_serving(E, W, ctx, count: int, rate: float, seed: int, cpu) -> dict:
     drift = [(s["end"]["total"] - s["start"]["total"]) / abs(s["start"]["total"]) for s in st]
     particle_steps = sum(n * k for n, k in zip(sizes, steps))
 
+    # The plain single-system form on the same arrival trace: each request alone
+    # on the device-resident path (orr_system: upload, start diagnostics, refresh,
+    # steps, end diagnostics, download), served first in, first out. Each service
+    # time is measured on the device: set-up and first step, then up to 100 more
+    # steps whose mean stands for the rest.
+    log("[bench] serving: the same requests one at a time on the single-system path")
+    service = []
+    for k in range(count):
+        cfg = E.StepConfig(integrator=integ[k], solver="direct" if sizes[k] < below else "barnes-hut",
+                           softening=eps, opening_angle=0.5, leaf_capacity=16)
+        t0 = time.perf_counter()
+        s = E.DeviceSystem(ctx, sizes[k])
+        s.upload(*work[k].arrays())
+        s.diagnostics(eps)
+        s.refresh(cfg)
+        s.step(cfg, dt, 1)
+        first = time.perf_counter() - t0
+        rest = min(steps[k] - 1, 100)
+        t1 = time.perf_counter()
+        if rest > 0:
+            s.step(cfg, dt, rest)
+        per_step = (time.perf_counter() - t1) / max(1, rest)
+        t2 = time.perf_counter()
+        s.diagnostics(eps)
+        s.download()
+        s.close()
+        service.append(first + per_step * (steps[k] - 1) + (time.perf_counter() - t2))
+    free_at, fifo = 0.0, []
+    for k in range(count):
+        free_at = max(float(arrivals[k]), free_at) + service[k]
+        fifo.append(free_at - float(arrivals[k]))
+    fifo = np.array(fifo)
+
     # Energy drift against the CPU reference (FP64 direct, same integrator and
     # initial state) for the requests the CPU can redo in a bounded time.
     cpu_drift = None
@@ -445,6 +479,15 @@ def bench_serving(E, W, ctx, count: int, rate: float, seed: int, cpu) -> dict:
         "latency_s": {"p50": float(np.percentile(latency, 50)), "p99": float(np.percentile(latency, 99)),
                       "max": float(latency.max()), "min": float(latency.min())},
         "queue_wait_s": {"p50": float(np.percentile(queued, 50)), "max": float(queued.max())},
+        "single_system_fifo": {
+            "latency_s": {"p50": float(np.percentile(fifo, 50)), "p99": float(np.percentile(fifo, 99)),
+                          "max": float(fifo.max()), "min": float(fifo.min())},
+            "makespan_s": float(max(float(arrivals[k]) + fifo[k] for k in range(count))),
+            "busy_s": float(sum(service)),
+            "method": "each request alone on the device-resident single-system path, first in first "
+                      "out on the same arrival times; service time = measured set-up + first step + "
+                      "the mean of up to 100 further steps x the remaining steps + end diagnostics "
+                      "and download"},
         "rounds": stats["rounds"], "batch_builds": stats["batch_builds"],
         "busy_s": stats["busy_s"], "peak_resident_bytes": stats["peak_resident_bytes"],
         "memory_budget": stats["memory_budget"],
