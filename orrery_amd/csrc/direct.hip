@@ -262,17 +262,20 @@ __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict_
         source_tile<R, kSafe, kRefine>(packed, packed_lo, centres, extents, J, eps2, ox, oy, oz,
                                        lx, ly, lz, cx, cy, cz, ce, sx, sy, sz, kx, ky, kz);
 
-    float* out = partial + static_cast<std::size_t>(split) * 6 * npad;
+    // Component c of target k at partial[(6 split + c) npad + k], in 64-bit
+    // offsets (6 npad exceeds 32 bits from npad = 2^30 / 1.5).
+    const std::size_t np = npad;
+    float* out = partial + static_cast<std::size_t>(split) * 6 * np;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const std::uint32_t k = base + r * kBlock;
         if (k < npad) {
             out[k] = sx[r];
-            out[npad + k] = sy[r];
-            out[2 * npad + k] = sz[r];
-            out[3 * npad + k] = kx[r];
-            out[4 * npad + k] = ky[r];
-            out[5 * npad + k] = kz[r];
+            out[np + k] = sy[r];
+            out[2 * np + k] = sz[r];
+            out[3 * np + k] = kx[r];
+            out[4 * np + k] = ky[r];
+            out[5 * np + k] = kz[r];
         }
     }
 }
@@ -289,10 +292,11 @@ __global__ __launch_bounds__(256) void direct_reduce(const float* __restrict__ p
     if (k >= n) return;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (std::uint32_t s = 0; s < splits; ++s) {
-        const float* p = partial + static_cast<std::size_t>(s) * 6 * npad;
-        a0 += static_cast<double>(p[k]) - static_cast<double>(p[3 * npad + k]);
-        a1 += static_cast<double>(p[npad + k]) - static_cast<double>(p[4 * npad + k]);
-        a2 += static_cast<double>(p[2 * npad + k]) - static_cast<double>(p[5 * npad + k]);
+        const std::size_t np = npad;
+        const float* p = partial + static_cast<std::size_t>(s) * 6 * np;
+        a0 += static_cast<double>(p[k]) - static_cast<double>(p[3 * np + k]);
+        a1 += static_cast<double>(p[np + k]) - static_cast<double>(p[4 * np + k]);
+        a2 += static_cast<double>(p[2 * np + k]) - static_cast<double>(p[5 * np + k]);
     }
     const std::uint64_t dst = order != nullptr ? order[k] : k;
     ax[dst] = G * a0;
