@@ -159,10 +159,12 @@ def measured_traffic(n: int):
 
 def measure_ceilings(ctx) -> dict:
     """FP32 FMA, v_rsq_f32, HBM read and copy rates measured by the engine's probes."""
+    # Each rate over tens of milliseconds: launches of a few milliseconds read
+    # low while the clocks ramp.
     ctx.probe("fma", 256)  # warm-up
-    fma, _ = ctx.probe("fma", 4096)
-    rsq, _ = ctx.probe("rsq", 1024)
-    mix, _ = ctx.probe("mix", 1024)
+    fma, _ = ctx.probe("fma", 65536)
+    rsq, _ = ctx.probe("rsq", 16384)
+    mix, _ = ctx.probe("mix", 16384)
     ctx.probe("read", 1 << 28)
     read, _ = ctx.probe("read", 1 << 31)
     copy, copy_ms = ctx.probe("copy", 1 << 30)
@@ -586,9 +588,10 @@ def main() -> int:
         eval_ms = sum(timings.get(p, (0.0, 0))[0] for p in
                       ("direct_prepare", "direct_kernel", "direct_reduce", "bbox", "encode", "sort")) / max(1, evals)
         achieved = FLOP_PER_INTERACTION * n * (n - 1) / (kernel_ms * 1e-3) / 1e12
-        # The issue-rate ceiling: one interaction's 12 VALU + 1 v_rsq_f32, no memory.
+        # The issue-rate ceiling: one interaction's 12 VALU + 1 v_rsq_f32, no
+        # memory, sustained for about 0.1 s.
         ctx.probe("mix", 256)
-        mix, _ = ctx.probe("mix", 2048)
+        mix, _ = ctx.probe("mix", 65536)
         result.update({
             "value": value,
             "ms_per_step": elapsed * 1e3 / max(1, args.steps),
