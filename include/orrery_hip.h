@@ -114,7 +114,8 @@ ORR_API int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* args);
  * targets_per_lane in {1, 2, 4}; refine = 1 applies a Newton step to the
  * hardware reciprocal square root (off by default: v_rsq_f32 already keeps the
  * force within 1e-6 relative); block_target is the number of thread blocks the
- * source range is split to reach. Zero fields keep the current value. */
+ * source range is split to reach (default 32768). targets_per_lane 0,
+ * refine -1 and block_target 0 keep the current value. */
 typedef struct {
     int32_t targets_per_lane;
     int32_t refine;
@@ -164,7 +165,8 @@ ORR_API int orr_tree_eval(orr_ctx* ctx, const orr_tree_args* args, orr_tree_stat
  * traversal): targets_per_lane 1 or 2 -- how many targets each lane of a
  * wave walks together -- or -1 to choose by size (2 for one system of 2^20 or
  * more particles, else 1; the default). Results are bit-identical either way;
- * only the speed differs. Zero keeps the current value. */
+ * only the speed differs. Zero keeps the current value; the getter reports
+ * -1 while the choice is by size. */
 typedef struct {
     int32_t targets_per_lane;
     int32_t reserved[3];
@@ -330,12 +332,20 @@ typedef struct {
     int32_t graph;                   /* a captured step is in use */
     uint64_t particles, direct_particles, tree_particles;
     uint64_t direct_pairs;           /* sum of N(N-1) over direct systems, per evaluation */
+    uint64_t device_bytes;           /* device memory the batch holds now */
 } orr_batch_info;
 
 ORR_API int orr_batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t count,
                              const orr_batch_config* config, orr_batch** out);
 ORR_API int orr_batch_destroy(orr_batch* batch);
 ORR_API int orr_batch_info_get(orr_batch* batch, orr_batch_info* out);
+/* No reference counterpart (the reference runs one system in host memory): an
+ * upper bound of the device bytes a batch of these systems holds at its
+ * largest -- state, tables and every scratch buffer after any refresh, step,
+ * autotune or diagnostics -- for callers that plan device memory, such as the
+ * serving loop's admission. Bounds of disjoint sets of systems add. */
+ORR_API int orr_batch_bytes_bound(const orr_batch_system* systems, uint32_t count,
+                                  const orr_batch_config* config, uint64_t* bytes_out);
 /* Host arrays of the system's `count` FP64 values each (null: left as is). */
 ORR_API int orr_batch_upload(orr_batch* batch, uint32_t system, const double* x, const double* y,
                              const double* z, const double* vx, const double* vy,
@@ -402,6 +412,7 @@ typedef struct {
     double submitted_s, admitted_s, completed_s;   /* server clock, seconds */
     double diagnostics_start[8];     /* at admission: {kinetic, potential, p, L} */
     double diagnostics_end[8];       /* at completion */
+    uint64_t charged_bytes;          /* device memory the request holds against the budget */
 } orr_request_status;
 
 typedef struct {

@@ -91,7 +91,8 @@ class OrrBatchInfo(ctypes.Structure):
                 ("leaf_capacity", ctypes.c_uint32), ("direct_tiles", ctypes.c_uint32),
                 ("direct_items", ctypes.c_uint32), ("graph", ctypes.c_int32),
                 ("particles", ctypes.c_uint64), ("direct_particles", ctypes.c_uint64),
-                ("tree_particles", ctypes.c_uint64), ("direct_pairs", ctypes.c_uint64)]
+                ("tree_particles", ctypes.c_uint64), ("direct_pairs", ctypes.c_uint64),
+                ("device_bytes", ctypes.c_uint64)]
 
 
 class OrrServerConfig(ctypes.Structure):
@@ -113,7 +114,7 @@ class OrrRequestStatus(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int32), ("reserved", ctypes.c_int32), ("steps_done", ctypes.c_uint64),
                 ("submitted_s", ctypes.c_double), ("admitted_s", ctypes.c_double),
                 ("completed_s", ctypes.c_double), ("diagnostics_start", ctypes.c_double * 8),
-                ("diagnostics_end", ctypes.c_double * 8)]
+                ("diagnostics_end", ctypes.c_double * 8), ("charged_bytes", ctypes.c_uint64)]
 
 
 class OrrServerStats(ctypes.Structure):
@@ -171,6 +172,7 @@ SIGNATURES = {
                          ctypes.POINTER(OrrBatchConfig), ctypes.POINTER(_vp)],
     "orr_batch_destroy": [_vp],
     "orr_batch_info_get": [_vp, ctypes.POINTER(OrrBatchInfo)],
+    "orr_batch_bytes_bound": [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_uint64)],
     "orr_batch_upload": [_vp, ctypes.c_uint32] + [_vp] * 7,
     "orr_batch_download": [_vp, ctypes.c_uint32] + [_vp] * 9,
     "orr_batch_copy_system": [_vp, ctypes.c_uint32, _vp, ctypes.c_uint32],

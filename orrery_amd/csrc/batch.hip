@@ -431,6 +431,7 @@ std::uint32_t internal_of(orr_batch* b, std::uint32_t system) {
 
 void use_device(orr_batch* b) { ORR_HIP(hipSetDevice(b->owner->device)); }
 
+
 void destroy(orr_batch* b) noexcept {
     if (b == nullptr) return;
     (void)hipSetDevice(b->owner->device);
@@ -447,6 +448,32 @@ void destroy(orr_batch* b) noexcept {
 }
 
 }  // namespace
+
+std::uint64_t system_bytes_bound(std::uint64_t n, int solver, const orr_batch_config& cfg) {
+    if (n == 0) return 0;
+    const std::uint32_t below = cfg.direct_below == 0 ? 4096u : cfg.direct_below;
+    const bool tree = solver == 1 || (solver == -1 && n >= below);
+    // FP64 state, its table rows, and the diagnostics scratch.
+    std::uint64_t total = grown_bytes(80 * n) + 4 * n + 256 + grown_bytes(diagnostics_scratch_bytes(n));
+    // Ordering buffers, whichever way the system's ordering is laid out (alone,
+    // or one of several segments with 2048-particle bounding-box chunks).
+    const std::uint32_t chunks = static_cast<std::uint32_t>((n + kChunk - 1) / kChunk);
+    const OrderSizes a = order_sizes(n, 2, chunks, tree), b = order_sizes(n, 1, 0, tree);
+    total += grown_bytes(std::max(a.aos, b.aos)) + grown_bytes(std::max(a.bbox, b.bbox)) +
+             2 * grown_bytes(a.codes) + 2 * grown_bytes(a.index) + grown_bytes(std::max(a.hist, b.hist));
+    if (tree) {
+        // The smallest leaf capacity the autotuner tries holds the most nodes.
+        const std::uint64_t cap = cfg.leaf_capacity == 0 ? 8 : cfg.leaf_capacity;
+        total += grown_bytes(tree_bytes(n, cap, cfg.quadrupole != 0, 1));
+    } else {
+        // The shortest work item the autotuner tries makes the most partial slots.
+        const std::uint64_t tiles = (n + 63) / 64;
+        const std::uint64_t per = cfg.tiles_per_item == 0 ? 4 : cfg.tiles_per_item;
+        const std::uint64_t slots = tiles * ((tiles + per - 1) / per);
+        total += grown_bytes(direct_batch_bytes(tiles, slots)) + 16 * (tiles + slots) + 12 * chunks;
+    }
+    return total;
+}
 
 void batch_step_enqueue(orr_batch* b, std::uint64_t steps) { step_enqueue(b, steps); }
 void batch_wait(orr_batch* b) { sync_all(b); }
@@ -571,6 +598,22 @@ int orr_batch_info_get(orr_batch* b, orr_batch_info* out) {
         out->direct_particles = b->n_direct;
         out->tree_particles = b->n - b->n_direct;
         out->direct_pairs = b->direct_pairs;
+        out->device_bytes = b->state.bytes + b->tables.bytes + orr::ctx_device_bytes(b->dctx) +
+                            orr::ctx_device_bytes(b->tctx);
+    });
+}
+
+int orr_batch_bytes_bound(const orr_batch_system* systems, uint32_t count,
+                          const orr_batch_config* config, uint64_t* bytes_out) {
+    return guarded([&] {
+        require(config != nullptr && bytes_out != nullptr, "null argument");
+        require(count == 0 || systems != nullptr, "systems is null");
+        std::uint64_t total = 16 * 256;  // table padding
+        for (std::uint32_t u = 0; u < count; ++u) {
+            require(systems[u].solver >= -1 && systems[u].solver <= 1, "solver must be -1, 0 or 1");
+            total += orr::system_bytes_bound(systems[u].count, systems[u].solver, *config) + 256;
+        }
+        *bytes_out = total;
     });
 }
 

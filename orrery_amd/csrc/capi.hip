@@ -288,7 +288,8 @@ int orr_ctx_get_tree_config(orr_ctx* ctx, orr_tree_config* config) {
     return guarded([&] {
         require(ctx != nullptr && config != nullptr, "null argument");
         *config = orr_tree_config{};
-        config->targets_per_lane = ctx->walk_targets_per_lane;
+        // -1 while the choice is by size, as orr_ctx_set_tree_config spells it.
+        config->targets_per_lane = ctx->walk_targets_per_lane != 0 ? ctx->walk_targets_per_lane : -1;
     });
 }
 

@@ -27,13 +27,15 @@ void check_hip(hipError_t e, const char* what, const char* file, int line) {
 
 std::atomic<std::uint64_t> g_reallocations{0};
 
+std::size_t grown_bytes(std::size_t wanted) { return wanted == 0 ? 0 : wanted + wanted / 4 + 256; }
+
 void Buffer::reserve(std::size_t wanted) {
     if (wanted <= bytes) return;
     g_reallocations.fetch_add(1, std::memory_order_relaxed);
     release();
     // Grow by a quarter beyond the request so that slowly growing systems do
     // not reallocate on every step.
-    const std::size_t grown = wanted + wanted / 4 + 256;
+    const std::size_t grown = grown_bytes(wanted);
     void* p = nullptr;
     ORR_HIP(hipMalloc(&p, grown));
     ptr = p;
@@ -44,6 +46,12 @@ void Buffer::release() noexcept {
     if (ptr != nullptr) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
+}
+
+std::size_t ctx_device_bytes(const orr_ctx* ctx) {
+    std::size_t total = 0;
+    for (const Buffer* b : const_cast<orr_ctx*>(ctx)->buffers()) total += b->bytes;
+    return total;
 }
 
 orr_ctx* make_context(int device) {
@@ -62,14 +70,7 @@ void free_context(orr_ctx* ctx) noexcept {
     if (ctx == nullptr) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (Buffer* b : {&ctx->direct_packed, &ctx->direct_centres, &ctx->direct_partials,
-                      &ctx->order_codes, &ctx->order_index, &ctx->order_codes_alt,
-                      &ctx->order_index_alt, &ctx->order_hist, &ctx->order_bbox,
-                      &ctx->order_flags, &ctx->tree_nodes, &ctx->tree_aux, &ctx->tree_work,
-                      &ctx->tree_counts, &ctx->tree_sorted, &ctx->tree_leafref,
-                      &ctx->diag_scratch}) {
-        b->release();
-    }
+    for (Buffer* b : ctx->buffers()) b->release();
     for (auto& p : ctx->pending) {
         (void)hipEventDestroy(p.start);
         (void)hipEventDestroy(p.stop);

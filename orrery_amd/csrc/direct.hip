@@ -490,14 +490,24 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
     }
 }
 
+namespace {
+std::size_t packed_bytes(std::uint64_t tiles) { return 2 * sizeof(float4) * kTile * tiles; }
+std::size_t centre_bytes(std::uint64_t tiles) { return (sizeof(double4) + sizeof(float4)) * tiles; }
+std::size_t slot_bytes(std::uint64_t slots) { return sizeof(float) * 6 * kTile * slots; }
+}  // namespace
+
+std::size_t direct_batch_bytes(std::uint64_t tiles, std::uint64_t slots) {
+    return packed_bytes(tiles) + centre_bytes(tiles) + slot_bytes(slots);
+}
+
 void direct_batch_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
                           const double* m, const std::uint32_t* order, const DirectBatchView& b,
                           double* ax, double* ay, double* az) {
     if (b.count == 0 || b.tiles == 0) return;
     const std::size_t npad = static_cast<std::size_t>(b.tiles) * kTile;
-    ctx->direct_packed.reserve(2 * sizeof(float4) * npad);
-    ctx->direct_centres.reserve((sizeof(double4) + sizeof(float4)) * b.tiles);
-    ctx->direct_partials.reserve(sizeof(float) * 6 * kTile * static_cast<std::size_t>(b.slots));
+    ctx->direct_packed.reserve(packed_bytes(b.tiles));
+    ctx->direct_centres.reserve(centre_bytes(b.tiles));
+    ctx->direct_partials.reserve(slot_bytes(b.slots));
     float4* packed = ctx->direct_packed.as<float4>();
     float4* packed_lo = packed + npad;
     double4* centres = ctx->direct_centres.as<double4>();

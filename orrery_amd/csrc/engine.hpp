@@ -10,6 +10,7 @@
 
 #include <atomic>
 #include <cstddef>
+#include <array>
 #include <cstdint>
 #include <new>
 #include <stdexcept>
@@ -68,6 +69,10 @@ extern std::atomic<std::uint64_t> g_reallocations;
 
 // A device allocation that only grows (the reference's solver pimpl rule,
 // cuda_direct_solver.cpp:78-137).
+// Bytes Buffer::reserve allocates for a request of `wanted` (a quarter more,
+// so slowly growing systems do not reallocate every step).
+std::size_t grown_bytes(std::size_t wanted);
+
 struct Buffer {
     void* ptr = nullptr;
     std::size_t bytes = 0;
@@ -101,10 +106,16 @@ struct orr_ctx {
 
     // Scratch, grown on demand.
     orr::Buffer direct_packed, direct_centres, direct_partials;
-    orr::Buffer order_codes, order_index, order_codes_alt, order_index_alt, order_hist,
-        order_bbox, order_flags;
-    orr::Buffer tree_nodes, tree_aux, tree_work, tree_counts, tree_sorted, tree_leafref;
+    orr::Buffer order_aos;  // {x, y, z, m} per particle, index order (tree path)
+    orr::Buffer order_codes, order_index, order_codes_alt, order_index_alt, order_hist, order_bbox;
+    orr::Buffer tree_work;  // one arena: per-particle, per-node and counter arrays
     orr::Buffer diag_scratch;
+    // Every scratch buffer above, for release and accounting.
+    std::array<orr::Buffer*, 12> buffers() {
+        return {&direct_packed, &direct_centres, &direct_partials, &order_aos,   &order_codes,
+                &order_index,   &order_codes_alt, &order_index_alt, &order_hist, &order_bbox,
+                &tree_work,     &diag_scratch};
+    }
 
     // Direct-kernel configuration (orr_ctx_set_direct_config).
     int direct_targets_per_lane = 1;
@@ -198,9 +209,42 @@ struct SegmentView {
 };
 
 // morton.hip, segmented: one cube per system (cubes at ctx->order_bbox[4 * s]),
-// order by (system, code, index). Systems keep their particle ranges.
+// order by (system, code, index). Systems keep their particle ranges. With
+// masses given, the encode pass also leaves each particle's {x, y, z, m} in
+// index order in ctx->order_aos (double4), so a later gather through the
+// permutation reads one 32-byte record per particle instead of four scattered
+// FP64 values.
 void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, const double* z,
-                             std::uint64_t n, const SegmentView& segments);
+                             std::uint64_t n, const SegmentView& segments,
+                             const double* m = nullptr);
+
+// Bytes the ordering stage reserves for n particles in `segments` systems
+// (`chunks` bounding-box chunks when segmented), with or without the
+// {x, y, z, m} records: the figures morton_enqueue_segments reserves with.
+struct OrderSizes {
+    std::uint32_t bbox_blocks, sort_blocks;
+    std::size_t aos, bbox, codes, index, hist, total;
+};
+OrderSizes order_sizes(std::uint64_t n, std::uint32_t segments, std::uint32_t chunks, bool aos);
+
+// tree.hip: bytes tree_evaluate_segments reserves for n particles in `segments`
+// systems at this leaf capacity (its node arrays are sized to the worst-case
+// node count, n + 21 floor(n / (cap + 1)) + 22 per system).
+std::size_t tree_bytes(std::uint64_t n, std::uint64_t leaf_capacity, bool quadrupole,
+                       std::uint32_t segments);
+
+// direct.hip: bytes direct_batch_enqueue reserves for `tiles` source tiles and
+// `slots` partial slots.
+std::size_t direct_batch_bytes(std::uint64_t tiles, std::uint64_t slots);
+
+// batch.hip: an upper bound of the device bytes one system of n particles
+// holds in a batch with this configuration (state, tables and every scratch
+// buffer after any refresh, step, autotune or diagnostics). Bounds add: a
+// batch of several systems holds at most the sum of theirs plus 4 KiB.
+std::uint64_t system_bytes_bound(std::uint64_t n, int solver, const orr_batch_config& cfg);
+
+// context.hip: device bytes held by a context's scratch buffers.
+std::size_t ctx_device_bytes(const orr_ctx* ctx);
 
 // morton.hip: bounding cube, codes and stable radix sort from identity order.
 // Leaves ascending codes and permutation in ctx->order_codes / order_index

@@ -207,24 +207,28 @@ __device__ inline std::uint32_t to_grid(double coordinate, double origin, double
 
 __global__ __launch_bounds__(kBlock) void encode(const double* __restrict__ x,
                                                  const double* __restrict__ y,
-                                                 const double* __restrict__ z, std::uint64_t n,
+                                                 const double* __restrict__ z,
+                                                 const double* __restrict__ m, std::uint64_t n,
                                                  const double* __restrict__ cubes,
                                                  const std::uint32_t* __restrict__ seg_of,
                                                  std::uint64_t* __restrict__ codes,
-                                                 std::uint32_t* __restrict__ index) {
+                                                 std::uint32_t* __restrict__ index,
+                                                 double4* __restrict__ aos) {
     const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     const double* cube = cubes + (seg_of != nullptr ? 4 * static_cast<std::size_t>(seg_of[i]) : 0);
     const double size = cube[3];
+    const double px = x[i], py = y[i], pz = z[i];
     std::uint64_t code = 0;
     if (size > 0) {
         const double scale = static_cast<double>(kGrid) / size;
-        code = (spread_bits(to_grid(x[i], cube[0], scale)) << 2) |
-               (spread_bits(to_grid(y[i], cube[1], scale)) << 1) |
-               spread_bits(to_grid(z[i], cube[2], scale));
+        code = (spread_bits(to_grid(px, cube[0], scale)) << 2) |
+               (spread_bits(to_grid(py, cube[1], scale)) << 1) |
+               spread_bits(to_grid(pz, cube[2], scale));
     }
     codes[i] = code;
     index[i] = static_cast<std::uint32_t>(i);
+    if (aos != nullptr) aos[i] = make_double4(px, py, pz, m[i]);
 }
 
 // ---- radix sort -------------------------------------------------------------
@@ -306,27 +310,42 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __r
 
 }  // namespace
 
+OrderSizes order_sizes(std::uint64_t n, std::uint32_t segments, std::uint32_t chunks, bool aos) {
+    OrderSizes z{};
+    z.bbox_blocks = segments > 1
+        ? chunks
+        : static_cast<std::uint32_t>(std::min<std::uint64_t>(1024, (n + kBlock - 1) / kBlock));
+    z.sort_blocks = static_cast<std::uint32_t>((n + kKeysPerBlock - 1) / kKeysPerBlock);
+    z.aos = aos ? sizeof(double4) * n : 0;
+    z.bbox = sizeof(double) * (6 * static_cast<std::size_t>(z.bbox_blocks) + 4 * static_cast<std::size_t>(segments));
+    z.codes = sizeof(std::uint64_t) * n;
+    z.index = sizeof(std::uint32_t) * n;
+    const std::size_t hist_words = 256 * static_cast<std::size_t>(z.sort_blocks);
+    z.hist = sizeof(std::uint32_t) * (hist_words + scan_scratch_words(hist_words));
+    z.total = z.aos + z.bbox + 2 * z.codes + 2 * z.index + z.hist;
+    return z;
+}
+
 void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
                     std::uint64_t n) {
     morton_enqueue_segments(ctx, x, y, z, n, SegmentView{});
 }
 
 void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, const double* z,
-                             std::uint64_t n, const SegmentView& segs) {
+                             std::uint64_t n, const SegmentView& segs, const double* m) {
     if (n == 0) return;
     if (n > (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "morton: more than 2^31 particles");
     const bool segmented = segs.count > 1;
-    const std::uint32_t bbox_blocks = segmented
-        ? segs.chunk_count
-        : static_cast<std::uint32_t>(std::min<std::uint64_t>(1024, (n + kBlock - 1) / kBlock));
-    const std::uint32_t sort_blocks = static_cast<std::uint32_t>((n + kKeysPerBlock - 1) / kKeysPerBlock);
-    ctx->order_bbox.reserve(sizeof(double) * (6 * bbox_blocks + 4 * static_cast<std::size_t>(segs.count)));
-    ctx->order_codes.reserve(sizeof(std::uint64_t) * n);
-    ctx->order_codes_alt.reserve(sizeof(std::uint64_t) * n);
-    ctx->order_index.reserve(sizeof(std::uint32_t) * n);
-    ctx->order_index_alt.reserve(sizeof(std::uint32_t) * n);
-    ctx->order_hist.reserve(sizeof(std::uint32_t) *
-                            (256 * sort_blocks + scan_scratch_words(256 * sort_blocks)));
+    const OrderSizes sz = order_sizes(n, segs.count, segs.chunk_count, m != nullptr);
+    const std::uint32_t bbox_blocks = sz.bbox_blocks;
+    const std::uint32_t sort_blocks = sz.sort_blocks;
+    ctx->order_bbox.reserve(sz.bbox);
+    if (m != nullptr) ctx->order_aos.reserve(sz.aos);
+    ctx->order_codes.reserve(sz.codes);
+    ctx->order_codes_alt.reserve(sz.codes);
+    ctx->order_index.reserve(sz.index);
+    ctx->order_index_alt.reserve(sz.index);
+    ctx->order_hist.reserve(sz.hist);
     double* cube = ctx->order_bbox.as<double>();  // cubes[4 * segment]
     double* partial = cube + 4 * static_cast<std::size_t>(segs.count);
     {
@@ -353,7 +372,8 @@ void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, con
     {
         PhaseTimer t(ctx, kEncode);
         hipLaunchKernelGGL(encode, dim3(static_cast<std::uint32_t>((n + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, ctx->stream, x, y, z, n, cube, seg_of, keys[0], vals[0]);
+                           dim3(kBlock), 0, ctx->stream, x, y, z, m, n, cube, seg_of, keys[0], vals[0],
+                           m != nullptr ? ctx->order_aos.as<double4>() : nullptr);
         ORR_LAUNCHED("encode");
     }
     {

@@ -39,11 +39,12 @@
 
 namespace {
 
-// HBM held per admitted particle: the FP64 state (80 B), the scratch of the
-// ordering, tree and direct tables (under 200 B) and of the diagnostics (up to
-// 8 term and 16 column-chunk FP64 values, 192 B), plus a fixed 1 MiB.
-constexpr std::uint64_t kBytesPerParticle = 480;
-constexpr std::uint64_t kBytesPerRequest = 1u << 20;
+// HBM charged per admitted request: the upper bound of what the request holds
+// as one system of a batch (orr_batch_bytes_bound: state, tables, ordering,
+// tree or direct scratch, diagnostics), a second copy of its state and
+// diagnostics scratch for the moment a rebuild holds the old and the new batch
+// at once, and a fixed 4 MiB for allocator rounding and small tables.
+constexpr std::uint64_t kBytesPerRequest = 4u << 20;
 constexpr double kMaxBudget = 288e9;
 
 enum : int { kQueued = 0, kRunning = 1, kDone = 2 };
@@ -125,6 +126,24 @@ GroupKey key_of(const orr_server* s, std::uint64_t id, const Request& r) {
                     alone ? id : 0};
 }
 
+orr_batch_config batch_config(const orr_server* s, std::int32_t integrator) {
+    orr_batch_config bc{};
+    bc.integrator = integrator;
+    bc.quadrupole = s->cfg.quadrupole;
+    bc.opening_angle = s->cfg.opening_angle;
+    bc.leaf_capacity = s->cfg.leaf_capacity;
+    bc.tiles_per_item = s->cfg.tiles_per_item;
+    bc.direct_below = s->cfg.direct_below;
+    bc.use_graph = s->cfg.use_graph;
+    return bc;
+}
+
+std::uint64_t request_bytes(const orr_server* s, std::uint64_t count) {
+    const orr_batch_config bc = batch_config(s, 0);
+    return orr::system_bytes_bound(count, -1, bc) + orr::grown_bytes(80 * count) +
+           orr::grown_bytes(orr::diagnostics_scratch_bytes(count)) + kBytesPerRequest;
+}
+
 void rebuild(orr_server* s, const GroupKey& key, Group& g) {
     orr_batch* old = g.batch;
     g.batch = nullptr;
@@ -137,14 +156,7 @@ void rebuild(orr_server* s, const GroupKey& key, Group& g) {
                                            q.gravitational_constant, q.timestep});
             particles += q.count;
         }
-        orr_batch_config bc{};
-        bc.integrator = key.integrator;
-        bc.quadrupole = s->cfg.quadrupole;
-        bc.opening_angle = s->cfg.opening_angle;
-        bc.leaf_capacity = s->cfg.leaf_capacity;
-        bc.tiles_per_item = s->cfg.tiles_per_item;
-        bc.direct_below = s->cfg.direct_below;
-        bc.use_graph = s->cfg.use_graph;
+        const orr_batch_config bc = batch_config(s, key.integrator);
         orr_batch* nb = nullptr;
         call(orr_batch_create(s->ctx, sys.data(), static_cast<std::uint32_t>(sys.size()), &bc, &nb));
         try {
@@ -165,9 +177,6 @@ void rebuild(orr_server* s, const GroupKey& key, Group& g) {
                     call(orr_batch_copy_system(nb, k, old, g.slot.at(id)));
                 }
             }
-            // Accelerations for the current positions (a continuing request's
-            // come out as the bits it already had), enqueued before the steps.
-            orr::batch_refresh_enqueue(nb);
             g.slot = std::move(slot);
         } catch (...) {
             orr_batch_destroy(nb);
@@ -175,6 +184,16 @@ void rebuild(orr_server* s, const GroupKey& key, Group& g) {
             throw;
         }
         g.batch = nb;
+        // The copies are done (they synchronise): the old batch goes before the
+        // new one allocates its scratch, so the two never hold scratch at once.
+        if (old != nullptr) {
+            orr_batch* gone = old;
+            old = nullptr;
+            call(orr_batch_destroy(gone));
+        }
+        // Accelerations for the current positions (a continuing request's
+        // come out as the bits it already had), enqueued before the steps.
+        orr::batch_refresh_enqueue(nb);
         if (g.particles > 0 && g.ms_per_step > 0) {
             g.ms_per_step *= static_cast<double>(particles) / static_cast<double>(g.particles);
         }
@@ -234,6 +253,13 @@ void retire(orr_server* s, Group& g) {
 
 void round(orr_server* s) {
     const double start = s->now();
+    // Groups that lost members last round still hold their memory in their
+    // old batches: rebuild them before admitting into the budget they freed.
+    if (!s->queue.empty()) {
+        for (auto& [key, g] : s->groups) {
+            if (g.dirty) rebuild(s, key, g);
+        }
+    }
     admit(s);
     for (auto& [key, g] : s->groups) {
         if (g.dirty) rebuild(s, key, g);
@@ -368,7 +394,7 @@ int orr_server_submit(orr_server* s, const orr_request* q, uint64_t* id) {
         Request r;
         r.spec = *q;
         r.spec.x = r.spec.y = r.spec.z = r.spec.vx = r.spec.vy = r.spec.vz = r.spec.m = nullptr;
-        r.bytes = q->count * kBytesPerParticle + kBytesPerRequest;
+        r.bytes = request_bytes(s, q->count);
         if (r.bytes > s->budget) orr::fail(ORR_ERR_OUT_OF_MEMORY, "request larger than the memory budget");
         for (int a = 0; a < 7; ++a) r.host[a].assign(src[a], src[a] + q->count);
         r.submitted = s->now();
@@ -402,6 +428,7 @@ int orr_server_status(orr_server* s, uint64_t id, orr_request_status* out) {
         out->submitted_s = r.submitted;
         out->admitted_s = r.admitted;
         out->completed_s = r.completed;
+        out->charged_bytes = r.bytes;
         // Start diagnostics exist once a round has run since admission, end
         // diagnostics once the request is done.
         if (r.diag != nullptr && r.state != kQueued) std::memcpy(out->diagnostics_start, r.diag, 8 * sizeof(double));

@@ -290,17 +290,18 @@ __global__ __launch_bounds__(kBlock) void tree_node_fill(
 // Sorted FP64 particles for the moments; FP32 walk data: position relative to
 // the root centre, and offset from the particle's leaf centre.
 __global__ __launch_bounds__(kBlock) void tree_gather(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ m, const std::uint32_t* __restrict__ index, std::uint32_t n,
+    const double4* __restrict__ aos, const std::uint32_t* __restrict__ index, std::uint32_t n,
     const double* __restrict__ cubes, const std::uint32_t* __restrict__ seg_of,
     const std::uint32_t* __restrict__ leaf_of,
     const double4* __restrict__ node_geo, double4* __restrict__ sorted,
     float4* __restrict__ root_rel, float4* __restrict__ leaf_rel) {
     const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const std::uint32_t src = index[i];
-    const double px = x[src], py = y[src], pz = z[src], pm = m[src];
-    sorted[i] = make_double4(px, py, pz, pm);
+    // One 32-byte record per particle ({x, y, z, m} written in index order by
+    // the encode pass) rather than four scattered FP64 loads.
+    const double4 p = aos[index[i]];
+    const double px = p.x, py = p.y, pz = p.z, pm = p.w;
+    sorted[i] = p;
     const double* cube = cubes + (seg_of != nullptr ? 4 * static_cast<std::size_t>(seg_of[i]) : 0);
     const double half = cube[3] / 2;
     const double c0x = cube[0] + half, c0y = cube[1] + half, c0z = cube[2] + half;
@@ -823,8 +824,8 @@ std::size_t node_bound(std::uint64_t n, std::uint64_t cap) {
     return static_cast<std::size_t>(std::min(by_levels, by_chains));
 }
 
-Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole, std::uint32_t segments,
-              bool allocate) {
+// Bytes of one tree arena for n particles, `nodes` node slots and `segments` systems.
+std::size_t layout_bytes(std::uint64_t n, std::size_t nodes, bool quadrupole, std::uint32_t segments) {
     // Every carved array is rounded up to 256 bytes: 21 arrays of padding.
     const std::size_t per_particle = 2 * n + 3 * 4 * n + 32 * n + 2 * 16 * n +
                                      4 * scan_scratch_words(n);
@@ -833,7 +834,12 @@ Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole,
     const std::size_t counters = 16 + 8 * 4 * static_cast<std::size_t>(segments) +
                                  8 * ((n + kBlock - 1) / kBlock) +
                                  32 * (kBlock / 64) * ((n + kBlock - 1) / kBlock) + 512;
-    const std::size_t bytes = per_particle + per_node + counters + 21 * 256 + 64;
+    return per_particle + per_node + counters + 21 * 256 + 64;
+}
+
+Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole, std::uint32_t segments,
+              bool allocate) {
+    const std::size_t bytes = layout_bytes(n, nodes, quadrupole, segments);
     if (allocate) ctx->tree_work.reserve(bytes);
     char* p = static_cast<char*>(ctx->tree_work.ptr);
     Layout L{};
@@ -866,6 +872,15 @@ Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole,
 
 }  // namespace
 
+std::size_t tree_bytes(std::uint64_t n, std::uint64_t leaf_capacity, bool quadrupole,
+                       std::uint32_t segments) {
+    if (n == 0) return 0;
+    // As tree_evaluate_segments sizes its arena.
+    const std::uint64_t cap = std::min<std::uint64_t>(std::max<std::uint64_t>(leaf_capacity, 1), n);
+    const std::size_t bound = node_bound(n, cap) + 22 * static_cast<std::size_t>(segments);
+    return layout_bytes(n, bound, quadrupole, segments);
+}
+
 void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) {
     tree_evaluate_segments(ctx, a, SegmentView{}, nullptr, stats);
 }
@@ -895,7 +910,7 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
     const std::uint32_t* seg_start = segs.count > 1 ? segs.start : nullptr;
     unsigned long long* counts64 = seg_stats != nullptr ? seg_stats : L.c64;
 
-    morton_enqueue_segments(ctx, a.x, a.y, a.z, n, segs);
+    morton_enqueue_segments(ctx, a.x, a.y, a.z, n, segs, a.m);
     const std::uint64_t* codes = ctx->order_codes.as<std::uint64_t>();
     const std::uint32_t* index = ctx->order_index.as<std::uint32_t>();
     const double* cube = ctx->order_bbox.as<double>();
@@ -925,9 +940,9 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
     }
     {
         PhaseTimer t(ctx, kGather);
-        hipLaunchKernelGGL(tree_gather, dim3(blocks), dim3(kBlock), 0, ctx->stream, a.x, a.y, a.z,
-                           a.m, index, n32, cube, seg_of, L.leaf_of, L.geo, L.sorted, L.root_rel,
-                           L.leaf_rel);
+        hipLaunchKernelGGL(tree_gather, dim3(blocks), dim3(kBlock), 0, ctx->stream,
+                           ctx->order_aos.as<double4>(), index, n32, cube, seg_of, L.leaf_of, L.geo,
+                           L.sorted, L.root_rel, L.leaf_rel);
         ORR_LAUNCHED("tree_gather");
     }
     {

@@ -608,6 +608,16 @@ class DeviceBatch:
         C.check(C.load().orr_batch_info_get(self.handle, ctypes.byref(out)))
         return {name: getattr(out, name) for name, _ in C.OrrBatchInfo._fields_}
 
+    @staticmethod
+    def bytes_bound(systems, config: BatchConfig) -> int:
+        """Upper bound of the device bytes a batch of these systems holds (orr_batch_bytes_bound)."""
+        systems = list(systems)
+        arr = (C.OrrBatchSystem * max(1, len(systems)))(*(s.native() for s in systems))
+        cfg = config.native()
+        out = ctypes.c_uint64()
+        C.check(C.load().orr_batch_bytes_bound(arr, len(systems), ctypes.byref(cfg), ctypes.byref(out)))
+        return int(out.value)
+
     def upload(self, system: int, x, y, z, vx, vy, vz, m) -> None:
         arrs = [DeviceSystem._p(a) for a in (x, y, z, vx, vy, vz, m)]
         ptrs = [None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in arrs]
@@ -722,7 +732,8 @@ class Server:
                         angular_momentum=list(a[5:8]))
         return dict(state=self.STATES[st.state], steps_done=st.steps_done, submitted=st.submitted_s,
                     admitted=st.admitted_s, completed=st.completed_s,
-                    start=diag(list(st.diagnostics_start)), end=diag(list(st.diagnostics_end)))
+                    start=diag(list(st.diagnostics_start)), end=diag(list(st.diagnostics_end)),
+                    charged_bytes=st.charged_bytes)
 
     def result(self, rid: int) -> dict:
         n = self.counts[rid]

@@ -252,3 +252,29 @@ def test_edge_systems_in_one_batch(ctx):
     b.step(3)
     assert np.all(np.isfinite(b.download(2)["x"]))
     b.close()
+
+
+@pytest.mark.parametrize("quadrupole,leaf,items", [(False, 16, 16), (True, 0, 0), (True, 8, 4)])
+def test_device_bytes_within_bound(ctx, quadrupole, leaf, items):
+    # orr_batch_bytes_bound is what the serving loop charges its budget with:
+    # after refreshes, graph steps, autotuning and diagnostics the batch holds
+    # no more device memory than it.
+    ns = [300, 4095, 1, 5000, 20000, 90000, 2600, 64, 7000]
+    cfg = E.BatchConfig(integrator="yoshida4", opening_angle=0.5, leaf_capacity=leaf,
+                        quadrupole=quadrupole, tiles_per_item=items, use_graph=True)
+    systems = [E.BatchSystem(n, EPS, 1e-3) for n in ns]
+    bound = E.DeviceBatch.bytes_bound(systems, cfg)
+    b, _ = make_batch(ctx, ns, cfg)
+    b.refresh()
+    if leaf == 0 or items == 0:
+        b.autotune()
+    b.step(3)
+    for k in (0, 5, 6):
+        b.diagnostics(k)
+    held = b.info()["device_bytes"]
+    print(f"held {held / 1e6:.1f} MB, bound {bound / 1e6:.1f} MB")
+    assert 0 < held <= bound, (held, bound)
+    # Bounds of disjoint sets of systems add.
+    parts = sum(E.DeviceBatch.bytes_bound([s], cfg) for s in systems)
+    assert bound <= parts
+    b.close()

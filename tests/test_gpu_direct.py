@@ -197,3 +197,15 @@ def test_device_resident_evaluate_matches_host_staged(ctx):
     s.evaluate_device(*(a.ptr for a in d), *(a.ptr for a in out), 3000)
     got = [a.to_host() for a in out]
     assert np.array_equal(got[0], ax) and np.array_equal(got[1], ay) and np.array_equal(got[2], az)
+
+
+def test_direct_config_round_trip(ctx):
+    assert ctx.direct_config() == {"targets_per_lane": 1, "refine": 0, "block_target": 32768}
+    try:
+        ctx.set_direct_config(2, 1, 4096)
+        ctx.set_direct_config(0, -1, 0)  # keeps every field
+        assert ctx.direct_config() == {"targets_per_lane": 2, "refine": 1, "block_target": 4096}
+        with pytest.raises(E.solvers.C.EngineError):
+            ctx.set_direct_config(3, 0, 0)
+    finally:
+        ctx.set_direct_config(1, 0, 32768)

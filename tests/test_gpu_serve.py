@@ -92,8 +92,19 @@ def test_matches_the_fp64_integrator_and_its_energy_drift(ctx):
 
 def test_memory_budget_bounds_admission(ctx):
     reqs = requests()
-    # Room for about two of these requests at a time (480 B per particle + 1 MiB each).
-    budget = 2 * (1 << 20) + 480 * 14000
+    # Each request's charge (the batch bound of its system plus a rebuild
+    # margin), read off a server with room for all; then a budget with room
+    # for about two of them at a time.
+    probe = E.Server(ctx, config())
+    charges = [probe.status(probe.submit(w, integ, EPS, 1e-3, steps))["charged_bytes"]
+               for w, integ, steps in reqs]
+    probe.close()
+    for (w, _, _), c in zip(reqs, charges):
+        bound = E.DeviceBatch.bytes_bound([E.BatchSystem(w.n, EPS, 1e-3)],
+                                          E.BatchConfig(opening_angle=0.5, leaf_capacity=16,
+                                                        tiles_per_item=16, direct_below=4096))
+        assert c > bound
+    budget = sorted(charges)[-1] + sorted(charges)[-2]
     s = E.Server(ctx, config(memory_budget=budget))
     ids = [s.submit(w, integ, EPS, 1e-3, steps) for w, integ, steps in reqs]
     s.round()

@@ -252,9 +252,12 @@ def test_two_targets_per_lane_is_bitwise(ctx, quadrupole):
     # same DFS order with the same arithmetic as a wave of 64.
     p = oracle.disc_collision(30000, 4)
     results = []
+    assert ctx.tree_config()["targets_per_lane"] == -1  # by size, the default
     try:
         for r in (1, 2):
             ctx.set_tree_config(targets_per_lane=r)
+            ctx.set_tree_config(targets_per_lane=0)  # keeps
+            assert ctx.tree_config()["targets_per_lane"] == r
             s = E.HipTreeSolver(E.TreeParameters(0.5, 16, quadrupole=quadrupole), E.Softening(0.01),
                                 context=ctx)
             acc = (np.zeros(p.n), np.zeros(p.n), np.zeros(p.n))
