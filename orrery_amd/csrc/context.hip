@@ -70,8 +70,10 @@ void free_context(orr_ctx* ctx) noexcept {
     if (ctx == nullptr) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    drop_tree_graph(ctx);
     for (Buffer* b : ctx->buffers()) b->release();
     for (auto& p : ctx->pending) {
+        if (!p.recycle) continue;  // a graph's, destroyed with it
         (void)hipEventDestroy(p.start);
         (void)hipEventDestroy(p.stop);
     }
@@ -113,8 +115,10 @@ void orr_ctx::sync() {
         ORR_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
         totals.milliseconds[p.phase] += ms;
         totals.launches[p.phase] += 1;
-        free_events.push_back(p.start);
-        free_events.push_back(p.stop);
+        if (p.recycle) {
+            free_events.push_back(p.start);
+            free_events.push_back(p.stop);
+        }
     }
     pending.clear();
 }

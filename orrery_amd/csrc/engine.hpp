@@ -137,10 +137,24 @@ struct orr_ctx {
     struct Pending {
         int phase;
         hipEvent_t start, stop;
+        bool recycle = true;  // false: the events belong to a captured graph
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> free_events;
     orr_timings totals{};
+
+    // A captured single-system tree evaluation (tree.hip), replayed while its
+    // arguments, the walk setting and every buffer stay those it was captured
+    // with (and phase timing is off): the evaluation is some 60 short
+    // launches, and replaying them removes the host's launch gaps between them.
+    struct TreeGraph {
+        orr_tree_args args{};
+        int walk = 0;
+        std::uint64_t epoch = 0;
+        bool seen = false;            // this key has run once eagerly
+        hipGraphExec_t exec = nullptr;
+        std::vector<Pending> timers;  // phase events recorded while capturing (none: timing is off)
+    } tree_graph;
 
     hipEvent_t take_event();
     void begin(int phase, hipEvent_t& start);
@@ -242,6 +256,9 @@ std::size_t direct_batch_bytes(std::uint64_t tiles, std::uint64_t slots);
 // buffer after any refresh, step, autotune or diagnostics). Bounds add: a
 // batch of several systems holds at most the sum of theirs plus 4 KiB.
 std::uint64_t system_bytes_bound(std::uint64_t n, int solver, const orr_batch_config& cfg);
+
+// tree.hip: releases the context's captured tree evaluation, if any.
+void drop_tree_graph(orr_ctx* ctx) noexcept;
 
 // context.hip: device bytes held by a context's scratch buffers.
 std::size_t ctx_device_bytes(const orr_ctx* ctx);

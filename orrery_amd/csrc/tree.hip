@@ -54,6 +54,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr unsigned kLevels = 21;
 constexpr std::uint8_t kNoBoundary = 22;
+constexpr int kMaxChildren = 8;                 // octants
+constexpr std::uint32_t kNoChild = 0xffffffffu;
 
 struct WalkNode {
     float4 a;  // centre of mass relative to the root centre (FP32), mass
@@ -417,13 +419,32 @@ __global__ __launch_bounds__(kBlock) void leaf_moments(
     }
 }
 
+// The children of every internal node in DFS order (the next chain from
+// node + 1 to the node's own next), written once for all levels in one launch
+// so that the per-level moment passes load them side by side instead of
+// following the chain one dependent load at a time.
+__global__ __launch_bounds__(kBlock) void tree_children(
+    const std::uint32_t* __restrict__ counters, const std::uint8_t* __restrict__ node_level,
+    const std::uint8_t* __restrict__ leaf_level_of_first, const std::uint32_t* __restrict__ node_begin,
+    const std::uint32_t* __restrict__ node_next, std::uint32_t* __restrict__ children) {
+    const std::uint32_t M = counters[0];
+    for (std::uint32_t node = grid_stride_start(); node < M; node += grid_stride_step()) {
+        if (node_level[node] == leaf_level_of_first[node_begin[node]]) continue;  // a leaf
+        const std::uint32_t end = node_next[node];
+        std::uint32_t* out = children + static_cast<std::size_t>(kMaxChildren) * node;
+        int k = 0;
+        for (std::uint32_t c = node + 1; c < end && k < kMaxChildren; c = max(node_next[c], c + 1)) out[k++] = c;
+        for (; k < kMaxChildren; ++k) out[k] = kNoChild;
+    }
+}
+
 // octree.cpp:142-185 (internal): children in DFS order via the next chain;
 // quadrupoles by the parallel-axis theorem, children's moments first.
 __global__ __launch_bounds__(kBlock) void internal_moments(
     unsigned level, const std::uint32_t* __restrict__ counters,
     const std::uint8_t* __restrict__ node_level, const std::uint8_t* __restrict__ leaf_level_of_first,
     const std::uint32_t* __restrict__ node_begin, const std::uint32_t* __restrict__ node_count,
-    const std::uint32_t* __restrict__ node_next,
+    const std::uint32_t* __restrict__ node_next, const std::uint32_t* __restrict__ children,
     const double4* __restrict__ node_geo, double theta, const double* __restrict__ cubes,
     const std::uint32_t* __restrict__ seg_of, const std::uint32_t* __restrict__ seg_start,
     std::uint32_t n, double4* __restrict__ moment,
@@ -436,11 +457,23 @@ __global__ __launch_bounds__(kBlock) void internal_moments(
         const std::uint32_t seg = seg_of != nullptr ? seg_of[b] : 0u;
         const double* cube = cubes + 4 * static_cast<std::size_t>(seg);
         const std::uint32_t system_end = seg_start != nullptr ? seg_start[seg + 1] : n;
-        const std::uint32_t end = node_next[node];
+        // The children's indices, then their moments, loaded side by side; the
+        // sums below still run in DFS order.
+        const std::uint32_t* list = children + static_cast<std::size_t>(kMaxChildren) * node;
+        std::uint32_t child[kMaxChildren];
+        double4 cms[kMaxChildren];
+#pragma unroll
+        for (int k = 0; k < kMaxChildren; ++k) child[k] = list[k];
+#pragma unroll
+        for (int k = 0; k < kMaxChildren; ++k) {
+            if (child[k] != kNoChild) cms[k] = moment[child[k]];
+        }
         double mass = 0, wx = 0, wy = 0, wz = 0, mx = 0, my = 0, mz = 0;
-        std::uint32_t children = 0;
-        for (std::uint32_t c = node + 1; c < end; c = max(node_next[c], c + 1)) {
-            const double4 cm = moment[c];
+        std::uint32_t count = 0;
+#pragma unroll
+        for (int k = 0; k < kMaxChildren; ++k) {
+            if (child[k] == kNoChild) continue;
+            const double4 cm = cms[k];
             mass += cm.w;
             wx += cm.x * cm.w;
             wy += cm.y * cm.w;
@@ -448,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void internal_moments(
             mx += cm.x;
             my += cm.y;
             mz += cm.z;
-            ++children;
+            ++count;
         }
         double cx, cy, cz;
         if (mass > 0) {
@@ -456,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void internal_moments(
             cy = wy / mass;
             cz = wz / mass;
         } else {
-            const double k = static_cast<double>(children == 0 ? 1 : children);
+            const double k = static_cast<double>(count == 0 ? 1 : count);
             cx = mx / k;
             cy = my / k;
             cz = mz / k;
@@ -465,10 +498,12 @@ __global__ __launch_bounds__(kBlock) void internal_moments(
                     walk_escape(node_next[node], b + node_count[node], system_end), moment, walk);
         if (quad64 != nullptr) {
             double q[6] = {0, 0, 0, 0, 0, 0};
-            for (std::uint32_t c = node + 1; c < end; c = max(node_next[c], c + 1)) {
-                const double* cq = quad64 + 6 * static_cast<std::size_t>(c);
-                for (int k = 0; k < 6; ++k) q[k] += cq[k];
-                const double4 cm = moment[c];
+#pragma unroll
+            for (int k = 0; k < kMaxChildren; ++k) {
+                if (child[k] == kNoChild) continue;
+                const double* cq = quad64 + 6 * static_cast<std::size_t>(child[k]);
+                for (int j = 0; j < 6; ++j) q[j] += cq[j];
+                const double4 cm = cms[k];
                 add_displaced(q, cm.w, cm.x - cx, cm.y - cy, cm.z - cz);
             }
             store_quad(node, q, quad64, quadf);
@@ -804,6 +839,7 @@ struct Layout {
     std::uint32_t* count;
     std::uint32_t* next;
     std::uint8_t* level;
+    std::uint32_t* children;  // 8 per node: an internal node's children in DFS order, then kNoChild
     double4* geo;
     double4* moment;
     WalkNode* walk;
@@ -826,15 +862,15 @@ std::size_t node_bound(std::uint64_t n, std::uint64_t cap) {
 
 // Bytes of one tree arena for n particles, `nodes` node slots and `segments` systems.
 std::size_t layout_bytes(std::uint64_t n, std::size_t nodes, bool quadrupole, std::uint32_t segments) {
-    // Every carved array is rounded up to 256 bytes: 21 arrays of padding.
+    // Every carved array is rounded up to 256 bytes: 22 arrays of padding.
     const std::size_t per_particle = 2 * n + 3 * 4 * n + 32 * n + 2 * 16 * n +
                                      4 * scan_scratch_words(n);
     const std::size_t per_quad = quadrupole ? 6 * sizeof(double) + sizeof(QuadF) : 0;
-    const std::size_t per_node = nodes * (4 * 3 + 1 + 32 + 32 + sizeof(WalkNode) + per_quad);
+    const std::size_t per_node = nodes * (4 * 3 + 1 + 4 * kMaxChildren + 32 + 32 + sizeof(WalkNode) + per_quad);
     const std::size_t counters = 16 + 8 * 4 * static_cast<std::size_t>(segments) +
                                  8 * ((n + kBlock - 1) / kBlock) +
                                  32 * (kBlock / 64) * ((n + kBlock - 1) / kBlock) + 512;
-    return per_particle + per_node + counters + 21 * 256 + 64;
+    return per_particle + per_node + counters + 22 * 256 + 64;
 }
 
 Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole, std::uint32_t segments,
@@ -856,6 +892,7 @@ Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole,
     L.count = carve<std::uint32_t>(p, nodes);
     L.next = carve<std::uint32_t>(p, nodes);
     L.level = carve<std::uint8_t>(p, nodes);
+    L.children = carve<std::uint32_t>(p, kMaxChildren * nodes);
     L.geo = carve<double4>(p, nodes);
     L.moment = carve<double4>(p, nodes);
     L.walk = carve<WalkNode>(p, nodes);
@@ -881,15 +918,13 @@ std::size_t tree_bytes(std::uint64_t n, std::uint64_t leaf_capacity, bool quadru
     return layout_bytes(n, bound, quadrupole, segments);
 }
 
-void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) {
-    tree_evaluate_segments(ctx, a, SegmentView{}, nullptr, stats);
-}
+namespace {
 
-void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentView& segs,
-                            unsigned long long* seg_stats, orr_tree_stats* stats) {
+// Every launch of one evaluation on ctx->stream, no host synchronisation: the
+// part a graph captures.
+void tree_enqueue(orr_ctx* ctx, const orr_tree_args& a, const SegmentView& segs,
+                  unsigned long long* seg_stats) {
     const std::uint64_t n = a.count;
-    if (stats != nullptr) std::memset(stats, 0, sizeof(*stats));
-    ctx->last_tree = orr_tree_stats{};
     ctx->tree_last_n = n;
     ctx->tree_last_segments = segs.count;
     if (n == 0) return;
@@ -951,11 +986,14 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
                            L.level, L.leaf_level, L.begin, L.count, L.next, L.geo, L.sorted, theta,
                            cube, seg_of, seg_start, n32, L.moment, L.walk, L.quad64, L.quadf);
         ORR_LAUNCHED("leaf_moments");
+        hipLaunchKernelGGL(tree_children, dim3(node_blocks), dim3(kBlock), 0, ctx->stream, L.c32,
+                           L.level, L.leaf_level, L.begin, L.next, L.children);
+        ORR_LAUNCHED("tree_children");
         for (int level = static_cast<int>(kLevels) - 1; level >= 0; --level) {
             hipLaunchKernelGGL(internal_moments, dim3(node_blocks), dim3(kBlock), 0, ctx->stream,
                                static_cast<unsigned>(level), L.c32, L.level, L.leaf_level, L.begin,
-                               L.count, L.next, L.geo, theta, cube, seg_of, seg_start, n32,
-                               L.moment, L.walk, L.quad64, L.quadf);
+                               L.count, L.next, L.children, L.geo, theta, cube, seg_of, seg_start,
+                               n32, L.moment, L.walk, L.quad64, L.quadf);
             ORR_LAUNCHED("internal_moments");
         }
     }
@@ -988,24 +1026,121 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
                            L.wave_stats, walk_blocks * (kBlock / 64), 64u * R, seg_start, counts64);
         ORR_LAUNCHED("walk_counts");
     }
-    if (stats != nullptr) {
-        std::uint32_t c32[4];
-        std::vector<unsigned long long> c64(4 * static_cast<std::size_t>(segs.count));
-        ORR_HIP(hipMemcpyAsync(c32, L.c32, sizeof(c32), hipMemcpyDeviceToHost, ctx->stream));
-        ORR_HIP(hipMemcpyAsync(c64.data(), counts64, c64.size() * sizeof(unsigned long long),
-                               hipMemcpyDeviceToHost, ctx->stream));
-        ctx->sync();
-        stats->node_count = c32[0];
-        stats->leaf_count = c32[1];
-        stats->depth = c32[2];
-        for (std::uint32_t sgm = 0; sgm < segs.count; ++sgm) {
-            stats->particle_particle += c64[4 * sgm];
-            stats->particle_cell += c64[4 * sgm + 1];
-            stats->node_visits += c64[4 * sgm + 2];
-            stats->wave_steps += c64[4 * sgm + 3];
-        }
-        ctx->last_tree = *stats;
+}
+
+// The counts and tree summary of the evaluation last enqueued on ctx (syncs).
+void tree_read_stats(orr_ctx* ctx, unsigned long long* seg_stats, orr_tree_stats* stats) {
+    const std::uint64_t n = ctx->tree_last_n;
+    const std::uint32_t segments = ctx->tree_last_segments;
+    if (n == 0) return;
+    const Layout L = layout(ctx, n, ctx->tree_last_bound, ctx->tree_last_quadrupole, segments, false);
+    const unsigned long long* counts64 = seg_stats != nullptr ? seg_stats : L.c64;
+    std::uint32_t c32[4];
+    std::vector<unsigned long long> c64(4 * static_cast<std::size_t>(segments));
+    ORR_HIP(hipMemcpyAsync(c32, L.c32, sizeof(c32), hipMemcpyDeviceToHost, ctx->stream));
+    ORR_HIP(hipMemcpyAsync(c64.data(), counts64, c64.size() * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    stats->node_count = c32[0];
+    stats->leaf_count = c32[1];
+    stats->depth = c32[2];
+    for (std::uint32_t sgm = 0; sgm < segments; ++sgm) {
+        stats->particle_particle += c64[4 * sgm];
+        stats->particle_cell += c64[4 * sgm + 1];
+        stats->node_visits += c64[4 * sgm + 2];
+        stats->wave_steps += c64[4 * sgm + 3];
     }
+    ctx->last_tree = *stats;
+}
+
+bool same_args(const orr_tree_args& p, const orr_tree_args& q) {
+    return p.x == q.x && p.y == q.y && p.z == q.z && p.m == q.m && p.ax == q.ax && p.ay == q.ay &&
+           p.az == q.az && p.count == q.count && p.softening_squared == q.softening_squared &&
+           p.gravitational_constant == q.gravitational_constant && p.opening_angle == q.opening_angle &&
+           p.leaf_capacity == q.leaf_capacity && p.quadrupole == q.quadrupole;
+}
+
+}  // namespace
+
+void drop_tree_graph(orr_ctx* ctx) noexcept {
+    orr_ctx::TreeGraph& g = ctx->tree_graph;
+    if (g.exec != nullptr) (void)hipGraphExecDestroy(g.exec);
+    for (const orr_ctx::Pending& p : g.timers) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    g = orr_ctx::TreeGraph{};
+}
+
+void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentView& segs,
+                            unsigned long long* seg_stats, orr_tree_stats* stats) {
+    if (stats != nullptr) std::memset(stats, 0, sizeof(*stats));
+    ctx->last_tree = orr_tree_stats{};
+    tree_enqueue(ctx, a, segs, seg_stats);
+    if (stats != nullptr) tree_read_stats(ctx, seg_stats, stats);
+}
+
+// One system: the first evaluation with a set of arguments runs eagerly (and
+// sizes every buffer), the second is captured as a hipGraph and launched, and
+// later ones replay it -- while the arguments, the walk setting and the
+// buffers (the reallocation counter) are unchanged. With phase timing on every
+// evaluation runs eagerly: events recorded inside a captured graph do not give
+// usable elapsed times.
+void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) {
+    if (stats != nullptr) std::memset(stats, 0, sizeof(*stats));
+    ctx->last_tree = orr_tree_stats{};
+    if (ctx->timing) {
+        drop_tree_graph(ctx);
+        tree_enqueue(ctx, a, SegmentView{}, nullptr);
+        if (stats != nullptr) tree_read_stats(ctx, nullptr, stats);
+        return;
+    }
+    orr_ctx::TreeGraph& g = ctx->tree_graph;
+    const std::uint64_t epoch = g_reallocations.load();
+    const bool same = g.seen && same_args(g.args, a) && g.walk == ctx->walk_targets_per_lane &&
+                      g.epoch == epoch;
+    if (!same) {
+        drop_tree_graph(ctx);
+        tree_enqueue(ctx, a, SegmentView{}, nullptr);
+        g.args = a;
+        g.walk = ctx->walk_targets_per_lane;
+        g.epoch = g_reallocations.load();
+        g.seen = a.count > 0;
+    } else {
+        if (g.exec == nullptr) {
+            const std::size_t before = ctx->pending.size();
+            ORR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            hipGraph_t graph = nullptr;
+            try {
+                tree_enqueue(ctx, a, SegmentView{}, nullptr);
+            } catch (...) {
+                (void)hipStreamEndCapture(ctx->stream, &graph);
+                if (graph != nullptr) (void)hipGraphDestroy(graph);
+                ctx->pending.resize(before);
+                throw;
+            }
+            ORR_HIP(hipStreamEndCapture(ctx->stream, &graph));
+            // The events recorded during capture are the graph's from now on.
+            for (std::size_t k = before; k < ctx->pending.size(); ++k) {
+                orr_ctx::Pending p = ctx->pending[k];
+                p.recycle = false;
+                g.timers.push_back(p);
+            }
+            ctx->pending.resize(before);
+            const hipError_t made = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (made != hipSuccess || g_reallocations.load() != g.epoch) {
+                // Nothing may allocate while capturing; if something did, run eagerly.
+                drop_tree_graph(ctx);
+                tree_enqueue(ctx, a, SegmentView{}, nullptr);
+                if (stats != nullptr) tree_read_stats(ctx, nullptr, stats);
+                return;
+            }
+        }
+        ORR_HIP(hipGraphLaunch(g.exec, ctx->stream));
+        for (const orr_ctx::Pending& p : g.timers) ctx->pending.push_back(p);
+    }
+    if (stats != nullptr) tree_read_stats(ctx, nullptr, stats);
 }
 
 void tree_copy_segment_counts(orr_ctx* ctx, std::uint64_t* out) {
