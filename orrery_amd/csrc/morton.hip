@@ -259,6 +259,14 @@ __global__ __launch_bounds__(kBlock) void radix_histogram(const std::uint64_t* _
     hist[static_cast<std::uint64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = counts[threadIdx.x];
 }
 
+// Stable scatter of one pass. Each block ranks its 4096 keys tile by tile
+// (256 at a time, in order: ballots give each lane the lanes of its wave with
+// the same digit, an LDS table the counts of the waves before it) into LDS in
+// digit order, then writes them out in that order: the keys of one digit leave
+// the block as one contiguous run (16 keys on average) instead of one
+// scattered 8-byte store per key. Positions are those of a plain stable
+// scatter: offsets (the scanned [digit][block] histogram) + rank within the
+// block's digit.
 __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __restrict__ keys_in,
                                                         const std::uint32_t* __restrict__ vals_in,
                                                         std::uint64_t n, int shift,
@@ -266,15 +274,44 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __r
                                                         const std::uint32_t* __restrict__ offsets,
                                                         std::uint64_t* __restrict__ keys_out,
                                                         std::uint32_t* __restrict__ vals_out) {
-    __shared__ std::uint32_t running[256];        // base offset of each digit for the next tile
+    __shared__ std::uint64_t skeys[kKeysPerBlock];
+    __shared__ std::uint32_t svals[kKeysPerBlock];
+    __shared__ std::uint32_t out_base[256];       // offsets[d][b] - block-local start of d
+    __shared__ std::uint32_t running[256];        // next local slot of each digit
     __shared__ std::uint32_t wave_counts[4][256];  // digit counts of each wave in this tile
+    __shared__ std::uint32_t wave_sums[4];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    running[threadIdx.x] = offsets[static_cast<std::uint64_t>(threadIdx.x) * gridDim.x + blockIdx.x];
+    const std::uint32_t grid = gridDim.x;
+    const std::uint32_t d0 = threadIdx.x;  // this thread's digit for the per-digit tables
+    const std::uint32_t here = d0 * grid + blockIdx.x;
+    const std::uint32_t base_d = offsets[here];
+    // The block's count of digit d0: the next entry of the digit-major offsets
+    // (the one after the last block of digit 255 is n).
+    const std::uint64_t next_d = here + 1 < 256u * grid ? offsets[here + 1] : n;
+    const std::uint32_t count_d = static_cast<std::uint32_t>(next_d - base_d);
+    // Exclusive scan of the 256 counts across the block: within each wave by
+    // shuffles, then the four wave sums.
+    std::uint32_t incl = count_d;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const std::uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wave_sums[wave] = incl;
     for (int w = 0; w < 4; ++w) wave_counts[w][threadIdx.x] = 0;
     __syncthreads();
+    std::uint32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += wave_sums[w];
+    out_base[d0] = base_d - (before + incl - count_d);
+    running[d0] = before + incl - count_d;
+    __syncthreads();
+
     const std::uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * kKeysPerBlock;
+    const std::uint32_t in_block = static_cast<std::uint32_t>(
+        n - base < static_cast<std::uint64_t>(kKeysPerBlock) ? n - base : kKeysPerBlock);
+#pragma unroll 1
     for (int t = 0; t < kTilesPerBlock; ++t) {
         const std::uint64_t i = base + static_cast<std::uint64_t>(t) * kBlock + threadIdx.x;
         const bool valid = i < n;
@@ -293,18 +330,27 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __r
         if (valid && rank == 0) wave_counts[wave][d] = static_cast<std::uint32_t>(__popcll(same));
         __syncthreads();
         if (valid) {
-            std::uint32_t pos = running[d] + rank;
-            for (int w = 0; w < wave; ++w) pos += wave_counts[w][d];
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            std::uint32_t slot = running[d] + rank;
+            for (int w = 0; w < wave; ++w) slot += wave_counts[w][d];
+            skeys[slot] = key;
+            svals[slot] = val;
         }
         __syncthreads();
-        // Advance each digit's base past this tile and clear the wave table.
+        // Advance each digit's slot past this tile and clear the wave table.
         const std::uint32_t total = wave_counts[0][threadIdx.x] + wave_counts[1][threadIdx.x] +
                                     wave_counts[2][threadIdx.x] + wave_counts[3][threadIdx.x];
         running[threadIdx.x] += total;
         for (int w = 0; w < 4; ++w) wave_counts[w][threadIdx.x] = 0;
         __syncthreads();
+    }
+    // Out in local (digit) order: slot s of digit d goes to offsets[d][b] + (s - start of d).
+#pragma unroll 4
+    for (std::uint32_t s = threadIdx.x; s < in_block; s += kBlock) {
+        const std::uint64_t key = skeys[s];
+        const std::uint32_t val = svals[s];
+        const std::uint32_t pos = out_base[digit_of(key, val, shift, seg_of)] + s;
+        keys_out[pos] = key;
+        vals_out[pos] = val;
     }
 }
 
