@@ -28,7 +28,7 @@ namespace orr {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTilesPerBlock = 16;                  // keys per sort block = 4096
+constexpr int kTilesPerBlock = 8;                   // keys per sort block = 2048
 constexpr int kKeysPerBlock = kBlock * kTilesPerBlock;
 constexpr unsigned kGrid = 1u << 21;
 
