@@ -1,9 +1,9 @@
-// Device-wide exclusive prefix sum of uint32 values (three passes, deterministic).
+// Device-wide exclusive prefix sum of uint32 values (two passes, deterministic).
 //
-// Pass 1 sums each 2048-element block; pass 2 scans the block sums in one
-// workgroup and writes the grand total; pass 3 scans each block again with its
-// offset. No atomics, so the result never depends on scheduling. In place is
-// allowed (in == out).
+// Pass 1 sums each 2048-element block; pass 2 scans each block again, with the
+// sum of the block sums before it as its offset, and its last block writes the
+// grand total. No atomics, so the result never depends on scheduling. In place
+// is allowed (in == out).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -47,37 +47,19 @@ __global__ __launch_bounds__(kThreads) void scan_block_sums(const std::uint32_t*
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(1024) void scan_sums(std::uint32_t* __restrict__ sums,
-                                                  std::uint32_t count,
-                                                  std::uint32_t* __restrict__ total_out) {
-    __shared__ std::uint32_t lds[1024];
-    const std::uint32_t per = (count + 1023) / 1024;
-    const std::uint32_t b = threadIdx.x * per;
-    const std::uint32_t e = min(count, b + per);
-    std::uint32_t local = 0;
-    for (std::uint32_t i = b; i < e; ++i) local += sums[i];
-    lds[threadIdx.x] = local;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const std::uint32_t a = threadIdx.x >= static_cast<unsigned>(off) ? lds[threadIdx.x - off] : 0u;
-        __syncthreads();
-        lds[threadIdx.x] += a;
-        __syncthreads();
-    }
-    std::uint32_t run = lds[threadIdx.x] - local;
-    for (std::uint32_t i = b; i < e; ++i) {
-        const std::uint32_t v = sums[i];
-        sums[i] = run;
-        run += v;
-    }
-    if (threadIdx.x == 1023 && total_out != nullptr) *total_out = lds[1023];
-}
-
+// Each block first adds up the block sums before it (at most a few thousand
+// values, read by the whole block), so no separate pass scans them.
 __global__ __launch_bounds__(kThreads) void scan_apply(const std::uint32_t* __restrict__ in,
                                                        std::uint64_t n,
                                                        const std::uint32_t* __restrict__ sums,
-                                                       std::uint32_t* __restrict__ out) {
+                                                       std::uint32_t blocks,
+                                                       std::uint32_t* __restrict__ out,
+                                                       std::uint32_t* __restrict__ total_out) {
     __shared__ std::uint32_t lds[kThreads];
+    std::uint32_t part = 0;
+    for (std::uint32_t i = threadIdx.x; i < blockIdx.x; i += kThreads) part += sums[i];
+    std::uint32_t offset = 0;
+    (void)block_exclusive(part, lds, offset);
     const std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * kPerBlock + threadIdx.x * kItems;
     std::uint32_t v[kItems];
     std::uint32_t local = 0;
@@ -87,12 +69,13 @@ __global__ __launch_bounds__(kThreads) void scan_apply(const std::uint32_t* __re
         local += v[k];
     }
     std::uint32_t total = 0;
-    std::uint32_t run = block_exclusive(local, lds, total) + sums[blockIdx.x];
+    std::uint32_t run = block_exclusive(local, lds, total) + offset;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         if (base + k < n) out[base + k] = run;
         run += v[k];
     }
+    if (total_out != nullptr && blockIdx.x + 1 == blocks && threadIdx.x == 0) *total_out = offset + total;
 }
 
 }  // namespace
@@ -107,9 +90,8 @@ void exclusive_scan_u32(orr_ctx* ctx, const std::uint32_t* in, std::uint32_t* ou
     const std::uint32_t blocks = static_cast<std::uint32_t>((n + kPerBlock - 1) / kPerBlock);
     hipLaunchKernelGGL(scan_block_sums, dim3(blocks), dim3(kThreads), 0, ctx->stream, in, n, scratch);
     ORR_LAUNCHED("scan_block_sums");
-    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, ctx->stream, scratch, blocks, total_out);
-    ORR_LAUNCHED("scan_sums");
-    hipLaunchKernelGGL(scan_apply, dim3(blocks), dim3(kThreads), 0, ctx->stream, in, n, scratch, out);
+    hipLaunchKernelGGL(scan_apply, dim3(blocks), dim3(kThreads), 0, ctx->stream, in, n, scratch,
+                       blocks, out, total_out);
     ORR_LAUNCHED("scan_apply");
 }
 
