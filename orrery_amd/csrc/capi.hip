@@ -84,12 +84,7 @@ void field_eval(orr_system* s, const orr_step_config& c) {
         d.softening_squared = c.softening_squared;
         d.gravitational_constant = c.gravitational_constant;
         d.spatial_order = 1;
-        const std::uint32_t* order = nullptr;
-        if (s->count > 64) {
-            orr::morton_enqueue(s->ctx, d.x, d.y, d.z, s->count);
-            order = s->ctx->order_index.as<std::uint32_t>();
-        }
-        orr::direct_enqueue(s->ctx, d, order);
+        orr::direct_evaluate(s->ctx, d);
     }
 }
 
@@ -251,12 +246,7 @@ int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* a) {
         if (a->count == 0) return;
         require(a->x && a->y && a->z && a->m && a->ax && a->ay && a->az, "null particle array");
         use_device(ctx);
-        const std::uint32_t* order = nullptr;
-        if (a->spatial_order != 0 && a->count > 64) {
-            orr::morton_enqueue(ctx, a->x, a->y, a->z, a->count);
-            order = ctx->order_index.as<std::uint32_t>();
-        }
-        orr::direct_enqueue(ctx, *a, order);
+        orr::direct_evaluate(ctx, *a);
         ctx->sync();
     });
 }

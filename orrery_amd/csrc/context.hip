@@ -54,6 +54,57 @@ std::size_t ctx_device_bytes(const orr_ctx* ctx) {
     return total;
 }
 
+void drop_captured(orr_ctx::Captured& g) noexcept {
+    if (g.exec != nullptr) (void)hipGraphExecDestroy(g.exec);
+    g = orr_ctx::Captured{};
+}
+
+void run_captured(orr_ctx* ctx, orr_ctx::Captured& g, std::vector<unsigned char> key,
+                  const std::function<void()>& enqueue, bool untimed) {
+    if (ctx->timing && !untimed) {
+        drop_captured(g);
+        enqueue();
+        return;
+    }
+    // Phase timers stay off inside (and are restored however this returns).
+    struct Untimed {
+        orr_ctx* ctx;
+        bool was;
+        ~Untimed() { ctx->timing = was; }
+    } restore{ctx, ctx->timing};
+    ctx->timing = false;
+    if (!(g.seen && g.key == key && g.epoch == g_reallocations.load())) {
+        drop_captured(g);
+        enqueue();
+        g.key = std::move(key);
+        g.epoch = g_reallocations.load();
+        g.seen = true;
+        return;
+    }
+    if (g.exec == nullptr) {
+        ORR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipGraph_t graph = nullptr;
+        try {
+            enqueue();
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &graph);
+            if (graph != nullptr) (void)hipGraphDestroy(graph);
+            drop_captured(g);
+            throw;
+        }
+        ORR_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        const hipError_t made = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (made != hipSuccess || g.epoch != g_reallocations.load()) {
+            // Nothing may allocate while capturing; if something did, run eagerly.
+            drop_captured(g);
+            enqueue();
+            return;
+        }
+    }
+    ORR_HIP(hipGraphLaunch(g.exec, ctx->stream));
+}
+
 orr_ctx* make_context(int device) {
     ORR_HIP(hipSetDevice(device));
     auto ctx = std::make_unique<orr_ctx>();
@@ -70,10 +121,10 @@ void free_context(orr_ctx* ctx) noexcept {
     if (ctx == nullptr) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    drop_tree_graph(ctx);
+    drop_captured(ctx->tree_graph);
+    drop_captured(ctx->direct_graph);
     for (Buffer* b : ctx->buffers()) b->release();
     for (auto& p : ctx->pending) {
-        if (!p.recycle) continue;  // a graph's, destroyed with it
         (void)hipEventDestroy(p.start);
         (void)hipEventDestroy(p.stop);
     }
@@ -115,10 +166,8 @@ void orr_ctx::sync() {
         ORR_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
         totals.milliseconds[p.phase] += ms;
         totals.launches[p.phase] += 1;
-        if (p.recycle) {
-            free_events.push_back(p.start);
-            free_events.push_back(p.stop);
-        }
+        free_events.push_back(p.start);
+        free_events.push_back(p.stop);
     }
     pending.clear();
 }

@@ -500,6 +500,24 @@ std::size_t direct_batch_bytes(std::uint64_t tiles, std::uint64_t slots) {
     return packed_bytes(tiles) + centre_bytes(tiles) + slot_bytes(slots);
 }
 
+void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a) {
+    if (a.count == 0) return;
+    const std::uint32_t* order = nullptr;
+    if (a.spatial_order != 0 && a.count > 64) {
+        // The Morton order is some 35 short launches: replayed as a graph (its
+        // phases untimed); the three direct launches stay eager and timed.
+        std::vector<unsigned char> key;
+        key_add(key, a.x);
+        key_add(key, a.y);
+        key_add(key, a.z);
+        key_add(key, a.count);
+        run_captured(ctx, ctx->direct_graph, std::move(key),
+                     [&] { morton_enqueue(ctx, a.x, a.y, a.z, a.count); }, true);
+        order = ctx->order_index.as<std::uint32_t>();
+    }
+    direct_enqueue(ctx, a, order);
+}
+
 void direct_batch_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
                           const double* m, const std::uint32_t* order, const DirectBatchView& b,
                           double* ax, double* ay, double* az) {

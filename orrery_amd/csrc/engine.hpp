@@ -11,6 +11,7 @@
 #include <atomic>
 #include <cstddef>
 #include <array>
+#include <functional>
 #include <cstdint>
 #include <new>
 #include <stdexcept>
@@ -137,24 +138,21 @@ struct orr_ctx {
     struct Pending {
         int phase;
         hipEvent_t start, stop;
-        bool recycle = true;  // false: the events belong to a captured graph
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> free_events;
     orr_timings totals{};
 
-    // A captured single-system tree evaluation (tree.hip), replayed while its
-    // arguments, the walk setting and every buffer stay those it was captured
-    // with (and phase timing is off): the evaluation is some 60 short
-    // launches, and replaying them removes the host's launch gaps between them.
-    struct TreeGraph {
-        orr_tree_args args{};
-        int walk = 0;
-        std::uint64_t epoch = 0;
-        bool seen = false;            // this key has run once eagerly
+    // Captured single-system evaluations (tree.hip, direct.hip; see
+    // orr::run_captured), replayed while their arguments, kernel settings and
+    // every buffer stay those they were captured with.
+    struct Captured {
+        std::vector<unsigned char> key;  // the arguments and settings, field by field
+        std::uint64_t epoch = 0;         // g_reallocations when they last ran
+        bool seen = false;               // this key has run once eagerly
         hipGraphExec_t exec = nullptr;
-        std::vector<Pending> timers;  // phase events recorded while capturing (none: timing is off)
-    } tree_graph;
+    };
+    Captured tree_graph, direct_graph;
 
     hipEvent_t take_event();
     void begin(int phase, hipEvent_t& start);
@@ -257,8 +255,26 @@ std::size_t direct_batch_bytes(std::uint64_t tiles, std::uint64_t slots);
 // batch of several systems holds at most the sum of theirs plus 4 KiB.
 std::uint64_t system_bytes_bound(std::uint64_t n, int solver, const orr_batch_config& cfg);
 
-// tree.hip: releases the context's captured tree evaluation, if any.
-void drop_tree_graph(orr_ctx* ctx) noexcept;
+// context.hip: a sequence of short launches with a fixed key (arguments and
+// settings as bytes): the first run with a key is eager (and sizes every
+// buffer), the second is captured as a hipGraph and launched, later ones replay
+// it -- removing the host's launch gaps -- while the key and every buffer
+// (g_reallocations) are unchanged. Events recorded inside a captured graph give
+// no usable times, so with phase timing on it runs eagerly, unless `untimed`:
+// then the sequence is replayed anyway and its phases go untimed.
+void run_captured(orr_ctx* ctx, orr_ctx::Captured& g, std::vector<unsigned char> key,
+                  const std::function<void()>& enqueue, bool untimed = false);
+void drop_captured(orr_ctx::Captured& g) noexcept;
+
+// Appends a value's bytes to a replay key.
+template<typename T> void key_add(std::vector<unsigned char>& key, const T& v) {
+    const auto* b = reinterpret_cast<const unsigned char*>(&v);
+    key.insert(key.end(), b, b + sizeof(T));
+}
+
+// direct.hip: Morton order (when asked for; replayed as a graph where
+// run_captured allows) and the direct sum.
+void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a);
 
 // context.hip: device bytes held by a context's scratch buffers.
 std::size_t ctx_device_bytes(const orr_ctx* ctx);

@@ -1053,24 +1053,7 @@ void tree_read_stats(orr_ctx* ctx, unsigned long long* seg_stats, orr_tree_stats
     ctx->last_tree = *stats;
 }
 
-bool same_args(const orr_tree_args& p, const orr_tree_args& q) {
-    return p.x == q.x && p.y == q.y && p.z == q.z && p.m == q.m && p.ax == q.ax && p.ay == q.ay &&
-           p.az == q.az && p.count == q.count && p.softening_squared == q.softening_squared &&
-           p.gravitational_constant == q.gravitational_constant && p.opening_angle == q.opening_angle &&
-           p.leaf_capacity == q.leaf_capacity && p.quadrupole == q.quadrupole;
-}
-
 }  // namespace
-
-void drop_tree_graph(orr_ctx* ctx) noexcept {
-    orr_ctx::TreeGraph& g = ctx->tree_graph;
-    if (g.exec != nullptr) (void)hipGraphExecDestroy(g.exec);
-    for (const orr_ctx::Pending& p : g.timers) {
-        (void)hipEventDestroy(p.start);
-        (void)hipEventDestroy(p.stop);
-    }
-    g = orr_ctx::TreeGraph{};
-}
 
 void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentView& segs,
                             unsigned long long* seg_stats, orr_tree_stats* stats) {
@@ -1080,66 +1063,31 @@ void tree_evaluate_segments(orr_ctx* ctx, const orr_tree_args& a, const SegmentV
     if (stats != nullptr) tree_read_stats(ctx, seg_stats, stats);
 }
 
-// One system: the first evaluation with a set of arguments runs eagerly (and
-// sizes every buffer), the second is captured as a hipGraph and launched, and
-// later ones replay it -- while the arguments, the walk setting and the
-// buffers (the reallocation counter) are unchanged. With phase timing on every
-// evaluation runs eagerly: events recorded inside a captured graph do not give
-// usable elapsed times.
+// One system: replayed as a captured graph where run_captured allows (the
+// key: every argument and the walk setting).
 void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) {
     if (stats != nullptr) std::memset(stats, 0, sizeof(*stats));
     ctx->last_tree = orr_tree_stats{};
-    if (ctx->timing) {
-        drop_tree_graph(ctx);
+    if (a.count == 0) {
         tree_enqueue(ctx, a, SegmentView{}, nullptr);
-        if (stats != nullptr) tree_read_stats(ctx, nullptr, stats);
         return;
     }
-    orr_ctx::TreeGraph& g = ctx->tree_graph;
-    const std::uint64_t epoch = g_reallocations.load();
-    const bool same = g.seen && same_args(g.args, a) && g.walk == ctx->walk_targets_per_lane &&
-                      g.epoch == epoch;
-    if (!same) {
-        drop_tree_graph(ctx);
-        tree_enqueue(ctx, a, SegmentView{}, nullptr);
-        g.args = a;
-        g.walk = ctx->walk_targets_per_lane;
-        g.epoch = g_reallocations.load();
-        g.seen = a.count > 0;
-    } else {
-        if (g.exec == nullptr) {
-            const std::size_t before = ctx->pending.size();
-            ORR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            hipGraph_t graph = nullptr;
-            try {
-                tree_enqueue(ctx, a, SegmentView{}, nullptr);
-            } catch (...) {
-                (void)hipStreamEndCapture(ctx->stream, &graph);
-                if (graph != nullptr) (void)hipGraphDestroy(graph);
-                ctx->pending.resize(before);
-                throw;
-            }
-            ORR_HIP(hipStreamEndCapture(ctx->stream, &graph));
-            // The events recorded during capture are the graph's from now on.
-            for (std::size_t k = before; k < ctx->pending.size(); ++k) {
-                orr_ctx::Pending p = ctx->pending[k];
-                p.recycle = false;
-                g.timers.push_back(p);
-            }
-            ctx->pending.resize(before);
-            const hipError_t made = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (made != hipSuccess || g_reallocations.load() != g.epoch) {
-                // Nothing may allocate while capturing; if something did, run eagerly.
-                drop_tree_graph(ctx);
-                tree_enqueue(ctx, a, SegmentView{}, nullptr);
-                if (stats != nullptr) tree_read_stats(ctx, nullptr, stats);
-                return;
-            }
-        }
-        ORR_HIP(hipGraphLaunch(g.exec, ctx->stream));
-        for (const orr_ctx::Pending& p : g.timers) ctx->pending.push_back(p);
+    std::vector<unsigned char> key;
+    for (const void* p : {static_cast<const void*>(a.x), static_cast<const void*>(a.y),
+                          static_cast<const void*>(a.z), static_cast<const void*>(a.m),
+                          static_cast<const void*>(a.ax), static_cast<const void*>(a.ay),
+                          static_cast<const void*>(a.az)}) {
+        key_add(key, p);
     }
+    key_add(key, a.count);
+    key_add(key, a.softening_squared);
+    key_add(key, a.gravitational_constant);
+    key_add(key, a.opening_angle);
+    key_add(key, a.leaf_capacity);
+    key_add(key, a.quadrupole);
+    key_add(key, ctx->walk_targets_per_lane);
+    run_captured(ctx, ctx->tree_graph, std::move(key),
+                 [&] { tree_enqueue(ctx, a, SegmentView{}, nullptr); });
     if (stats != nullptr) tree_read_stats(ctx, nullptr, stats);
 }
 
