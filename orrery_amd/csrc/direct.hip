@@ -135,32 +135,43 @@ __device__ __forceinline__ void kahan(float p, float& s, float& c) {
     s = t;
 }
 
+// Near: the gap between the two tiles' boxes (centre c, half-extents e; the
+// source tile's C, E) is at most kNearGap of their summed half-widths. Outside
+// it every separation exceeds that gap, so the FP32 offsets round it to
+// O(epsilon) relative error. One definition, without contraction, so the mask
+// pass and the kernels classify every tile pair alike.
+__device__ __forceinline__ bool near_tiles(const double4& C, const float4& E, double cx, double cy,
+                                           double cz, const float4& e) {
+#pragma clang fp contract(off)
+    const float gx = fmaxf(0.f, fabsf(static_cast<float>(C.x - cx)) - e.x - E.x);
+    const float gy = fmaxf(0.f, fabsf(static_cast<float>(C.y - cy)) - e.y - E.y);
+    const float gz = fmaxf(0.f, fabsf(static_cast<float>(C.z - cz)) - e.z - E.z);
+    const float reach = kNearGap * (e.w + E.w);
+    return gx * gx + gy * gy + gz * gz <= reach * reach;
+}
+
 // Targets' sums against one 64-source tile J: the tile's terms summed plainly
 // in FP32, then Kahan-added per component into (s, k). kUnroll sets how many
 // far-path sources are in flight (their SGPR loads included).
-template<int R, bool kSafe, bool kRefine, int kUnroll = 16>
+template<int R, bool kSafe, bool kRefine, bool kGivenNear = false, int kUnroll = 16>
 __device__ __forceinline__ void source_tile(
     const float4* __restrict__ packed, const float4* __restrict__ packed_lo,
     const double4* __restrict__ centres, const float4* __restrict__ extents, std::uint32_t J,
-    float eps2, const float (&ox)[R], const float (&oy)[R], const float (&oz)[R],
+    bool given_near, float eps2, const float (&ox)[R], const float (&oy)[R], const float (&oz)[R],
     const float (&lx)[R], const float (&ly)[R], const float (&lz)[R], const double (&cx)[R],
     const double (&cy)[R], const double (&cz)[R], const float4 (&ce)[R], float (&sx)[R],
     float (&sy)[R], float (&sz)[R], float (&kx)[R], float (&ky)[R], float (&kz)[R]) {
     const double4 C = centres[J];
     const float4 E = extents[J];
     const float4* __restrict__ src = packed + static_cast<std::size_t>(J) * kTile;
-    // Near: the gap between the two tiles' boxes is at most kNearGap of their
-    // summed half-widths. Outside it every separation exceeds that gap, so the
-    // FP32 offsets round it to O(epsilon) relative error. All lanes of a wave
-    // share one target tile per r, so the branch is uniform.
+    // All lanes of a wave share one target tile per r, so the branch is
+    // uniform. With kGivenNear the caller read the pair's flag from the mask.
     bool near = false;
+    if constexpr (kGivenNear) {
+        near = given_near;
+    } else {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const float gx = fmaxf(0.f, fabsf(static_cast<float>(C.x - cx[r])) - ce[r].x - E.x);
-        const float gy = fmaxf(0.f, fabsf(static_cast<float>(C.y - cy[r])) - ce[r].y - E.y);
-        const float gz = fmaxf(0.f, fabsf(static_cast<float>(C.z - cz[r])) - ce[r].z - E.z);
-        const float reach = kNearGap * (ce[r].w + E.w);
-        near = near || (gx * gx + gy * gy + gz * gz <= reach * reach);
+        for (int r = 0; r < R; ++r) near = near || near_tiles(C, E, cx[r], cy[r], cz[r], ce[r]);
     }
     float px[R], py[R], pz[R];
 #pragma unroll
@@ -217,11 +228,49 @@ __device__ __forceinline__ void source_tile(
     }
 }
 
-template<int R, bool kSafe, bool kRefine>
+// The near flags of every (target tile, source tile) pair, one bit each, rows
+// of ceil(tiles / 32) words: computed once per evaluation so that the kernels
+// read one word per 32 source tiles through the scalar unit instead of each
+// wave repeating the (wave-uniform) box test on its vector lanes.
+constexpr int kMaskRows = 8;  // target tiles per block of the mask pass
+
+__global__ __launch_bounds__(256) void near_mask(const double4* __restrict__ centres,
+                                                 const float4* __restrict__ extents,
+                                                 std::uint32_t tiles, std::uint32_t words,
+                                                 std::uint32_t* __restrict__ mask) {
+    // kMaskRows target tiles per block (wave-uniform: scalar loads); each
+    // thread loads one source tile per chunk and tests it against all of them;
+    // a wave's 64 results become two words by ballot.
+    const std::uint32_t row0 = blockIdx.x * kMaskRows;
+    const int lane = threadIdx.x & 63;
+    for (std::uint32_t chunk = 0; chunk < tiles; chunk += blockDim.x) {
+        const std::uint32_t J = chunk + threadIdx.x;
+        const bool valid = J < tiles;
+        const double4 C = valid ? centres[J] : make_double4(0, 0, 0, 0);
+        const float4 E = valid ? extents[J] : make_float4(0, 0, 0, 0);
+        const std::uint32_t first_word = (chunk + (threadIdx.x & ~63u)) / 32;
+#pragma unroll
+        for (int r = 0; r < kMaskRows; ++r) {
+            const std::uint32_t row = row0 + r;
+            if (row >= tiles) break;
+            const double4 c = centres[row];
+            const float4 e = extents[row];
+            const std::uint64_t bits = __ballot(valid && near_tiles(C, E, c.x, c.y, c.z, e));
+            if (lane == 0) {
+                std::uint32_t* out = mask + static_cast<std::size_t>(row) * words;
+                if (first_word < words) out[first_word] = static_cast<std::uint32_t>(bits);
+                if (first_word + 1 < words) out[first_word + 1] = static_cast<std::uint32_t>(bits >> 32);
+            }
+        }
+    }
+}
+
+template<int R, bool kSafe, bool kRefine, bool kMask>
 __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict__ packed,
                                                        const float4* __restrict__ packed_lo,
                                                        const double4* __restrict__ centres,
                                                        const float4* __restrict__ extents,
+                                                       const std::uint32_t* __restrict__ mask,
                                                        std::uint32_t npad, std::uint32_t tiles,
                                                        std::uint32_t tiles_per_split, float eps2,
                                                        float* __restrict__ partial) {
@@ -234,6 +283,8 @@ __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict_
     double cx[R], cy[R], cz[R];
     float4 ce[R];
     float sx[R], sy[R], sz[R], kx[R], ky[R], kz[R];
+    std::uint32_t row[R];  // first mask word of each target tile's row
+    const std::uint32_t words = (tiles + 31) / 32;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const std::uint32_t k = min(base + r * kBlock, npad - 1);
@@ -254,13 +305,26 @@ __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict_
         cy[r] = c.y;
         cz[r] = c.z;
         ce[r] = extents[tile];
+        row[r] = tile * words;
         sx[r] = sy[r] = sz[r] = 0.f;
         kx[r] = ky[r] = kz[r] = 0.f;
     }
 
-    for (std::uint32_t J = j0; J < j1; ++J)
-        source_tile<R, kSafe, kRefine>(packed, packed_lo, centres, extents, J, eps2, ox, oy, oz,
-                                       lx, ly, lz, cx, cy, cz, ce, sx, sy, sz, kx, ky, kz);
+    std::uint32_t bits = 0;
+    for (std::uint32_t J = j0; J < j1; ++J) {
+        bool near = false;
+        if constexpr (kMask) {
+            if (J == j0 || (J & 31u) == 0) {
+                bits = 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) bits |= mask[row[r] + J / 32];
+            }
+            near = ((bits >> (J & 31u)) & 1u) != 0;
+        }
+        source_tile<R, kSafe, kRefine, kMask>(packed, packed_lo, centres, extents, J, near, eps2,
+                                              ox, oy, oz, lx, ly, lz, cx, cy, cz, ce, sx, sy, sz, kx,
+                                              ky, kz);
+    }
 
     // Component c of target k at partial[(6 split + c) npad + k], in 64-bit
     // offsets (6 npad exceeds 32 bits from npad = 2^30 / 1.5).
@@ -370,7 +434,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         float sx[1] = {0.f}, sy[1] = {0.f}, sz[1] = {0.f}, kx[1] = {0.f}, ky[1] = {0.f},
               kz[1] = {0.f};
         for (std::uint32_t J = it.y; J < it.z; ++J)
-            source_tile<1, kSafe, kRefine, kItemUnroll>(packed, packed_lo, centres, extents, J, eps2,
+            source_tile<1, kSafe, kRefine, false, kItemUnroll>(packed, packed_lo, centres, extents, J, false, eps2,
                                                         ox, oy, oz, lx, ly, lz, cx, cy, cz, ce, sx,
                                                         sy, sz, kx, ky, kz);
         float* out = partial + static_cast<std::size_t>(it.w) * 6 * kTile;
@@ -417,25 +481,32 @@ __global__ __launch_bounds__(256) void batch_reduce(const float* __restrict__ pa
     az[dst] = G * a2;
 }
 
-template<int R, bool kSafe, bool kRefine>
+template<int R, bool kSafe, bool kRefine, bool kMask>
 void launch_tiles(orr_ctx* ctx, std::uint32_t blocks_i, std::uint32_t splits, const float4* packed,
                   const float4* packed_lo, const double4* centres, const float4* extents,
-                  std::uint32_t npad,
-                  std::uint32_t tiles, std::uint32_t per_split, float eps2, float* partials) {
-    hipLaunchKernelGGL((direct_tiles<R, kSafe, kRefine>), dim3(blocks_i, splits), dim3(kBlock), 0,
-                       ctx->stream, packed, packed_lo, centres, extents, npad, tiles, per_split,
+                  const std::uint32_t* mask, std::uint32_t npad, std::uint32_t tiles,
+                  std::uint32_t per_split, float eps2, float* partials) {
+    hipLaunchKernelGGL((direct_tiles<R, kSafe, kRefine, kMask>), dim3(blocks_i, splits), dim3(kBlock), 0,
+                       ctx->stream, packed, packed_lo, centres, extents, mask, npad, tiles, per_split,
                        eps2, partials);
 }
 
 using TilesLauncher = void (*)(orr_ctx*, std::uint32_t, std::uint32_t, const float4*,
-                               const float4*, const double4*, const float4*, std::uint32_t,
-                               std::uint32_t,
-                               std::uint32_t, float, float*);
+                               const float4*, const double4*, const float4*, const std::uint32_t*,
+                               std::uint32_t, std::uint32_t, std::uint32_t, float, float*);
 
-template<int R> TilesLauncher pick(bool safe, bool refine) {
-    if (safe) return refine ? launch_tiles<R, true, true> : launch_tiles<R, true, false>;
-    return refine ? launch_tiles<R, false, true> : launch_tiles<R, false, false>;
+template<int R, bool kMask> TilesLauncher pick_mask(bool safe, bool refine) {
+    if (safe) return refine ? launch_tiles<R, true, true, kMask> : launch_tiles<R, true, false, kMask>;
+    return refine ? launch_tiles<R, false, true, kMask> : launch_tiles<R, false, false, kMask>;
 }
+
+template<int R> TilesLauncher pick(bool safe, bool refine, bool mask) {
+    return mask ? pick_mask<R, true>(safe, refine) : pick_mask<R, false>(safe, refine);
+}
+
+// Pairs of tiles flagged by a mask up to this many tiles (a 128 MiB mask);
+// beyond it the kernels test each pair themselves.
+constexpr std::uint32_t kMaskTiles = 32768;
 
 }  // namespace
 
@@ -463,22 +534,34 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
     float4* extents = reinterpret_cast<float4*>(centres + tiles);
     float* partials = ctx->direct_partials.as<float>();
 
+    const bool masked = tiles <= kMaskTiles;
+    const std::uint32_t words = (tiles + 31) / 32;
+    std::uint32_t* mask = nullptr;
+    if (masked) {
+        ctx->direct_near.reserve(sizeof(std::uint32_t) * static_cast<std::size_t>(words) * tiles);
+        mask = ctx->direct_near.as<std::uint32_t>();
+    }
     {
         PhaseTimer t(ctx, kDirectPrepare);
         hipLaunchKernelGGL(direct_prepare, dim3(tiles), dim3(kTile), 0, ctx->stream, a.x, a.y, a.z,
                            a.m, order, n, packed, packed_lo, centres, extents);
         ORR_LAUNCHED("direct_prepare");
+        if (masked) {
+            hipLaunchKernelGGL(near_mask, dim3((tiles + kMaskRows - 1) / kMaskRows), dim3(256), 0,
+                               ctx->stream, centres, extents, tiles, words, mask);
+            ORR_LAUNCHED("near_mask");
+        }
     }
     const float eps2 = static_cast<float>(a.softening_squared);
     {
         PhaseTimer t(ctx, kDirectKernel);
         const bool safe = !(a.softening_squared > 0);
         const bool refine = ctx->direct_refine != 0;
-        TilesLauncher launch = R == 1 ? pick<1>(safe, refine)
-                             : R == 4 ? pick<4>(safe, refine)
-                                      : pick<2>(safe, refine);
-        launch(ctx, blocks_i, splits, packed, packed_lo, centres, extents, npad, tiles, per_split,
-               eps2, partials);
+        TilesLauncher launch = R == 1 ? pick<1>(safe, refine, masked)
+                             : R == 4 ? pick<4>(safe, refine, masked)
+                                      : pick<2>(safe, refine, masked);
+        launch(ctx, blocks_i, splits, packed, packed_lo, centres, extents, mask, npad, tiles,
+               per_split, eps2, partials);
         ORR_LAUNCHED("direct_tiles");
     }
     {

@@ -106,16 +106,16 @@ struct orr_ctx {
     int compute_units = 0;
 
     // Scratch, grown on demand.
-    orr::Buffer direct_packed, direct_centres, direct_partials;
+    orr::Buffer direct_packed, direct_centres, direct_partials, direct_near;
     orr::Buffer order_aos;  // {x, y, z, m} per particle, index order (tree path)
     orr::Buffer order_codes, order_index, order_codes_alt, order_index_alt, order_hist, order_bbox;
     orr::Buffer tree_work;  // one arena: per-particle, per-node and counter arrays
     orr::Buffer diag_scratch;
     // Every scratch buffer above, for release and accounting.
-    std::array<orr::Buffer*, 12> buffers() {
-        return {&direct_packed, &direct_centres, &direct_partials, &order_aos,   &order_codes,
-                &order_index,   &order_codes_alt, &order_index_alt, &order_hist, &order_bbox,
-                &tree_work,     &diag_scratch};
+    std::array<orr::Buffer*, 13> buffers() {
+        return {&direct_packed,   &direct_centres,  &direct_partials, &direct_near, &order_aos,
+                &order_codes,     &order_index,     &order_codes_alt, &order_index_alt,
+                &order_hist,      &order_bbox,      &tree_work,       &diag_scratch};
     }
 
     // Direct-kernel configuration (orr_ctx_set_direct_config).
