@@ -546,6 +546,24 @@ __device__ __forceinline__ void kahan(float p, float& s, float& c) {
     s = t;
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn (block b to XCD b mod 8),
+// and each XCD has its own L2. Neighbouring targets in Morton order visit the
+// same deep nodes and leaves, so block b walks the logical block that gives
+// each XCD runs of kXcdRun consecutive logical blocks, the runs dealt to the
+// XCDs in turn: the nodes a wave pulls into an L2 are the ones its
+// neighbours on that XCD need, while every XCD still sees all parts of the
+// system (one contiguous eighth each would leave the XCD with the densest part
+// finishing last). A bijection of [0, blocks): blocks past the last whole
+// round of runs keep their own index.
+constexpr std::uint32_t kXcds = 8;
+constexpr std::uint32_t kXcdRun = 16;
+__device__ __forceinline__ std::uint32_t xcd_block(std::uint32_t b, std::uint32_t blocks) {
+    const std::uint32_t whole = blocks / (kXcds * kXcdRun) * (kXcds * kXcdRun);
+    if (b >= whole) return b;
+    const std::uint32_t x = b % kXcds, k = b / kXcds;
+    return ((k / kXcdRun) * kXcds + x) * kXcdRun + k % kXcdRun;
+}
+
 // One target's walk state: FP32 position relative to the root centre and
 // offset from its leaf centre, its leaf, the next node it needs, Kahan sums.
 struct WalkTarget {
@@ -666,7 +684,7 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     unsigned long long* __restrict__ stats, unsigned long long* __restrict__ wave_stats) {
     constexpr std::uint32_t kDone = 0xffffffffu;
-    const std::uint32_t wave = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    const std::uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + threadIdx.x / 64;
     const std::uint32_t lane = threadIdx.x & 63;
     const std::uint32_t M = counters[0];
     WalkTarget t[kR];
