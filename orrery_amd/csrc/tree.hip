@@ -650,11 +650,15 @@ __device__ __forceinline__ void leaf_pairs(WalkTarget& t, const double4& cb,
         t.py = __builtin_fmaf(f, dy, t.py);
         t.pz = __builtin_fmaf(f, dz, t.pz);
     };
-    // Four sources per round so their scalar loads share one wait.
+    // Four sources per round, addressed from one base so that their scalar
+    // loads are issued together and share one wait.
     std::uint32_t j = b;
     for (; j + 4 <= e; j += 4) {
-        const float4 q0 = leaf_rel[j], q1 = leaf_rel[j + 1];
-        const float4 q2 = leaf_rel[j + 2], q3 = leaf_rel[j + 3];
+        const float4* q = leaf_rel + j;
+        const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        asm volatile("" ::"s"(q0.x), "s"(q0.y), "s"(q0.z), "s"(q0.w), "s"(q1.x), "s"(q1.y),
+                     "s"(q1.z), "s"(q1.w), "s"(q2.x), "s"(q2.y), "s"(q2.z), "s"(q2.w),
+                     "s"(q3.x), "s"(q3.y), "s"(q3.z), "s"(q3.w));
         pair(q0);
         pair(q1);
         pair(q2);
