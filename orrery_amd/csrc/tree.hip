@@ -56,6 +56,9 @@ constexpr unsigned kLevels = 21;
 constexpr std::uint8_t kNoBoundary = 22;
 constexpr int kMaxChildren = 8;                 // octants
 constexpr std::uint32_t kNoChild = 0xffffffffu;
+// Spare float4 entries past the last particle of leaf_rel: a leaf's sources are
+// read in rounds of four.
+constexpr std::size_t kLeafPad = 3;
 
 struct WalkNode {
     float4 a;  // centre of mass relative to the root centre (FP32), mass
@@ -651,20 +654,20 @@ __device__ __forceinline__ void leaf_pairs(WalkTarget& t, const double4& cb,
         t.pz = __builtin_fmaf(f, dz, t.pz);
     };
     // Four sources per round, addressed from one base so that their scalar
-    // loads are issued together and share one wait.
-    std::uint32_t j = b;
-    for (; j + 4 <= e; j += 4) {
+    // loads are issued together and share one wait. A last round of fewer than
+    // four still loads four (leaf_rel holds kLeafPad spare entries past the
+    // last particle) and uses those in the leaf: most leaves hold under four.
+    for (std::uint32_t j = b; j < e; j += 4) {
         const float4* q = leaf_rel + j;
         const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
         asm volatile("" ::"s"(q0.x), "s"(q0.y), "s"(q0.z), "s"(q0.w), "s"(q1.x), "s"(q1.y),
                      "s"(q1.z), "s"(q1.w), "s"(q2.x), "s"(q2.y), "s"(q2.z), "s"(q2.w),
                      "s"(q3.x), "s"(q3.y), "s"(q3.z), "s"(q3.w));
         pair(q0);
-        pair(q1);
-        pair(q2);
-        pair(q3);
+        if (j + 1 < e) pair(q1);
+        if (j + 2 < e) pair(q2);
+        if (j + 3 < e) pair(q3);
     }
-    for (; j < e; ++j) pair(leaf_rel[j]);
     kahan(t.px, t.sx, t.kx);
     kahan(t.py, t.sy, t.ky);
     kahan(t.pz, t.sz, t.kz);
@@ -885,7 +888,7 @@ std::size_t node_bound(std::uint64_t n, std::uint64_t cap) {
 // Bytes of one tree arena for n particles, `nodes` node slots and `segments` systems.
 std::size_t layout_bytes(std::uint64_t n, std::size_t nodes, bool quadrupole, std::uint32_t segments) {
     // Every carved array is rounded up to 256 bytes: 22 arrays of padding.
-    const std::size_t per_particle = 2 * n + 3 * 4 * n + 32 * n + 2 * 16 * n +
+    const std::size_t per_particle = 2 * n + 3 * 4 * n + 32 * n + 2 * 16 * n + 16 * kLeafPad +
                                      4 * scan_scratch_words(n);
     const std::size_t per_quad = quadrupole ? 6 * sizeof(double) + sizeof(QuadF) : 0;
     const std::size_t per_node = nodes * (4 * 3 + 1 + 4 * kMaxChildren + 32 + 32 + sizeof(WalkNode) + per_quad);
@@ -908,7 +911,7 @@ Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole,
     L.leaf_of = carve<std::uint32_t>(p, n);
     L.sorted = carve<double4>(p, n);
     L.root_rel = carve<float4>(p, n);
-    L.leaf_rel = carve<float4>(p, n);
+    L.leaf_rel = carve<float4>(p, n + kLeafPad);
     L.scan_scratch = carve<std::uint32_t>(p, scan_scratch_words(n));
     L.begin = carve<std::uint32_t>(p, nodes);
     L.count = carve<std::uint32_t>(p, nodes);
