@@ -629,18 +629,15 @@ __device__ __forceinline__ bool visit(WalkTarget& t, std::uint32_t cur, const Wa
 }
 
 // Particle-particle terms of leaf [b, e) for one target, then the partial is
-// Kahan-added. The target is re-expressed against the leaf's centre cb in FP64.
+// Kahan-added. The target is re-expressed against the leaf's centre cb in FP64
+// (ca: the centre of the target's own leaf).
 template<bool kSafe>
-__device__ __forceinline__ void leaf_pairs(WalkTarget& t, const double4& cb,
-                                           const double4* __restrict__ node_geo,
-                                           const float4* __restrict__ leaf_rel, std::uint32_t b,
-                                           std::uint32_t e, float eps2) {
-    // The target's own leaf centre, read here rather than held through the walk
-    // (a leaf visit is about one iteration in a hundred).
-    const double4 ca = node_geo[t.own];
-    const float tx = t.oi.x - static_cast<float>(cb.x - ca.x);
-    const float ty = t.oi.y - static_cast<float>(cb.y - ca.y);
-    const float tz = t.oi.z - static_cast<float>(cb.z - ca.z);
+__device__ __forceinline__ void leaf_pairs(WalkTarget& t, const double4& cb, double cax, double cay,
+                                           double caz, const float4* __restrict__ leaf_rel,
+                                           std::uint32_t b, std::uint32_t e, float eps2) {
+    const float tx = t.oi.x - static_cast<float>(cb.x - cax);
+    const float ty = t.oi.y - static_cast<float>(cb.y - cay);
+    const float tz = t.oi.z - static_cast<float>(cb.z - caz);
     auto pair = [&](const float4 q) {
         const float dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
         float r2 = __builtin_fmaf(dx, dx, eps2);
@@ -697,6 +694,10 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     WalkTarget t[kR];
     std::uint32_t seg[kR];
     bool valid[kR];
+    // Each lane's targets' own leaf centres, read at every leaf visit: held in
+    // LDS rather than registers (12 KB a block at two targets a lane) or
+    // re-read from memory at each visit. A lane reads only its own slots.
+    __shared__ double s_own[kR][3][kBlock];
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
         const std::uint32_t i = wave * (64u * kR) + 64u * r + lane;
@@ -705,6 +706,10 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
         t[r].xi = root_rel[ic];
         t[r].oi = leaf_rel[ic];
         t[r].own = leaf_of[ic];
+        const double4 ca = node_geo[t[r].own];
+        s_own[r][0][threadIdx.x] = ca.x;
+        s_own[r][1][threadIdx.x] = ca.y;
+        s_own[r][2][threadIdx.x] = ca.z;
         // The target's system starts at its root; the walk records' escape
         // indices end the walk at the system's last subtree.
         seg[r] = seg_of != nullptr ? seg_of[ic] : 0u;
@@ -751,7 +756,10 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
             const std::uint32_t b = nd.b.z, e = nd.b.z + nd.b.w;
 #pragma unroll
             for (int r = 0; r < kR; ++r) {
-                if (leaf[r]) leaf_pairs<kSafe>(t[r], cb, node_geo, leaf_rel, b, e, eps2);
+                if (leaf[r]) {
+                    leaf_pairs<kSafe>(t[r], cb, s_own[r][0][threadIdx.x], s_own[r][1][threadIdx.x],
+                                      s_own[r][2][threadIdx.x], leaf_rel, b, e, eps2);
+                }
             }
         }
         const std::uint32_t step = cur + 1;
