@@ -114,13 +114,18 @@ ORR_API int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* args);
  * targets_per_lane in {1, 2, 4}; refine = 1 applies a Newton step to the
  * hardware reciprocal square root (off by default: v_rsq_f32 already keeps the
  * force within 1e-6 relative); block_target is the number of thread blocks the
- * source range is split to reach (default 32768). targets_per_lane 0,
- * refine -1 and block_target 0 keep the current value. */
+ * source range is split to reach (default 32768). pair_mode 1 (default)
+ * evaluates each far pair of 64-particle tiles once and applies it to both
+ * tiles, where its slots fit (at least 32 tiles; 3 floats per particle and
+ * later block of 8 tiles within 2 GiB, i.e. up to about 420,000 particles);
+ * pair_mode 2 always runs the one-sided kernel, to which targets_per_lane and
+ * block_target apply. targets_per_lane 0, refine -1, block_target 0 and
+ * pair_mode 0 keep the current value. */
 typedef struct {
     int32_t targets_per_lane;
     int32_t refine;
     uint32_t block_target;
-    uint32_t reserved;
+    int32_t pair_mode;
 } orr_direct_config;
 ORR_API int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* config);
 ORR_API int orr_ctx_get_direct_config(orr_ctx* ctx, orr_direct_config* config);

@@ -481,6 +481,306 @@ __global__ __launch_bounds__(256) void batch_reduce(const float* __restrict__ pa
     az[dst] = G * a2;
 }
 
+// ---- pairs: each far tile pair once, applied to both tiles ---------------
+//
+// The one-sided kernel above spends 12 VALU operations and one v_rsq_f32 on
+// every interaction. Gravity is symmetric: the term that pulls target i toward
+// source j, scaled by m_i / m_j, pulls j toward i. Evaluating each tile pair
+// once for both sides costs 16 VALU and one v_rsq_f32 per PAIR, i.e. 8 VALU and
+// half a v_rsq_f32 per interaction:
+//
+//   * Tiles are grouped into blocks of kStay. A wave owns one block: each lane
+//     holds one particle of each of its kStay tiles (the stationary side),
+//     with their sums and Kahan compensations in registers.
+//   * The wave streams the tiles I before its block (a chunk of them per
+//     wave). Lane l loads particle l of I; after every step the streamed
+//     particles and their force accumulators move one lane around the wave
+//     (DPP wave rotate), so in 64 steps every stationary particle meets every
+//     streamed one, and the accumulator of each streamed particle comes home
+//     holding its sum over the block's 64 kStay particles.
+//   * Separations are FP32 offsets from the tile centres, the stationary
+//     offset re-expressed against the streamed tile's centre once per tile
+//     (as the one-sided kernel re-expresses its target). Far pairs only:
+//     near tile pairs (the mask) are skipped here, together with the pairs
+//     inside a block, and left to the complement kernel, which evaluates them
+//     one-sided with the double-float near path.
+//   * Stationary side: the 64 terms of one streamed tile summed in FP32, then
+//     Kahan-added, per item written to its own slot. Streamed side: per step
+//     the kStay terms summed, then added to the travelling accumulator (64
+//     steps), written per (particle, block) to its own slot. A fixed-order
+//     FP64 fold (pairs_reduce) sums a particle's slots.
+//
+// Items: block Q >= 1 streams tiles [0, Q kStay) in chunks of s kStay tiles;
+// items are numbered block by block, items_before(Q) of them before block Q.
+constexpr int kStay = 8;
+
+__host__ __device__ inline std::uint64_t pair_items_before(std::uint32_t Q, std::uint32_t s) {
+    // sum over q < Q of ceil(q / s): q = a s + r
+    if (Q == 0) return 0;
+    const std::uint64_t m = Q - 1, a = m / s, r = m % s;
+    return s * a * (a + 1) / 2 + (a + 1) * r;
+}
+
+__host__ __device__ inline std::uint64_t pair_slot(std::uint32_t Q, std::uint32_t tile) {
+    // Block Q's streamed tiles [0, Q kStay) after those of blocks 1 .. Q - 1.
+    return static_cast<std::uint64_t>(kStay) * Q * (Q - 1) / 2 + tile;
+}
+
+// Every lane takes the value of its neighbour, around the whole wave (DPP
+// wave_ror:1); 64 rotations bring every value home. wave_rotate moves in
+// place; rotate_add(a, t) = rotate(a) + t folds the move into the add.
+__device__ __forceinline__ float wave_rotate(float v) {
+    const int u = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(u, u, 0x13C, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float rotate_add(float a, float t) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, a), 0x13C,
+                                                                 0xf, 0xf, false)) + t;
+}
+
+// Four waves per SIMD (128 VGPRs): the stationary side holds 7 kStay values
+// per lane.
+template<bool kSafe, bool kRefine>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void direct_pairs(const float4* __restrict__ packed,
+                                                       const double4* __restrict__ centres,
+                                                       const std::uint32_t* __restrict__ mask,
+                                                       std::uint32_t tiles, std::uint32_t words,
+                                                       std::uint32_t blocks, std::uint32_t s,
+                                                       std::uint32_t items, float eps2,
+                                                       float* __restrict__ jpart,
+                                                       float* __restrict__ ipart) {
+    const std::uint32_t item =
+        __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kTile) + threadIdx.x / kTile);
+    if (item >= items) return;
+    // The item's block: the largest Q with pair_items_before(Q) <= item.
+    std::uint32_t lo = 1, hi = blocks - 1;
+    while (lo < hi) {
+        const std::uint32_t mid = (lo + hi + 1) / 2;
+        if (pair_items_before(mid, s) <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    const std::uint32_t Q = lo;
+    const std::uint32_t chunk = item - static_cast<std::uint32_t>(pair_items_before(Q, s));
+    const std::uint32_t t0 = Q * kStay;
+    const std::uint32_t i0 = chunk * s * kStay;
+    const std::uint32_t i1 = min(i0 + s * kStay, t0);
+    const unsigned lane = threadIdx.x & (kTile - 1);
+
+    // Stationary tiles past the last (a partial last block) repeat the last
+    // tile massless: they pull nothing and their sums are never read.
+    std::uint32_t stay[kStay];
+    float m[kStay], sx[kStay], sy[kStay], sz[kStay], kx[kStay], ky[kStay], kz[kStay];
+#pragma unroll
+    for (int r = 0; r < kStay; ++r) {
+        stay[r] = min(t0 + r, tiles - 1);
+        m[r] = t0 + r < tiles ? packed[static_cast<std::size_t>(stay[r]) * kTile + lane].w : 0.f;
+        sx[r] = sy[r] = sz[r] = kx[r] = ky[r] = kz[r] = 0.f;
+    }
+
+    for (std::uint32_t I = i0; I < i1; ++I) {
+        const double4 CI = centres[I];
+        const float4 qi = packed[static_cast<std::size_t>(I) * kTile + lane];
+        float xi = qi.x, yi = qi.y, zi = qi.z, mi = qi.w;
+        std::uint32_t near = 0;
+        float wx[kStay], wy[kStay], wz[kStay];
+        // The stationary offsets and centres are re-read for every streamed
+        // tile (cache hits) rather than held: the index is hidden from the
+        // compiler, which would otherwise keep 9 kStay more VGPRs live.
+        std::uint32_t first = t0;
+        asm volatile("" : "+s"(first));
+#pragma unroll
+        for (int r = 0; r < kStay; ++r) {
+            const std::uint32_t tile = min(first + r, tiles - 1);
+            near |= ((mask[static_cast<std::size_t>(stay[r]) * words + I / 32] >> (I & 31u)) & 1u) << r;
+            const double4 C = centres[tile];
+            const float4 o = packed[static_cast<std::size_t>(tile) * kTile + lane];
+            // The stationary offset re-expressed relative to the streamed tile's centre.
+            wx[r] = o.x - static_cast<float>(CI.x - C.x);
+            wy[r] = o.y - static_cast<float>(CI.y - C.y);
+            wz[r] = o.z - static_cast<float>(CI.z - C.z);
+        }
+        float px[kStay], py[kStay], pz[kStay];
+#pragma unroll
+        for (int r = 0; r < kStay; ++r) px[r] = py[r] = pz[r] = 0.f;
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        // d = x_i - x_j; j gains m_i d / r^3, i gains -m_j d / r^3.
+        auto pair = [&](int r, float& tx, float& ty, float& tz) {
+            const float dx = xi - wx[r], dy = yi - wy[r], dz = zi - wz[r];
+            float r2 = __builtin_fmaf(dx, dx, eps2);
+            r2 = __builtin_fmaf(dy, dy, r2);
+            r2 = __builtin_fmaf(dz, dz, r2);
+            if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            float inv3 = (inv * inv) * inv;
+            if constexpr (kRefine) {
+                const float e = __builtin_fmaf(-(r2 * inv), inv, 1.f);
+                inv3 = inv3 * __builtin_fmaf(1.5f, e, 1.f);
+            }
+            const float fj = mi * inv3, fi = m[r] * inv3;
+            px[r] = __builtin_fmaf(fj, dx, px[r]);
+            py[r] = __builtin_fmaf(fj, dy, py[r]);
+            pz[r] = __builtin_fmaf(fj, dz, pz[r]);
+            tx = __builtin_fmaf(-fi, dx, tx);
+            ty = __builtin_fmaf(-fi, dy, ty);
+            tz = __builtin_fmaf(-fi, dz, tz);
+        };
+        auto advance = [&](float tx, float ty, float tz) {
+            ax = rotate_add(ax, tx);
+            ay = rotate_add(ay, ty);
+            az = rotate_add(az, tz);
+            xi = wave_rotate(xi);
+            yi = wave_rotate(yi);
+            zi = wave_rotate(zi);
+            mi = wave_rotate(mi);
+        };
+        if (near == 0) {
+#pragma unroll 1
+            for (int step = 0; step < kTile; ++step) {
+                float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+                for (int r = 0; r < kStay; ++r) pair(r, tx, ty, tz);
+                advance(tx, ty, tz);
+            }
+        } else {
+#pragma unroll 1
+            for (int step = 0; step < kTile; ++step) {
+                float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+                for (int r = 0; r < kStay; ++r) {
+                    if (((near >> r) & 1u) == 0) pair(r, tx, ty, tz);
+                }
+                advance(tx, ty, tz);
+            }
+        }
+        // One more rotation: 64 in all, each accumulator is back in its lane.
+        float* out = ipart + pair_slot(Q, I) * 3 * kTile;
+        out[lane] = wave_rotate(ax);
+        out[kTile + lane] = wave_rotate(ay);
+        out[2 * kTile + lane] = wave_rotate(az);
+#pragma unroll
+        for (int r = 0; r < kStay; ++r) {
+            kahan(px[r], sx[r], kx[r]);
+            kahan(py[r], sy[r], ky[r]);
+            kahan(pz[r], sz[r], kz[r]);
+        }
+    }
+    float* out = jpart + static_cast<std::size_t>(item) * 6 * kStay * kTile;
+    constexpr int kRow = kStay * kTile;
+#pragma unroll
+    for (int r = 0; r < kStay; ++r) {
+        const unsigned k = r * kTile + lane;
+        out[k] = sx[r];
+        out[kRow + k] = sy[r];
+        out[2 * kRow + k] = sz[r];
+        out[3 * kRow + k] = kx[r];
+        out[4 * kRow + k] = ky[r];
+        out[5 * kRow + k] = kz[r];
+    }
+}
+
+// What the pairs kernel leaves out, one-sided: for each target tile, the
+// tiles of its own block and every near tile. A sparse outer tile's box can be
+// near most of the system, so each target tile's source range is cut into
+// `groups` word-aligned spans of `span` tiles, one wave each: wave (t, g)
+// evaluates the own-block and near tiles of span g, in index order, into slot g
+// of cpart[groups][6][npad].
+template<bool kSafe, bool kRefine>
+__global__ __launch_bounds__(kBlock) void direct_complement(const float4* __restrict__ packed,
+                                                            const float4* __restrict__ packed_lo,
+                                                            const double4* __restrict__ centres,
+                                                            const float4* __restrict__ extents,
+                                                            const std::uint32_t* __restrict__ mask,
+                                                            std::uint32_t tiles, std::uint32_t words,
+                                                            std::uint32_t groups, std::uint32_t span,
+                                                            float eps2, float* __restrict__ cpart) {
+    const std::uint32_t w =
+        __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kTile) + threadIdx.x / kTile);
+    if (w >= tiles * groups) return;
+    const std::uint32_t t = w / groups, g = w % groups;
+    const unsigned lane = threadIdx.x & (kTile - 1);
+    const std::size_t k = static_cast<std::size_t>(t) * kTile + lane;
+    const float4 o = packed[k];
+    const float4 l = packed_lo[k];
+    const double4 c = centres[t];
+    const float ox[1] = {o.x}, oy[1] = {o.y}, oz[1] = {o.z};
+    const float lx[1] = {l.x}, ly[1] = {l.y}, lz[1] = {l.z};
+    const double cx[1] = {c.x}, cy[1] = {c.y}, cz[1] = {c.z};
+    const float4 ce[1] = {extents[t]};
+    float sx[1] = {0.f}, sy[1] = {0.f}, sz[1] = {0.f}, kx[1] = {0.f}, ky[1] = {0.f}, kz[1] = {0.f};
+    const std::uint32_t* row = mask + static_cast<std::size_t>(t) * words;
+    const std::uint32_t b0 = t / kStay * kStay, b1 = min(b0 + kStay, tiles);
+    const std::uint32_t j0 = g * span, j1 = min(j0 + span, tiles);
+    for (std::uint32_t wd = j0 / 32; wd * 32 < j1; ++wd) {
+        const std::uint32_t word = row[wd];
+        // Own-block tiles in this word, near or not, and near tiles.
+        std::uint32_t own = 0;
+        for (std::uint32_t J = max(b0, wd * 32); J < min(b1, wd * 32 + 32); ++J) own |= 1u << (J & 31u);
+        std::uint32_t bits = word | own;
+        while (bits != 0) {
+            const std::uint32_t J = wd * 32 + static_cast<std::uint32_t>(__builtin_ctz(bits));
+            bits &= bits - 1;
+            const bool near = ((word >> (J & 31u)) & 1u) != 0;
+            source_tile<1, kSafe, kRefine, true>(packed, packed_lo, centres, extents, J, near, eps2,
+                                                 ox, oy, oz, lx, ly, lz, cx, cy, cz, ce, sx, sy, sz,
+                                                 kx, ky, kz);
+        }
+    }
+    const std::size_t np = static_cast<std::size_t>(tiles) * kTile;
+    float* out = cpart + static_cast<std::size_t>(g) * 6 * np;
+    out[k] = sx[0];
+    out[np + k] = sy[0];
+    out[2 * np + k] = sz[0];
+    out[3 * np + k] = kx[0];
+    out[4 * np + k] = ky[0];
+    out[5 * np + k] = kz[0];
+}
+
+// A particle's slots in a fixed order, in FP64: its block's items (sum minus
+// compensation), its tile's streamed sums for every later block, the
+// complement's groups; times G, scattered through `order`.
+__global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jpart,
+                                                    const float* __restrict__ ipart,
+                                                    const float* __restrict__ cpart,
+                                                    std::uint32_t tiles, std::uint32_t blocks,
+                                                    std::uint32_t s, std::uint32_t groups,
+                                                    std::uint64_t n,
+                                                    const std::uint32_t* __restrict__ order,
+                                                    double G, double* __restrict__ ax,
+                                                    double* __restrict__ ay,
+                                                    double* __restrict__ az) {
+    const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const std::uint32_t t = static_cast<std::uint32_t>(k / kTile), lane = static_cast<std::uint32_t>(k % kTile);
+    const std::uint32_t P = t / kStay, r = t % kStay;
+    constexpr std::size_t kRow = kStay * kTile;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const std::uint64_t first = pair_items_before(P, s);
+    const std::uint32_t count = (P + s - 1) / s;
+    for (std::uint32_t u = 0; u < count; ++u) {
+        const float* p = jpart + (first + u) * 6 * kRow + r * kTile + lane;
+        a0 += static_cast<double>(p[0]) - static_cast<double>(p[3 * kRow]);
+        a1 += static_cast<double>(p[kRow]) - static_cast<double>(p[4 * kRow]);
+        a2 += static_cast<double>(p[2 * kRow]) - static_cast<double>(p[5 * kRow]);
+    }
+    for (std::uint32_t Q = P + 1; Q < blocks; ++Q) {
+        const float* p = ipart + pair_slot(Q, t) * 3 * kTile + lane;
+        a0 += static_cast<double>(p[0]);
+        a1 += static_cast<double>(p[kTile]);
+        a2 += static_cast<double>(p[2 * kTile]);
+    }
+    const std::size_t np = static_cast<std::size_t>(tiles) * kTile;
+    for (std::uint32_t g = 0; g < groups; ++g) {
+        const float* p = cpart + static_cast<std::size_t>(g) * 6 * np + k;
+        a0 += static_cast<double>(p[0]) - static_cast<double>(p[3 * np]);
+        a1 += static_cast<double>(p[np]) - static_cast<double>(p[4 * np]);
+        a2 += static_cast<double>(p[2 * np]) - static_cast<double>(p[5 * np]);
+    }
+    const std::uint64_t dst = order != nullptr ? order[k] : k;
+    ax[dst] = G * a0;
+    ay[dst] = G * a1;
+    az[dst] = G * a2;
+}
+
 template<int R, bool kSafe, bool kRefine, bool kMask>
 void launch_tiles(orr_ctx* ctx, std::uint32_t blocks_i, std::uint32_t splits, const float4* packed,
                   const float4* packed_lo, const double4* centres, const float4* extents,
@@ -508,6 +808,61 @@ template<int R> TilesLauncher pick(bool safe, bool refine, bool mask) {
 // beyond it the kernels test each pair themselves.
 constexpr std::uint32_t kMaskTiles = 32768;
 
+// The pairs path: blocks of kStay tiles, chunks of s blocks per item, slot
+// counts. Taken when the mask exists, there are at least four blocks and the
+// streamed-side slots (3 floats per particle and later block) fit in 2 GiB;
+// otherwise, or when configured off, the one-sided kernel runs.
+struct PairPlan {
+    bool use = false;
+    std::uint32_t blocks = 0, s = 1, groups = 1, span = 32;
+    std::uint64_t items = 0, slots = 0;
+    std::size_t jpart_floats = 0, ipart_floats = 0;
+};
+
+constexpr std::size_t kPairSlotBytes = std::size_t{1} << 31;
+// Items wanted per evaluation: about four rounds of 4096 resident waves.
+constexpr std::uint64_t kPairItems = 16384;
+// Source spans per target tile in the complement.
+constexpr std::uint32_t kComplementGroups = 16;
+
+PairPlan pair_plan(std::uint32_t tiles, bool masked, int mode) {
+    PairPlan p;
+    if (mode != 1 || !masked) return p;
+    p.blocks = (tiles + kStay - 1) / kStay;
+    if (p.blocks < 4) return p;
+    p.slots = pair_slot(p.blocks, 0);
+    p.ipart_floats = static_cast<std::size_t>(p.slots) * 3 * kTile;
+    if (p.ipart_floats * sizeof(float) > kPairSlotBytes) return p;
+    const std::uint64_t B = p.blocks;
+    p.s = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, B * B / (2 * kPairItems)));
+    p.items = pair_items_before(p.blocks, p.s);
+    p.jpart_floats = static_cast<std::size_t>(p.items) * 6 * kStay * kTile;
+    // Complement spans: whole mask words, at most kComplementGroups of them.
+    const std::uint32_t words = (tiles + 31) / 32;
+    p.span = 32 * ((words + kComplementGroups - 1) / kComplementGroups);
+    p.groups = (tiles + p.span - 1) / p.span;
+    p.use = true;
+    return p;
+}
+
+template<bool kSafe, bool kRefine>
+void launch_pairs(orr_ctx* ctx, const PairPlan& plan, const float4* packed, const float4* packed_lo,
+                  const double4* centres, const float4* extents, const std::uint32_t* mask,
+                  std::uint32_t tiles, std::uint32_t words, float eps2, float* jpart, float* ipart,
+                  float* cpart) {
+    constexpr std::uint32_t kWaves = kBlock / kTile;
+    const std::uint32_t items = static_cast<std::uint32_t>(plan.items);
+    hipLaunchKernelGGL((direct_pairs<kSafe, kRefine>), dim3((items + kWaves - 1) / kWaves),
+                       dim3(kBlock), 0, ctx->stream, packed, centres, mask, tiles, words, plan.blocks,
+                       plan.s, items, eps2, jpart, ipart);
+    ORR_LAUNCHED("direct_pairs");
+    const std::uint32_t waves = tiles * plan.groups;
+    hipLaunchKernelGGL((direct_complement<kSafe, kRefine>), dim3((waves + kWaves - 1) / kWaves),
+                       dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, extents, mask, tiles,
+                       words, plan.groups, plan.span, eps2, cpart);
+    ORR_LAUNCHED("direct_complement");
+}
+
 }  // namespace
 
 void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order) {
@@ -525,16 +880,20 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
     const std::uint32_t per_split = (tiles + splits - 1) / splits;
     splits = (tiles + per_split - 1) / per_split;
 
+    const bool masked = tiles <= kMaskTiles;
+    const PairPlan plan = pair_plan(tiles, masked, ctx->direct_pair_mode);
+    const std::size_t cpart_floats = 6 * static_cast<std::size_t>(npad) * plan.groups;
     ctx->direct_packed.reserve(2 * sizeof(float4) * npad);
     ctx->direct_centres.reserve((sizeof(double4) + sizeof(float4)) * tiles);
-    ctx->direct_partials.reserve(sizeof(float) * 6 * static_cast<std::size_t>(npad) * splits);
+    ctx->direct_partials.reserve(
+        sizeof(float) * (plan.use ? plan.jpart_floats + plan.ipart_floats + cpart_floats
+                                  : 6 * static_cast<std::size_t>(npad) * splits));
     float4* packed = ctx->direct_packed.as<float4>();
     float4* packed_lo = packed + npad;
     double4* centres = ctx->direct_centres.as<double4>();
     float4* extents = reinterpret_cast<float4*>(centres + tiles);
     float* partials = ctx->direct_partials.as<float>();
 
-    const bool masked = tiles <= kMaskTiles;
     const std::uint32_t words = (tiles + 31) / 32;
     std::uint32_t* mask = nullptr;
     if (masked) {
@@ -553,10 +912,31 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
         }
     }
     const float eps2 = static_cast<float>(a.softening_squared);
+    const bool safe = !(a.softening_squared > 0);
+    const bool refine = ctx->direct_refine != 0;
+    if (plan.use) {
+        float* jpart = partials;
+        float* ipart = jpart + plan.jpart_floats;
+        float* cpart = ipart + plan.ipart_floats;
+        {
+            PhaseTimer t(ctx, kDirectKernel);
+            auto launch = safe ? (refine ? launch_pairs<true, true> : launch_pairs<true, false>)
+                               : (refine ? launch_pairs<false, true> : launch_pairs<false, false>);
+            launch(ctx, plan, packed, packed_lo, centres, extents, mask, tiles, words, eps2, jpart,
+                   ipart, cpart);
+        }
+        {
+            PhaseTimer t(ctx, kDirectReduce);
+            const std::uint32_t blocks = static_cast<std::uint32_t>((n + 255) / 256);
+            hipLaunchKernelGGL(pairs_reduce, dim3(blocks), dim3(256), 0, ctx->stream, jpart, ipart,
+                               cpart, tiles, plan.blocks, plan.s, plan.groups, n, order,
+                               a.gravitational_constant, a.ax, a.ay, a.az);
+            ORR_LAUNCHED("pairs_reduce");
+        }
+        return;
+    }
     {
         PhaseTimer t(ctx, kDirectKernel);
-        const bool safe = !(a.softening_squared > 0);
-        const bool refine = ctx->direct_refine != 0;
         TilesLauncher launch = R == 1 ? pick<1>(safe, refine, masked)
                              : R == 4 ? pick<4>(safe, refine, masked)
                                       : pick<2>(safe, refine, masked);
