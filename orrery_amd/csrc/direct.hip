@@ -500,15 +500,16 @@ __global__ __launch_bounds__(256) void batch_reduce(const float* __restrict__ pa
 //     holding its sum over the block's 64 kStay particles.
 //   * Separations are FP32 offsets from the tile centres, the stationary
 //     offset re-expressed against the streamed tile's centre once per tile
-//     (as the one-sided kernel re-expresses its target). Far pairs only:
-//     near tile pairs (the mask) are skipped here, together with the pairs
-//     inside a block, and left to the complement kernel, which evaluates them
-//     one-sided with the double-float near path.
+//     (as the one-sided kernel re-expresses its target). Near tile pairs
+//     (the mask) are skipped in that pass and get a pass of their own with the
+//     one-sided kernel's double-float separation. The pairs inside a block are
+//     left to the complement kernel, one-sided.
 //   * Stationary side: the 64 terms of one streamed tile summed in FP32, then
 //     Kahan-added, per item written to its own slot. Streamed side: per step
 //     the kStay terms summed, then added to the travelling accumulator (64
-//     steps), written per (particle, block) to its own slot. A fixed-order
-//     FP64 fold (pairs_reduce) sums a particle's slots.
+//     steps), written per (particle, block) to its own FP64 slot (the near
+//     passes' sums are Kahan-added into it). A fixed-order FP64 fold
+//     (pairs_reduce) sums a particle's slots.
 //
 // Items: block Q >= 1 streams tiles [0, Q kStay) in chunks of s kStay tiles;
 // items are numbered block by block, items_before(Q) of them before block Q.
@@ -542,13 +543,14 @@ __device__ __forceinline__ float rotate_add(float a, float t) {
 // per lane.
 template<bool kSafe, bool kRefine>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void direct_pairs(const float4* __restrict__ packed,
+                                                       const float4* __restrict__ packed_lo,
                                                        const double4* __restrict__ centres,
                                                        const std::uint32_t* __restrict__ mask,
                                                        std::uint32_t tiles, std::uint32_t words,
                                                        std::uint32_t blocks, std::uint32_t s,
                                                        std::uint32_t items, float eps2,
                                                        float* __restrict__ jpart,
-                                                       float* __restrict__ ipart) {
+                                                       double* __restrict__ ipart) {
     const std::uint32_t item =
         __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kTile) + threadIdx.x / kTile);
     if (item >= items) return;
@@ -602,7 +604,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
         float px[kStay], py[kStay], pz[kStay];
 #pragma unroll
         for (int r = 0; r < kStay; ++r) px[r] = py[r] = pz[r] = 0.f;
-        float ax = 0.f, ay = 0.f, az = 0.f;
+        float ax = 0.f, ay = 0.f, az = 0.f, cax = 0.f, cay = 0.f, caz = 0.f;
         // d = x_i - x_j; j gains m_i d / r^3, i gains -m_j d / r^3.
         auto pair = [&](int r, float& tx, float& ty, float& tz) {
             const float dx = xi - wx[r], dy = yi - wy[r], dz = zi - wz[r];
@@ -652,11 +654,77 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
                 advance(tx, ty, tz);
             }
         }
-        // One more rotation: 64 in all, each accumulator is back in its lane.
-        float* out = ipart + pair_slot(Q, I) * 3 * kTile;
-        out[lane] = wave_rotate(ax);
-        out[kTile + lane] = wave_rotate(ay);
-        out[2 * kTile + lane] = wave_rotate(az);
+        // After a pass of 64 steps the streamed particles are home and each
+        // accumulator sits one lane short of home, which is where the next
+        // pass's first rotate_add expects it; one more rotation before the
+        // store brings it home.
+        if (near != 0) {
+            // Near stationary tiles, one pass of 64 steps each, with the
+            // one-sided kernel's double-float separation: the streamed offset
+            // (hi, lo) against the stationary position taken relative to the
+            // streamed tile's centre in FP64 and split into (hi, lo).
+            const float4 il = packed_lo[static_cast<std::size_t>(I) * kTile + lane];
+            float xl = il.x, yl = il.y, zl = il.z;
+#pragma unroll
+            for (int r = 0; r < kStay; ++r) {
+                if (((near >> r) & 1u) == 0) continue;
+                const std::uint32_t tile = min(first + r, tiles - 1);
+                const double4 C = centres[tile];
+                const float4 o = packed[static_cast<std::size_t>(tile) * kTile + lane];
+                const float4 l = packed_lo[static_cast<std::size_t>(tile) * kTile + lane];
+                const double t[3] = {(static_cast<double>(o.x) + l.x) - (CI.x - C.x),
+                                     (static_cast<double>(o.y) + l.y) - (CI.y - C.y),
+                                     (static_cast<double>(o.z) + l.z) - (CI.z - C.z)};
+                float th[3], tl[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    th[c] = static_cast<float>(t[c]);
+                    tl[c] = static_cast<float>(t[c] - th[c]);
+                }
+                // The tile's 64 terms are summed on their own (as the one-sided
+                // kernel sums a tile) and added once.
+                float nx = 0.f, ny = 0.f, nz = 0.f;
+#pragma unroll 1
+                for (int step = 0; step < kTile; ++step) {
+                    const float dx = (xi - th[0]) + (xl - tl[0]);
+                    const float dy = (yi - th[1]) + (yl - tl[1]);
+                    const float dz = (zi - th[2]) + (zl - tl[2]);
+                    float r2 = __builtin_fmaf(dx, dx, eps2);
+                    r2 = __builtin_fmaf(dy, dy, r2);
+                    r2 = __builtin_fmaf(dz, dz, r2);
+                    if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    float inv3 = (inv * inv) * inv;
+                    if constexpr (kRefine) {
+                        const float e = __builtin_fmaf(-(r2 * inv), inv, 1.f);
+                        inv3 = inv3 * __builtin_fmaf(1.5f, e, 1.f);
+                    }
+                    const float fj = mi * inv3, fi = m[r] * inv3;
+                    px[r] = __builtin_fmaf(fj, dx, px[r]);
+                    py[r] = __builtin_fmaf(fj, dy, py[r]);
+                    pz[r] = __builtin_fmaf(fj, dz, pz[r]);
+                    nx = rotate_add(nx, -fi * dx);
+                    ny = rotate_add(ny, -fi * dy);
+                    nz = rotate_add(nz, -fi * dz);
+                    xi = wave_rotate(xi);
+                    yi = wave_rotate(yi);
+                    zi = wave_rotate(zi);
+                    mi = wave_rotate(mi);
+                    xl = wave_rotate(xl);
+                    yl = wave_rotate(yl);
+                    zl = wave_rotate(zl);
+                }
+                // Both sums are one lane short of home: Kahan-added lane by lane.
+                kahan(nx, ax, cax);
+                kahan(ny, ay, cay);
+                kahan(nz, az, caz);
+            }
+        }
+        // The slot holds sum - compensation in FP64.
+        double* out = ipart + pair_slot(Q, I) * 3 * kTile;
+        out[lane] = static_cast<double>(wave_rotate(ax)) - static_cast<double>(wave_rotate(cax));
+        out[kTile + lane] = static_cast<double>(wave_rotate(ay)) - static_cast<double>(wave_rotate(cay));
+        out[2 * kTile + lane] = static_cast<double>(wave_rotate(az)) - static_cast<double>(wave_rotate(caz));
 #pragma unroll
         for (int r = 0; r < kStay; ++r) {
             kahan(px[r], sx[r], kx[r]);
@@ -678,12 +746,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     }
 }
 
-// What the pairs kernel leaves out, one-sided: for each target tile, the
-// tiles of its own block and every near tile. A sparse outer tile's box can be
-// near most of the system, so each target tile's source range is cut into
-// `groups` word-aligned spans of `span` tiles, one wave each: wave (t, g)
-// evaluates the own-block and near tiles of span g, in index order, into slot g
-// of cpart[groups][6][npad].
+// What the pairs kernel leaves out: the pairs inside each block, one-sided.
+// One wave per target tile over the tiles of its own block (near flags from
+// the mask); sums and compensations to cpart[6][npad].
 template<bool kSafe, bool kRefine>
 __global__ __launch_bounds__(kBlock) void direct_complement(const float4* __restrict__ packed,
                                                             const float4* __restrict__ packed_lo,
@@ -691,12 +756,10 @@ __global__ __launch_bounds__(kBlock) void direct_complement(const float4* __rest
                                                             const float4* __restrict__ extents,
                                                             const std::uint32_t* __restrict__ mask,
                                                             std::uint32_t tiles, std::uint32_t words,
-                                                            std::uint32_t groups, std::uint32_t span,
                                                             float eps2, float* __restrict__ cpart) {
-    const std::uint32_t w =
+    const std::uint32_t t =
         __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kTile) + threadIdx.x / kTile);
-    if (w >= tiles * groups) return;
-    const std::uint32_t t = w / groups, g = w % groups;
+    if (t >= tiles) return;
     const unsigned lane = threadIdx.x & (kTile - 1);
     const std::size_t k = static_cast<std::size_t>(t) * kTile + lane;
     const float4 o = packed[k];
@@ -709,41 +772,28 @@ __global__ __launch_bounds__(kBlock) void direct_complement(const float4* __rest
     float sx[1] = {0.f}, sy[1] = {0.f}, sz[1] = {0.f}, kx[1] = {0.f}, ky[1] = {0.f}, kz[1] = {0.f};
     const std::uint32_t* row = mask + static_cast<std::size_t>(t) * words;
     const std::uint32_t b0 = t / kStay * kStay, b1 = min(b0 + kStay, tiles);
-    const std::uint32_t j0 = g * span, j1 = min(j0 + span, tiles);
-    for (std::uint32_t wd = j0 / 32; wd * 32 < j1; ++wd) {
-        const std::uint32_t word = row[wd];
-        // Own-block tiles in this word, near or not, and near tiles.
-        std::uint32_t own = 0;
-        for (std::uint32_t J = max(b0, wd * 32); J < min(b1, wd * 32 + 32); ++J) own |= 1u << (J & 31u);
-        std::uint32_t bits = word | own;
-        while (bits != 0) {
-            const std::uint32_t J = wd * 32 + static_cast<std::uint32_t>(__builtin_ctz(bits));
-            bits &= bits - 1;
-            const bool near = ((word >> (J & 31u)) & 1u) != 0;
-            source_tile<1, kSafe, kRefine, true>(packed, packed_lo, centres, extents, J, near, eps2,
-                                                 ox, oy, oz, lx, ly, lz, cx, cy, cz, ce, sx, sy, sz,
-                                                 kx, ky, kz);
-        }
+    for (std::uint32_t J = b0; J < b1; ++J) {
+        const bool near = ((row[J / 32] >> (J & 31u)) & 1u) != 0;
+        source_tile<1, kSafe, kRefine, true>(packed, packed_lo, centres, extents, J, near, eps2, ox,
+                                             oy, oz, lx, ly, lz, cx, cy, cz, ce, sx, sy, sz, kx, ky, kz);
     }
     const std::size_t np = static_cast<std::size_t>(tiles) * kTile;
-    float* out = cpart + static_cast<std::size_t>(g) * 6 * np;
-    out[k] = sx[0];
-    out[np + k] = sy[0];
-    out[2 * np + k] = sz[0];
-    out[3 * np + k] = kx[0];
-    out[4 * np + k] = ky[0];
-    out[5 * np + k] = kz[0];
+    cpart[k] = sx[0];
+    cpart[np + k] = sy[0];
+    cpart[2 * np + k] = sz[0];
+    cpart[3 * np + k] = kx[0];
+    cpart[4 * np + k] = ky[0];
+    cpart[5 * np + k] = kz[0];
 }
 
 // A particle's slots in a fixed order, in FP64: its block's items (sum minus
 // compensation), its tile's streamed sums for every later block, the
-// complement's groups; times G, scattered through `order`.
+// complement; times G, scattered through `order`.
 __global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jpart,
-                                                    const float* __restrict__ ipart,
+                                                    const double* __restrict__ ipart,
                                                     const float* __restrict__ cpart,
                                                     std::uint32_t tiles, std::uint32_t blocks,
-                                                    std::uint32_t s, std::uint32_t groups,
-                                                    std::uint64_t n,
+                                                    std::uint32_t s, std::uint64_t n,
                                                     const std::uint32_t* __restrict__ order,
                                                     double G, double* __restrict__ ax,
                                                     double* __restrict__ ay,
@@ -763,18 +813,15 @@ __global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jp
         a2 += static_cast<double>(p[2 * kRow]) - static_cast<double>(p[5 * kRow]);
     }
     for (std::uint32_t Q = P + 1; Q < blocks; ++Q) {
-        const float* p = ipart + pair_slot(Q, t) * 3 * kTile + lane;
-        a0 += static_cast<double>(p[0]);
-        a1 += static_cast<double>(p[kTile]);
-        a2 += static_cast<double>(p[2 * kTile]);
+        const double* p = ipart + pair_slot(Q, t) * 3 * kTile + lane;
+        a0 += p[0];
+        a1 += p[kTile];
+        a2 += p[2 * kTile];
     }
     const std::size_t np = static_cast<std::size_t>(tiles) * kTile;
-    for (std::uint32_t g = 0; g < groups; ++g) {
-        const float* p = cpart + static_cast<std::size_t>(g) * 6 * np + k;
-        a0 += static_cast<double>(p[0]) - static_cast<double>(p[3 * np]);
-        a1 += static_cast<double>(p[np]) - static_cast<double>(p[4 * np]);
-        a2 += static_cast<double>(p[2 * np]) - static_cast<double>(p[5 * np]);
-    }
+    a0 += static_cast<double>(cpart[k]) - static_cast<double>(cpart[3 * np + k]);
+    a1 += static_cast<double>(cpart[np + k]) - static_cast<double>(cpart[4 * np + k]);
+    a2 += static_cast<double>(cpart[2 * np + k]) - static_cast<double>(cpart[5 * np + k]);
     const std::uint64_t dst = order != nullptr ? order[k] : k;
     ax[dst] = G * a0;
     ay[dst] = G * a1;
@@ -810,20 +857,18 @@ constexpr std::uint32_t kMaskTiles = 32768;
 
 // The pairs path: blocks of kStay tiles, chunks of s blocks per item, slot
 // counts. Taken when the mask exists, there are at least four blocks and the
-// streamed-side slots (3 floats per particle and later block) fit in 2 GiB;
+// streamed-side slots (3 doubles per particle and later block) fit in 4 GiB;
 // otherwise, or when configured off, the one-sided kernel runs.
 struct PairPlan {
     bool use = false;
-    std::uint32_t blocks = 0, s = 1, groups = 1, span = 32;
+    std::uint32_t blocks = 0, s = 1;
     std::uint64_t items = 0, slots = 0;
-    std::size_t jpart_floats = 0, ipart_floats = 0;
+    std::size_t jpart_floats = 0, ipart_doubles = 0;
 };
 
-constexpr std::size_t kPairSlotBytes = std::size_t{1} << 31;
+constexpr std::size_t kPairSlotBytes = std::size_t{1} << 32;
 // Items wanted per evaluation: about four rounds of 4096 resident waves.
 constexpr std::uint64_t kPairItems = 16384;
-// Source spans per target tile in the complement.
-constexpr std::uint32_t kComplementGroups = 16;
 
 PairPlan pair_plan(std::uint32_t tiles, bool masked, int mode) {
     PairPlan p;
@@ -831,16 +876,12 @@ PairPlan pair_plan(std::uint32_t tiles, bool masked, int mode) {
     p.blocks = (tiles + kStay - 1) / kStay;
     if (p.blocks < 4) return p;
     p.slots = pair_slot(p.blocks, 0);
-    p.ipart_floats = static_cast<std::size_t>(p.slots) * 3 * kTile;
-    if (p.ipart_floats * sizeof(float) > kPairSlotBytes) return p;
+    p.ipart_doubles = static_cast<std::size_t>(p.slots) * 3 * kTile;
+    if (p.ipart_doubles * sizeof(double) > kPairSlotBytes) return p;
     const std::uint64_t B = p.blocks;
     p.s = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, B * B / (2 * kPairItems)));
     p.items = pair_items_before(p.blocks, p.s);
     p.jpart_floats = static_cast<std::size_t>(p.items) * 6 * kStay * kTile;
-    // Complement spans: whole mask words, at most kComplementGroups of them.
-    const std::uint32_t words = (tiles + 31) / 32;
-    p.span = 32 * ((words + kComplementGroups - 1) / kComplementGroups);
-    p.groups = (tiles + p.span - 1) / p.span;
     p.use = true;
     return p;
 }
@@ -848,18 +889,17 @@ PairPlan pair_plan(std::uint32_t tiles, bool masked, int mode) {
 template<bool kSafe, bool kRefine>
 void launch_pairs(orr_ctx* ctx, const PairPlan& plan, const float4* packed, const float4* packed_lo,
                   const double4* centres, const float4* extents, const std::uint32_t* mask,
-                  std::uint32_t tiles, std::uint32_t words, float eps2, float* jpart, float* ipart,
+                  std::uint32_t tiles, std::uint32_t words, float eps2, float* jpart, double* ipart,
                   float* cpart) {
     constexpr std::uint32_t kWaves = kBlock / kTile;
     const std::uint32_t items = static_cast<std::uint32_t>(plan.items);
     hipLaunchKernelGGL((direct_pairs<kSafe, kRefine>), dim3((items + kWaves - 1) / kWaves),
-                       dim3(kBlock), 0, ctx->stream, packed, centres, mask, tiles, words, plan.blocks,
+                       dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, mask, tiles, words, plan.blocks,
                        plan.s, items, eps2, jpart, ipart);
     ORR_LAUNCHED("direct_pairs");
-    const std::uint32_t waves = tiles * plan.groups;
-    hipLaunchKernelGGL((direct_complement<kSafe, kRefine>), dim3((waves + kWaves - 1) / kWaves),
+    hipLaunchKernelGGL((direct_complement<kSafe, kRefine>), dim3((tiles + kWaves - 1) / kWaves),
                        dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, extents, mask, tiles,
-                       words, plan.groups, plan.span, eps2, cpart);
+                       words, eps2, cpart);
     ORR_LAUNCHED("direct_complement");
 }
 
@@ -882,12 +922,12 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
 
     const bool masked = tiles <= kMaskTiles;
     const PairPlan plan = pair_plan(tiles, masked, ctx->direct_pair_mode);
-    const std::size_t cpart_floats = 6 * static_cast<std::size_t>(npad) * plan.groups;
+    const std::size_t cpart_floats = 6 * static_cast<std::size_t>(npad);
     ctx->direct_packed.reserve(2 * sizeof(float4) * npad);
     ctx->direct_centres.reserve((sizeof(double4) + sizeof(float4)) * tiles);
     ctx->direct_partials.reserve(
-        sizeof(float) * (plan.use ? plan.jpart_floats + plan.ipart_floats + cpart_floats
-                                  : 6 * static_cast<std::size_t>(npad) * splits));
+        plan.use ? sizeof(double) * plan.ipart_doubles + sizeof(float) * (plan.jpart_floats + cpart_floats)
+                 : sizeof(float) * 6 * static_cast<std::size_t>(npad) * splits);
     float4* packed = ctx->direct_packed.as<float4>();
     float4* packed_lo = packed + npad;
     double4* centres = ctx->direct_centres.as<double4>();
@@ -915,9 +955,10 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
     const bool safe = !(a.softening_squared > 0);
     const bool refine = ctx->direct_refine != 0;
     if (plan.use) {
-        float* jpart = partials;
-        float* ipart = jpart + plan.jpart_floats;
-        float* cpart = ipart + plan.ipart_floats;
+        // Doubles first: every part stays aligned to its type.
+        double* ipart = reinterpret_cast<double*>(partials);
+        float* jpart = reinterpret_cast<float*>(ipart + plan.ipart_doubles);
+        float* cpart = jpart + plan.jpart_floats;
         {
             PhaseTimer t(ctx, kDirectKernel);
             auto launch = safe ? (refine ? launch_pairs<true, true> : launch_pairs<true, false>)
@@ -929,7 +970,7 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
             PhaseTimer t(ctx, kDirectReduce);
             const std::uint32_t blocks = static_cast<std::uint32_t>((n + 255) / 256);
             hipLaunchKernelGGL(pairs_reduce, dim3(blocks), dim3(256), 0, ctx->stream, jpart, ipart,
-                               cpart, tiles, plan.blocks, plan.s, plan.groups, n, order,
+                               cpart, tiles, plan.blocks, plan.s, n, order,
                                a.gravitational_constant, a.ax, a.ay, a.az);
             ORR_LAUNCHED("pairs_reduce");
         }
