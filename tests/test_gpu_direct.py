@@ -217,16 +217,17 @@ def test_direct_config_round_trip(ctx):
 
 # Pair mode 1 evaluates each far pair of tiles once for both tiles (blocks of
 # 8 stationary tiles, streamed tiles rotating around the wave); pair mode 2 is
-# the one-sided kernel. Sizes: the smallest that takes the pairs path (32
-# tiles = 4 blocks), a ragged one (a partial last block of 3 tiles and a
-# partial last tile), and one with several items per block. The worst
+# the one-sided kernel. Sizes: the smallest that takes the pairs path (25
+# tiles = 4 blocks, the last with one tile of one particle), a ragged one (a
+# partial last block of 3 tiles and a partial last tile), and one with several
+# items per block. The worst
 # particle of these small softened systems sits near 1e-6 in either mode; the
 # pairs path's streamed side sums the terms of eight tiles per step, which
 # puts its worst particle up to some 60 % above the one-sided kernel's at these
 # sizes (RMS alike). Here RMS <= 1e-6, worst <= 2e-6 and the pairs path's
 # worst within twice the one-sided kernel's; the 1e-6 worst bound of the
 # benchmark configurations is held by the tests above.
-@pytest.mark.parametrize("n", [2048, 64 * (8 * 5 + 3) - 47, 20000])
+@pytest.mark.parametrize("n", [64 * 24 + 1, 64 * (8 * 5 + 3) - 47, 20000])
 @pytest.mark.parametrize("eps", [0.0, 0.01])
 def test_pair_modes_against_fp64_reference(ctx, n, eps):
     p = oracle.plummer(n, 11)
@@ -292,3 +293,42 @@ def test_replayed_ordering_is_bitwise(ctx):
     ref = [np.zeros(q.n) for _ in range(3)]
     fresh.evaluate((q.x, q.y, q.z), q.m, ref)
     assert all(np.array_equal(acc[k], ref[k]) for k in range(3))
+
+
+def test_pair_mode_with_newton_step(ctx):
+    # The Newton-refined reciprocal square root in the pairs kernel (its own
+    # template instance) against the FP64 sum.
+    p = oracle.plummer(20000, 13)
+    eps = 0.01
+    rx, ry, rz = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps)
+    try:
+        ctx.set_direct_config(refine=1, pair_mode=1)
+        (ax, ay, az), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
+    finally:
+        ctx.set_direct_config(refine=0, pair_mode=1)
+    worst, rms = oracle.error_summary(oracle.relative_errors(ax, ay, az, rx, ry, rz))
+    print(f"refined pairs worst={worst:.3e} rms={rms:.3e}")
+    assert rms <= RMS_BOUND and worst <= 2 * WORST_BOUND, (rms, worst)
+
+
+def test_pair_mode_beyond_slot_budget_is_one_sided(ctx):
+    # Above 428,032 particles the streamed-side slots would pass 4 GiB and pair
+    # mode 1 runs the one-sided kernel: bit for bit pair mode 2's result, and
+    # within the direct bounds of the FP64 sum on sampled targets.
+    n, eps = 428032 + 64, 0.005
+    p = oracle.plummer(n, 14)
+    out = {}
+    try:
+        for mode in (1, 2):
+            ctx.set_direct_config(pair_mode=mode)
+            out[mode], _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
+    finally:
+        ctx.set_direct_config(pair_mode=1)
+    assert all(np.array_equal(out[1][k], out[2][k]) for k in range(3))
+    rng = np.random.default_rng(14)
+    tgt = np.sort(rng.choice(n, 512, replace=False)).astype(np.uint64)
+    rx, ry, rz = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps, targets=tgt)
+    i = tgt.astype(np.int64)
+    ax, ay, az = out[1]
+    worst, rms = oracle.error_summary(oracle.relative_errors(ax[i], ay[i], az[i], rx, ry, rz))
+    assert rms <= RMS_BOUND and worst <= WORST_BOUND, (rms, worst)
