@@ -806,12 +806,14 @@ __global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jp
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     const std::uint64_t first = pair_items_before(P, s);
     const std::uint32_t count = (P + s - 1) / s;
+#pragma unroll 4
     for (std::uint32_t u = 0; u < count; ++u) {
         const float* p = jpart + (first + u) * 6 * kRow + r * kTile + lane;
         a0 += static_cast<double>(p[0]) - static_cast<double>(p[3 * kRow]);
         a1 += static_cast<double>(p[kRow]) - static_cast<double>(p[4 * kRow]);
         a2 += static_cast<double>(p[2 * kRow]) - static_cast<double>(p[5 * kRow]);
     }
+#pragma unroll 4
     for (std::uint32_t Q = P + 1; Q < blocks; ++Q) {
         const double* p = ipart + pair_slot(Q, t) * 3 * kTile + lane;
         a0 += p[0];
@@ -1018,8 +1020,10 @@ void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a) {
         key_add(key, a.y);
         key_add(key, a.z);
         key_add(key, a.count);
+        // Compact tiles need only the top 32 bits of the 63-bit codes: the four
+        // upper radix passes of eight.
         run_captured(ctx, ctx->direct_graph, std::move(key),
-                     [&] { morton_enqueue(ctx, a.x, a.y, a.z, a.count); }, true);
+                     [&] { morton_enqueue(ctx, a.x, a.y, a.z, a.count, 4); }, true);
         order = ctx->order_index.as<std::uint32_t>();
     }
     direct_enqueue(ctx, a, order);

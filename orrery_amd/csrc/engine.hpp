@@ -229,7 +229,7 @@ struct SegmentView {
 // FP64 values.
 void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, const double* z,
                              std::uint64_t n, const SegmentView& segments,
-                             const double* m = nullptr);
+                             const double* m = nullptr, int first_digit = 0);
 
 // Bytes the ordering stage reserves for n particles in `segments` systems
 // (`chunks` bounding-box chunks when segmented), with or without the
@@ -283,8 +283,10 @@ std::size_t ctx_device_bytes(const orr_ctx* ctx);
 // morton.hip: bounding cube, codes and stable radix sort from identity order.
 // Leaves ascending codes and permutation in ctx->order_codes / order_index
 // (uint64 / uint32). Returns nothing on the host; cube lands in ctx->order_bbox.
+// first_digit > 0 skips the sort's low 8-bit digits: the order is then by the
+// code's top bits only (stable within equal tops), enough for compact tiles.
 void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
-                    std::uint64_t n);
+                    std::uint64_t n, int first_digit = 0);
 // Copies the cube {origin, size} computed by the last morton_enqueue (syncs).
 void morton_cube(orr_ctx* ctx, double cube[4]);
 

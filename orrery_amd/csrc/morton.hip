@@ -373,12 +373,13 @@ OrderSizes order_sizes(std::uint64_t n, std::uint32_t segments, std::uint32_t ch
 }
 
 void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
-                    std::uint64_t n) {
-    morton_enqueue_segments(ctx, x, y, z, n, SegmentView{});
+                    std::uint64_t n, int first_digit) {
+    morton_enqueue_segments(ctx, x, y, z, n, SegmentView{}, nullptr, first_digit);
 }
 
 void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, const double* z,
-                             std::uint64_t n, const SegmentView& segs, const double* m) {
+                             std::uint64_t n, const SegmentView& segs, const double* m,
+                             int first_digit) {
     if (n == 0) return;
     if (n > (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "morton: more than 2^31 particles");
     const bool segmented = segs.count > 1;
@@ -425,12 +426,13 @@ void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, con
     {
         PhaseTimer t(ctx, kSort);
         std::uint32_t* hist = ctx->order_hist.as<std::uint32_t>();
-        // Eight passes over the 63-bit codes, then (segments > 1) one or two
-        // passes over the segment of each element: segment-major, then (code, index).
+        // Eight passes over the 63-bit codes (from first_digit: the order of the
+        // top digits only), then (segments > 1) one or two passes over the
+        // segment of each element: segment-major, then (code, index).
         int passes = 8;
         if (segmented) passes += segs.count > 256 ? 2 : 1;
         int cur = 0;
-        for (int pass = 0; pass < passes; ++pass) {
+        for (int pass = first_digit; pass < passes; ++pass) {
             const bool by_segment = pass >= 8;
             const int shift = by_segment ? 8 * (pass - 8) : 8 * pass;
             const std::uint32_t* digits = by_segment ? seg_of : nullptr;
