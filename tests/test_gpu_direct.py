@@ -332,3 +332,19 @@ def test_pair_mode_beyond_slot_budget_is_one_sided(ctx):
     ax, ay, az = out[1]
     worst, rms = oracle.error_summary(oracle.relative_errors(ax[i], ay[i], az[i], rx, ry, rz))
     assert rms <= RMS_BOUND and worst <= WORST_BOUND, (rms, worst)
+
+
+def test_pair_mode_at_largest_size(ctx):
+    # 428,032 particles: the largest system whose streamed-side slots fit the
+    # pairs path's 4 GiB (6,688 tiles, 836 blocks; slot offsets past 2^32
+    # bytes). Every particle outside the last block sums slots up to the last.
+    n, eps = 428032, 0.005
+    p = oracle.plummer(n, 15)
+    (ax, ay, az), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
+    rng = np.random.default_rng(15)
+    tgt = np.sort(rng.choice(n, 512, replace=False)).astype(np.uint64)
+    rx, ry, rz = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps, targets=tgt)
+    i = tgt.astype(np.int64)
+    worst, rms = oracle.error_summary(oracle.relative_errors(ax[i], ay[i], az[i], rx, ry, rz))
+    print(f"N={n} pairs worst={worst:.3e} rms={rms:.3e}")
+    assert rms <= RMS_BOUND and worst <= WORST_BOUND, (rms, worst)
