@@ -168,11 +168,13 @@ def test_config_plummer_262144_sampled(ctx):
     assert np.all(net < 1e-6), net
 
 
-def test_zero_masses_give_zero_finite_forces(ctx):
+@pytest.mark.parametrize("n", [300, 2048])
+def test_zero_masses_give_zero_finite_forces(ctx, n):
     # A sized but unfilled container: zero masses at the origin. With softening
     # every pair term is finite and massless, so the sum is exactly zero (without
-    # it coincident pairs are 0 * inf in the reference's kernel too).
-    z = np.zeros(300)
+    # it coincident pairs are 0 * inf in the reference's kernel too). At 2048
+    # particles the pairs path runs, every tile pair near (zero-size tiles).
+    z = np.zeros(n)
     (ax, ay, az), _ = evaluate(ctx, (z, z, z), z, eps=0.1)
     assert np.all(ax == 0) and np.all(ay == 0) and np.all(az == 0)
 
