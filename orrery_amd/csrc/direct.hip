@@ -27,6 +27,10 @@
 //     granularity).
 //   * The source range is split over gridDim.y; a fixed-order FP64 reduction
 //     kernel folds the splits, multiplies by G and scatters to particle order.
+//   * By default (direct_pairs, below) each far pair of tiles is evaluated
+//     once and applied to both tiles, with the same packing, near test and
+//     compensation; the one-sided kernel above then covers what the pairs path
+//     does not take (configured off, too few tiles, or slots beyond 4 GiB).
 //
 // The reference counts 20 flop per interaction (docs/performance.md:103-109).
 #include <hip/hip_runtime.h>
@@ -539,8 +543,9 @@ __device__ __forceinline__ float rotate_add(float a, float t) {
                                                                  0xf, 0xf, false)) + t;
 }
 
-// Four waves per SIMD (128 VGPRs): the stationary side holds 7 kStay values
-// per lane.
+// Four waves per SIMD (128 VGPRs): the stationary side holds 13 values per
+// tile per lane (offset re-expressed against the streamed tile, mass, the
+// streamed tile's partial sums, Kahan sums and compensations), 104 in all.
 template<bool kSafe, bool kRefine>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void direct_pairs(const float4* __restrict__ packed,
                                                        const float4* __restrict__ packed_lo,
