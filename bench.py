@@ -16,6 +16,7 @@ direct_complement, the pairs inside each block of 8 tiles) from HIP events
 recorded on the engine's own stream.
 
 Further legs in the same JSON line:
+* ``direct_n65536`` -- the same direct stepping at config 2's other size;
 * ``barnes_hut`` -- two-disc collision at N = 2,097,152, theta 0.5, leaf 16;
 * ``batched``    -- 512 Plummer spheres, N log-uniform in [64, 16384], eps 0.02,
   direct below 4096 particles and Barnes-Hut (theta 0.6) above, velocity Verlet
@@ -194,6 +195,33 @@ def direct_accuracy(E, W, ctx, cpu, seeds=(1, 2, 3)) -> list:
             rows.append({"n": n, "seed": seed, "rms_relative": e["rms_relative"],
                          "max_relative": e["max_relative"], "targets": e["targets"]})
     return rows
+
+
+def direct_rate(E, W, ctx, n: int, steps: int, seed: int = 1) -> dict:
+    """Config 2's other size: the same Yoshida-4 stepping of a device-resident
+    Plummer sphere, timed over `steps` steps after one warm-up step."""
+    eps, dt = 0.005, 5e-4
+    state = W.plummer(n, seed)
+    s = E.DeviceSystem(ctx, n)
+    s.upload(*state.arrays())
+    cfg = E.StepConfig(integrator="yoshida4", solver="direct", softening=eps)
+    s.refresh(cfg)
+    s.step(cfg, dt, 1)
+    ctx.set_timing(True)
+    ctx.reset_timings()
+    t0 = time.perf_counter()
+    s.step(cfg, dt, steps)
+    elapsed = time.perf_counter() - t0
+    timings = ctx.timings()
+    ctx.set_timing(False)
+    s.close()
+    evals = 3 * steps
+    k_ms, k_launches = timings.get("direct_kernel", (0.0, 0))
+    kernel_ms = k_ms / max(1, k_launches)
+    return {"n": n, "steps": steps, "interactions_per_s": evals * n * (n - 1) / elapsed,
+            "ms_per_evaluation": elapsed * 1e3 / evals, "kernel_ms_per_evaluation": kernel_ms,
+            "kernel_frac_of_fp32_peak": FLOP_PER_INTERACTION * n * (n - 1) / (kernel_ms * 1e-3) / 1e12
+            / FP32_PEAK_TFLOPS}
 
 
 def bench_barnes_hut(E, W, ctx, n: int, evals: int, error_targets: int, cpu) -> dict:
@@ -636,6 +664,8 @@ def main() -> int:
                                  "FETCH_SIZE + WRITE_SIZE bytes per evaluation "
                                  "(profiles/direct_pairs_traffic.json)"},
         })
+        log("[bench] direct at N = 65,536")
+        result["direct_n65536"] = direct_rate(E, W, ctx, 65536, 20)
         if cpu is not None:
             log("[bench] CPU leg: FP64 reference forces and the AVX2 baseline")
             rng = np.random.default_rng(12345)

@@ -515,14 +515,22 @@ __global__ __launch_bounds__(256) void batch_reduce(const float* __restrict__ pa
 //     passes' sums are Kahan-added into it). A fixed-order FP64 fold
 //     (pairs_reduce) sums a particle's slots.
 //
-// Items: block Q >= 1 streams tiles [0, Q kStay) in chunks of s kStay tiles;
-// items are numbered block by block, items_before(Q) of them before block Q.
+// Items: block Q >= 1 streams tiles [0, Q kStay) in chunks of S tiles, S a
+// divisor of kStay (kStay Q / S chunks) or a multiple of it (s = S / kStay:
+// ceil(Q / s) chunks); items are numbered block by block, items_before(Q) of
+// them before block Q.
 constexpr int kStay = 8;
 
-__host__ __device__ inline std::uint64_t pair_items_before(std::uint32_t Q, std::uint32_t s) {
-    // sum over q < Q of ceil(q / s): q = a s + r
+__host__ __device__ inline std::uint32_t pair_chunks(std::uint32_t Q, std::uint32_t S) {
+    return (kStay * Q + S - 1) / S;
+}
+
+__host__ __device__ inline std::uint64_t pair_items_before(std::uint32_t Q, std::uint32_t S) {
     if (Q == 0) return 0;
-    const std::uint64_t m = Q - 1, a = m / s, r = m % s;
+    const std::uint64_t q = Q;
+    if (S < kStay) return (kStay / S) * q * (q - 1) / 2;
+    // sum over q' < Q of ceil(q' / s): q' = a s + r
+    const std::uint64_t s = S / kStay, m = q - 1, a = m / s, r = m % s;
     return s * a * (a + 1) / 2 + (a + 1) * r;
 }
 
@@ -552,7 +560,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
                                                        const double4* __restrict__ centres,
                                                        const std::uint32_t* __restrict__ mask,
                                                        std::uint32_t tiles, std::uint32_t words,
-                                                       std::uint32_t blocks, std::uint32_t s,
+                                                       std::uint32_t blocks, std::uint32_t S,
                                                        std::uint32_t items, float eps2,
                                                        float* __restrict__ jpart,
                                                        double* __restrict__ ipart) {
@@ -563,14 +571,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     std::uint32_t lo = 1, hi = blocks - 1;
     while (lo < hi) {
         const std::uint32_t mid = (lo + hi + 1) / 2;
-        if (pair_items_before(mid, s) <= item) lo = mid;
+        if (pair_items_before(mid, S) <= item) lo = mid;
         else hi = mid - 1;
     }
     const std::uint32_t Q = lo;
-    const std::uint32_t chunk = item - static_cast<std::uint32_t>(pair_items_before(Q, s));
+    const std::uint32_t chunk = item - static_cast<std::uint32_t>(pair_items_before(Q, S));
     const std::uint32_t t0 = Q * kStay;
-    const std::uint32_t i0 = chunk * s * kStay;
-    const std::uint32_t i1 = min(i0 + s * kStay, t0);
+    const std::uint32_t i0 = chunk * S;
+    const std::uint32_t i1 = min(i0 + S, t0);
     const unsigned lane = threadIdx.x & (kTile - 1);
 
     // Stationary tiles past the last (a partial last block) repeat the last
@@ -798,7 +806,7 @@ __global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jp
                                                     const double* __restrict__ ipart,
                                                     const float* __restrict__ cpart,
                                                     std::uint32_t tiles, std::uint32_t blocks,
-                                                    std::uint32_t s, std::uint64_t n,
+                                                    std::uint32_t S, std::uint64_t n,
                                                     const std::uint32_t* __restrict__ order,
                                                     double G, double* __restrict__ ax,
                                                     double* __restrict__ ay,
@@ -809,8 +817,8 @@ __global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jp
     const std::uint32_t P = t / kStay, r = t % kStay;
     constexpr std::size_t kRow = kStay * kTile;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    const std::uint64_t first = pair_items_before(P, s);
-    const std::uint32_t count = (P + s - 1) / s;
+    const std::uint64_t first = pair_items_before(P, S);
+    const std::uint32_t count = pair_chunks(P, S);
 #pragma unroll 4
     for (std::uint32_t u = 0; u < count; ++u) {
         const float* p = jpart + (first + u) * 6 * kRow + r * kTile + lane;
@@ -868,7 +876,7 @@ constexpr std::uint32_t kMaskTiles = 32768;
 // otherwise, or when configured off, the one-sided kernel runs.
 struct PairPlan {
     bool use = false;
-    std::uint32_t blocks = 0, s = 1;
+    std::uint32_t blocks = 0, S = kStay;  // S: streamed tiles per item
     std::uint64_t items = 0, slots = 0;
     std::size_t jpart_floats = 0, ipart_doubles = 0;
 };
@@ -888,9 +896,14 @@ PairPlan pair_plan(std::uint32_t tiles, bool masked, int mode) {
     p.slots = pair_slot(p.blocks, 0);
     p.ipart_doubles = static_cast<std::size_t>(p.slots) * 3 * kTile;
     if (p.ipart_doubles * sizeof(double) > kPairSlotBytes) return p;
+    // About B^2 kStay / (2 S) items: S = kStay s for large systems, down to
+    // kStay / 4 for small ones (more, shorter items; below that the
+    // stationary-side slots outgrow the work).
     const std::uint64_t B = p.blocks;
-    p.s = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, B * B / (2 * kPairItems)));
-    p.items = pair_items_before(p.blocks, p.s);
+    const std::uint64_t s = B * B / (2 * kPairItems);
+    p.S = s >= 1 ? static_cast<std::uint32_t>(s) * kStay
+                 : (B * B * 2 >= kPairItems ? kStay / 2 : kStay / 4);
+    p.items = pair_items_before(p.blocks, p.S);
     p.jpart_floats = static_cast<std::size_t>(p.items) * 6 * kStay * kTile;
     p.use = true;
     return p;
@@ -905,7 +918,7 @@ void launch_pairs(orr_ctx* ctx, const PairPlan& plan, const float4* packed, cons
     const std::uint32_t items = static_cast<std::uint32_t>(plan.items);
     hipLaunchKernelGGL((direct_pairs<kSafe, kRefine>), dim3((items + kWaves - 1) / kWaves),
                        dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, mask, tiles, words, plan.blocks,
-                       plan.s, items, eps2, jpart, ipart);
+                       plan.S, items, eps2, jpart, ipart);
     ORR_LAUNCHED("direct_pairs");
     hipLaunchKernelGGL((direct_complement<kSafe, kRefine>), dim3((tiles + kWaves - 1) / kWaves),
                        dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, extents, mask, tiles,
@@ -980,7 +993,7 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
             PhaseTimer t(ctx, kDirectReduce);
             const std::uint32_t blocks = static_cast<std::uint32_t>((n + 255) / 256);
             hipLaunchKernelGGL(pairs_reduce, dim3(blocks), dim3(256), 0, ctx->stream, jpart, ipart,
-                               cpart, tiles, plan.blocks, plan.s, n, order,
+                               cpart, tiles, plan.blocks, plan.S, n, order,
                                a.gravitational_constant, a.ax, a.ay, a.az);
             ORR_LAUNCHED("pairs_reduce");
         }
