@@ -565,6 +565,8 @@ def main() -> int:
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-direct", action="store_true", help="skip the headline direct leg")
+    ap.add_argument("--n65536-steps", type=int, default=20,
+                    help="Yoshida-4 steps of the N = 65,536 direct leg (0 skips it)")
     ap.add_argument("--walk-targets", type=int, default=0,
                     help="targets per lane of the Barnes-Hut walk (1 or 2; 0: the engine's default)")
     ap.add_argument("--probes-only", action="store_true",
@@ -664,8 +666,9 @@ def main() -> int:
                                  "FETCH_SIZE + WRITE_SIZE bytes per evaluation "
                                  "(profiles/direct_pairs_traffic.json)"},
         })
-        log("[bench] direct at N = 65,536")
-        result["direct_n65536"] = direct_rate(E, W, ctx, 65536, 20)
+        if args.n65536_steps > 0:
+            log("[bench] direct at N = 65,536")
+            result["direct_n65536"] = direct_rate(E, W, ctx, 65536, args.n65536_steps)
         if cpu is not None:
             log("[bench] CPU leg: FP64 reference forces and the AVX2 baseline")
             rng = np.random.default_rng(12345)
