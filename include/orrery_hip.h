@@ -116,12 +116,12 @@ ORR_API int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* args);
  * force within 1e-6 relative); block_target is the number of thread blocks the
  * source range is split to reach (default 32768). pair_mode 1 (default)
  * evaluates each far pair of 64-particle tiles once and applies it to both
- * tiles, where it applies (at least 4 blocks of 8 tiles, i.e. from 1,537
- * particles, and 3 doubles per particle and later block within 4 GiB, i.e. up
- * to 428,032 particles);
- * pair_mode 2 always runs the one-sided kernel, to which targets_per_lane and
- * block_target apply. targets_per_lane 0, refine -1, block_target 0 and
- * pair_mode 0 keep the current value. */
+ * tiles where that is faster and fits: from 704 tiles (44,993 particles) up to
+ * 428,032 particles (3 doubles per particle and later block of 8 tiles within
+ * 4 GiB); pair_mode 3 does so wherever it fits (from 4 blocks of 8 tiles,
+ * 1,537 particles); pair_mode 2 always runs the one-sided kernel, to which
+ * targets_per_lane and block_target apply. targets_per_lane 0, refine -1,
+ * block_target 0 and pair_mode 0 keep the current value. */
 typedef struct {
     int32_t targets_per_lane;
     int32_t refine;

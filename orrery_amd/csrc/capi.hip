@@ -258,7 +258,7 @@ int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* c) {
                     c->targets_per_lane == 4,
                 "targets_per_lane must be 1, 2 or 4");
         require(c->refine == 0 || c->refine == 1 || c->refine == -1, "refine must be 0 or 1 (-1 keeps)");
-        require(c->pair_mode >= 0 && c->pair_mode <= 2, "pair_mode must be 1 or 2 (0 keeps)");
+        require(c->pair_mode >= 0 && c->pair_mode <= 3, "pair_mode must be 1, 2 or 3 (0 keeps)");
         if (c->targets_per_lane != 0) ctx->direct_targets_per_lane = c->targets_per_lane;
         if (c->pair_mode != 0) ctx->direct_pair_mode = c->pair_mode;
         if (c->refine >= 0) ctx->direct_refine = c->refine;

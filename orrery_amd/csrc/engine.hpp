@@ -122,7 +122,9 @@ struct orr_ctx {
     int direct_targets_per_lane = 1;
     int direct_refine = 0;
     std::uint32_t direct_block_target = 32768;
-    int direct_pair_mode = 1;  // 1: far tile pairs once for both tiles where it fits, 2: one-sided
+    // 1: far tile pairs once for both tiles where that is faster and fits,
+    // 2: one-sided, 3: pairs wherever they fit.
+    int direct_pair_mode = 1;
     // Barnes-Hut walk configuration (orr_ctx_set_tree_config); 0 chooses by size.
     int walk_targets_per_lane = 0;
 

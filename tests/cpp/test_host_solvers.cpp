@@ -143,7 +143,7 @@ void direct_cases() {
     std::printf("hip-direct N=3000 rms relative error %.3e\n", rms);
     CHECK(rms < 1e-6);
     CHECK(s2->timings().kernel > 0);
-    // Ordered pairs, N(N-1), also where each far pair of tiles is evaluated once.
+    // Ordered pairs, N(N-1).
     CHECK(s2->interaction_count().particle_particle == 3000ull * 2999ull);
     // Mismatched spans are a usage error.
     System bad(3);

@@ -168,14 +168,19 @@ def test_config_plummer_262144_sampled(ctx):
     assert np.all(net < 1e-6), net
 
 
-@pytest.mark.parametrize("n", [300, 2048])
-def test_zero_masses_give_zero_finite_forces(ctx, n):
+@pytest.mark.parametrize("n,mode", [(300, 1), (2048, 3)])
+def test_zero_masses_give_zero_finite_forces(ctx, n, mode):
     # A sized but unfilled container: zero masses at the origin. With softening
     # every pair term is finite and massless, so the sum is exactly zero (without
     # it coincident pairs are 0 * inf in the reference's kernel too). At 2048
-    # particles the pairs path runs, every tile pair near (zero-size tiles).
+    # particles in pair mode 3 the pairs path runs, every tile pair near
+    # (zero-size tiles).
     z = np.zeros(n)
-    (ax, ay, az), _ = evaluate(ctx, (z, z, z), z, eps=0.1)
+    try:
+        ctx.set_direct_config(pair_mode=mode)
+        (ax, ay, az), _ = evaluate(ctx, (z, z, z), z, eps=0.1)
+    finally:
+        ctx.set_direct_config(pair_mode=1)
     assert np.all(ax == 0) and np.all(ay == 0) and np.all(az == 0)
 
 
@@ -212,17 +217,18 @@ def test_direct_config_round_trip(ctx):
         with pytest.raises(E.solvers.C.EngineError):
             ctx.set_direct_config(3, 0, 0)
         with pytest.raises(E.solvers.C.EngineError):
-            ctx.set_direct_config(0, -1, 0, 3)
+            ctx.set_direct_config(0, -1, 0, 4)
     finally:
         ctx.set_direct_config(1, 0, 32768, 1)
 
 
-# Pair mode 1 evaluates each far pair of tiles once for both tiles (blocks of
-# 8 stationary tiles, streamed tiles rotating around the wave); pair mode 2 is
-# the one-sided kernel. Sizes: the smallest that takes the pairs path (25
-# tiles = 4 blocks, the last with one tile of one particle), a ragged one (a
-# partial last block of 3 tiles and a partial last tile), and one with several
-# items per block. The worst
+# Pair mode 3 evaluates each far pair of tiles once for both tiles (blocks of
+# 8 stationary tiles, streamed tiles rotating around the wave) wherever that
+# applies; pair mode 1, the default, only from 704 tiles, where it is faster;
+# pair mode 2 is the one-sided kernel. Sizes: the smallest that takes the pairs
+# path (25 tiles = 4 blocks, the last with one tile of one particle), a ragged
+# one (a partial last block of 3 tiles and a partial last tile), and one with
+# several items per block. The worst
 # particle of these small softened systems sits near 1e-6 in either mode; the
 # pairs path's streamed side sums the terms of eight tiles per step, which
 # puts its worst particle up to some 60 % above the one-sided kernel's at these
@@ -236,7 +242,7 @@ def test_pair_modes_against_fp64_reference(ctx, n, eps):
     rx, ry, rz = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps)
     results, worsts = {}, {}
     try:
-        for mode in (1, 2):
+        for mode in (3, 2):
             ctx.set_direct_config(pair_mode=mode)
             (ax, ay, az), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
             assert np.all(np.isfinite(ax)) and np.all(np.isfinite(ay)) and np.all(np.isfinite(az))
@@ -250,8 +256,8 @@ def test_pair_modes_against_fp64_reference(ctx, n, eps):
             results[mode], worsts[mode] = (ax, ay, az), worst
     finally:
         ctx.set_direct_config(pair_mode=1)
-    assert worsts[1] <= 2.0 * worsts[2] + 1e-7, worsts
-    rel = oracle.relative_errors(*results[1], *results[2])
+    assert worsts[3] <= 2.0 * worsts[2] + 1e-7, worsts
+    rel = oracle.relative_errors(*results[3], *results[2])
     assert oracle.error_summary(rel)[1] < (2e-6 if eps > 0 else 2e-5)
 
 
@@ -260,8 +266,12 @@ def test_pair_mode_momentum_and_repeatability(ctx):
     # so sum m a cancels to rounding; two evaluations are bitwise equal.
     p = oracle.plummer(24576, 12)
     eps = 0.005
-    (a0, b0, c0), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
-    (a1, b1, c1), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
+    try:
+        ctx.set_direct_config(pair_mode=3)
+        (a0, b0, c0), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
+        (a1, b1, c1), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
+    finally:
+        ctx.set_direct_config(pair_mode=1)
     assert np.array_equal(a0, a1) and np.array_equal(b0, b1) and np.array_equal(c0, c1)
     scale = np.sum(p.m * np.sqrt(a0 * a0 + b0 * b0 + c0 * c0))
     net = np.abs([np.sum(p.m * a0), np.sum(p.m * b0), np.sum(p.m * c0)]) / scale
@@ -304,7 +314,7 @@ def test_pair_mode_with_newton_step(ctx):
     eps = 0.01
     rx, ry, rz = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps)
     try:
-        ctx.set_direct_config(refine=1, pair_mode=1)
+        ctx.set_direct_config(refine=1, pair_mode=3)
         (ax, ay, az), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
     finally:
         ctx.set_direct_config(refine=0, pair_mode=1)
