@@ -618,6 +618,8 @@ def main() -> int:
         acc = sys_.download()
         log(f"[bench] warmup {args.warmup} steps")
         sys_.step(cfg, dt, args.warmup)
+        # FP64 energy before and after the timed steps (outside the timed region).
+        e_before = sys_.diagnostics(eps)
         ctx.set_timing(True)
         ctx.reset_timings()
         log(f"[bench] timing {args.steps} steps")
@@ -626,7 +628,13 @@ def main() -> int:
         elapsed = time.perf_counter() - t0
         timings = ctx.timings()
         ctx.set_timing(False)
+        e_after = sys_.diagnostics(eps)
         sys_.close()
+        energy = {"relative_drift": (e_after["total"] - e_before["total"]) / abs(e_before["total"]),
+                  "steps": args.steps, "total_before": e_before["total"],
+                  "momentum_after": [float(v) for v in e_after["momentum"]],
+                  "note": "FP64 kinetic + pairwise potential over all pairs (softened), before and after "
+                          "the timed Yoshida-4 steps"}
 
         evals = 3 * args.steps
         value = evals * n * (n - 1) / elapsed
@@ -647,6 +655,7 @@ def main() -> int:
             "ms_per_step": elapsed * 1e3 / max(1, args.steps),
             "ms_per_evaluation": eval_ms,
             "kernel_ms_per_evaluation": kernel_ms,
+            "energy": energy,
             "phases_ms": {k: v[0] for k, v in timings.items()},
             "roofline": {"bound": "fp32-valu", "kernel": "direct_pairs+direct_complement",
                          "achieved": achieved,
