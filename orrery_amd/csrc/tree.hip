@@ -762,15 +762,26 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
                 }
             }
         }
+        // The wave's next node: cur + 1 when any target descends; else, most
+        // often, this node's escape `next` (some target is there and none
+        // before it), without the wave-wide minimum; else that minimum.
         const std::uint32_t step = cur + 1;
-        bool descends = false;
+        bool descends = false, at_next = false, before_next = false;
         least = t[0].mine;
 #pragma unroll
         for (int r = 0; r < kR; ++r) {
             descends = descends || t[r].mine == step;
+            at_next = at_next || t[r].mine == next;
+            before_next = before_next || t[r].mine < next;
             least = min(least, t[r].mine);
         }
-        cur = (__ballot(descends) != 0) ? step : __builtin_amdgcn_readfirstlane(wave_min_u32(least));
+        if (__ballot(descends) != 0) {
+            cur = step;
+        } else if (__ballot(before_next) == 0 && __ballot(at_next) != 0) {
+            cur = next;
+        } else {
+            cur = __builtin_amdgcn_readfirstlane(wave_min_u32(least));
+        }
     }
     std::uint32_t pp = 0, pc = 0, visits = 0;
     bool mixed = false;
