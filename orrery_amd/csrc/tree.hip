@@ -719,10 +719,11 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
         t[r].px = t[r].py = t[r].pz = 0.f;
         t[r].pp = t[r].pc = t[r].visits = 0;
     }
-    unsigned long long steps = 0;
+    std::uint32_t steps = 0;
     // Each iteration advances at least one target, so a walk takes at most
     // 64 kR M iterations; the cap only turns a corrupted tree into a wrong result.
-    const unsigned long long cap = 64ull * kR * M + 64;
+    // 32-bit counts (a 32-bit compare in the loop condition), the cap clamped.
+    const std::uint32_t cap = static_cast<std::uint32_t>(min(64ull * kR * M + 64, 0xffffffffull));
     std::uint32_t least = t[0].mine;
 #pragma unroll
     for (int r = 1; r < kR; ++r) least = min(least, t[r].mine);
@@ -814,7 +815,7 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
             mine_slot[0] = wpp;
             mine_slot[1] = wpc;
             mine_slot[2] = wvisits;
-            mine_slot[3] = steps * kR;  // lane-target slots: 64 per count
+            mine_slot[3] = static_cast<unsigned long long>(steps) * kR;  // lane-target slots: 64 per count
         }
     } else {
 #pragma unroll
@@ -827,7 +828,8 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
             }
         }
         if (lane == 0) {
-            atomicAdd(&stats[4 * static_cast<std::size_t>(seg0) + 3], steps * kR);
+            atomicAdd(&stats[4 * static_cast<std::size_t>(seg0) + 3],
+                      static_cast<unsigned long long>(steps) * kR);
             mine_slot[0] = mine_slot[1] = mine_slot[2] = mine_slot[3] = 0;
         }
     }
