@@ -432,6 +432,8 @@ HipTreeSolver.quadrupoles = _tree_quadrupoles
 def morton_order(ctx: Context, x: np.ndarray, y: np.ndarray, z: np.ndarray):
     """Device Morton ordering of host positions: (codes, index, origin, size)."""
     n = len(x)
+    if len(y) != n or len(z) != n:
+        raise ValueError(f"x, y, z hold {n}, {len(y)}, {len(z)} values; they must match")
     dx, dy, dz = (DeviceArray.from_host(ctx, np.ascontiguousarray(a, dtype=np.float64)) for a in (x, y, z))
     codes = DeviceArray(ctx, n, np.uint64)
     index = DeviceArray(ctx, n, np.uint32)
@@ -492,11 +494,23 @@ class DeviceSystem:
             pass
 
     @staticmethod
-    def _p(a):
-        return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    def _p(a, count=None, name="array"):
+        """FP64 contiguous copy of a; with count, exactly count values or ValueError (the
+        C ABI copies count values from the pointer, so a shorter buffer would be over-read)."""
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if count is not None and a.shape != (count,):
+            raise ValueError(f"{name}: expected {count} values, got shape {a.shape}")
+        return a
+
+    @staticmethod
+    def _checked(count, x, y, z, vx, vy, vz, m):
+        names = ("x", "y", "z", "vx", "vy", "vz", "m")
+        return [DeviceSystem._p(a, count, n) for a, n in zip((x, y, z, vx, vy, vz, m), names)]
 
     def upload(self, x, y, z, vx, vy, vz, m) -> None:
-        arrs = [self._p(a) for a in (x, y, z, vx, vy, vz, m)]
+        arrs = self._checked(int(self.count), x, y, z, vx, vy, vz, m)
         ptrs = [None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in arrs]
         C.check(C.load().orr_system_upload(self.handle, *ptrs))
 
@@ -621,7 +635,9 @@ class DeviceBatch:
         return int(out.value)
 
     def upload(self, system: int, x, y, z, vx, vy, vz, m) -> None:
-        arrs = [DeviceSystem._p(a) for a in (x, y, z, vx, vy, vz, m)]
+        if not 0 <= int(system) < len(self.systems):
+            raise IndexError(f"system {system} of a batch of {len(self.systems)}")
+        arrs = DeviceSystem._checked(int(self.systems[system].count), x, y, z, vx, vy, vz, m)
         ptrs = [None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in arrs]
         C.check(C.load().orr_batch_upload(self.handle, int(system), *ptrs))
 

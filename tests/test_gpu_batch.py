@@ -215,6 +215,47 @@ def test_invalid_batches_are_rejected(ctx):
     b.close()
 
 
+def test_uploads_of_the_wrong_length_are_rejected(ctx):
+    # The C ABI copies each system's own count of values from the pointer it is
+    # given, so the host side must refuse shorter (over-read) or longer arrays.
+    b = E.DeviceBatch(ctx, [E.BatchSystem(100, EPS, 1e-3), E.BatchSystem(5000, EPS, 1e-3)],
+                      E.BatchConfig())
+    small, large = oracle.plummer(100, 1), oracle.plummer(5000, 2)
+    with pytest.raises(ValueError):
+        b.upload(1, *small.arrays())
+    with pytest.raises(ValueError):
+        b.upload(0, *large.arrays())
+    with pytest.raises(IndexError):
+        b.upload(2, *small.arrays())
+    b.close()
+    s = E.DeviceSystem(ctx, 100)
+    with pytest.raises(ValueError):
+        s.upload(*large.arrays())
+    s.close()
+    with pytest.raises(ValueError):
+        E.solvers.morton_order(ctx, np.zeros(10), np.zeros(9), np.zeros(10))
+
+
+def test_system_order_does_not_change_results(ctx):
+    # Every system is its own tree or direct sum: the same systems in another
+    # order (sorted by size, as a caller may well hold them) give identical bits.
+    ns = sizes(24, 64, 9000, 11)
+    config = E.BatchConfig(opening_angle=0.6, leaf_capacity=16, tiles_per_item=8, direct_below=3000)
+    order = sorted(range(len(ns)), key=lambda k: -ns[k])
+    a, _ = make_batch(ctx, ns, config)
+    b, _ = make_batch(ctx, [ns[k] for k in order], config, seeds={j: 100 + k for j, k in enumerate(order)})
+    a.refresh()
+    b.refresh()
+    for j, k in enumerate(order):
+        da, db = a.download(k), b.download(j)
+        for key in ("ax", "ay", "az"):
+            assert np.array_equal(da[key], db[key]), (k, key)
+        if ns[k] >= 3000:
+            assert a.counts(k) == b.counts(j)
+    a.close()
+    b.close()
+
+
 def test_edge_systems_in_one_batch(ctx):
     # Zero softening (masked self terms), tiny systems forced onto the tree,
     # systems smaller than a leaf, and one direct system alone in its branch.
