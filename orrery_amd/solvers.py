@@ -727,7 +727,8 @@ class Server:
 
     def submit(self, state, integrator: str, softening: float, timestep: float, steps: int,
                gravitational_constant: float = 1.0) -> int:
-        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in state.arrays()]
+        raw = state.arrays()
+        arrs = DeviceSystem._checked(len(raw[6]), *raw)
         req = C.OrrRequest(len(arrs[6]), INTEGRATORS[integrator], 0, float(softening) ** 2,
                            float(gravitational_constant), float(timestep), int(steps),
                            *(a.ctypes.data_as(ctypes.c_void_p) for a in arrs))

@@ -127,6 +127,9 @@ def test_invalid_requests(ctx):
     with pytest.raises(E.solvers.C.EngineError) as e:
         s.submit(w, "velocity-verlet", EPS, 1e-3, 0)
     assert e.value.status == 1
+    ragged = W.Workload(w.x[:50], w.y, w.z, w.vx, w.vy, w.vz, w.m)
+    with pytest.raises(ValueError):
+        s.submit(ragged, "velocity-verlet", EPS, 1e-3, 5)
     big = W.plummer(200000, 1)
     with pytest.raises(E.solvers.C.EngineError) as e:
         s.submit(big, "velocity-verlet", EPS, 1e-3, 5)
