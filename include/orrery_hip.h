@@ -120,13 +120,19 @@ ORR_API int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* args);
  * 428,032 particles (3 doubles per particle and later block of 8 tiles within
  * 4 GiB); pair_mode 3 does so wherever it fits (from 4 blocks of 8 tiles,
  * 1,537 particles); pair_mode 2 always runs the one-sided kernel, to which
- * targets_per_lane and block_target apply. targets_per_lane 0, refine -1,
- * block_target 0 and pair_mode 0 keep the current value. */
+ * targets_per_lane and block_target apply. equal_masses 1 (default) lets the
+ * pairs path take its equal-mass form when every particle has one mass (the
+ * mass leaves the pair term and multiplies the folded sums: two multiplies
+ * fewer per pair; the same bits when that mass is a power of two), 2 never.
+ * targets_per_lane 0, refine -1, block_target 0, pair_mode 0 and
+ * equal_masses 0 keep the current value. */
 typedef struct {
     int32_t targets_per_lane;
     int32_t refine;
     uint32_t block_target;
     int32_t pair_mode;
+    int32_t equal_masses;
+    int32_t reserved;
 } orr_direct_config;
 ORR_API int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* config);
 ORR_API int orr_ctx_get_direct_config(orr_ctx* ctx, orr_direct_config* config);
@@ -226,6 +232,11 @@ ORR_API int orr_probe(orr_ctx* ctx, int32_t kind, uint64_t size, double* rate_ou
 typedef struct {
     double milliseconds[ORR_PHASES];
     uint64_t launches[ORR_PHASES];
+    /* Captured evaluations replayed as one hipGraph launch since the last
+     * reset (counted whether or not timing is on; with timing on evaluations
+     * run eagerly so that their phases can be timed). */
+    uint64_t graph_launches;
+    uint64_t reserved[3];
 } orr_timings;
 ORR_API int orr_ctx_set_timing(orr_ctx* ctx, int32_t enabled);
 ORR_API int orr_ctx_timings(orr_ctx* ctx, orr_timings* out);
@@ -397,6 +408,12 @@ typedef struct {
     uint32_t tiles_per_item;         /* 0: 16 */
     int32_t quadrupole;
     int32_t use_graph;
+    /* Testing aid for the recovery path: the first `fail_rebuilds` batch
+     * rebuilds fail after their uploads and copies, as a failed allocation or
+     * copy would (orr_server_round returns ORR_ERR_RUNTIME); the requests and
+     * the old batch are kept, and a later round tries again. 0 in service. */
+    uint32_t fail_rebuilds;
+    uint32_t reserved;
 } orr_server_config;
 
 typedef struct {
@@ -442,6 +459,21 @@ ORR_API int orr_server_status(orr_server* server, uint64_t id, orr_request_statu
 ORR_API int orr_server_result(orr_server* server, uint64_t id, double* x, double* y, double* z,
                               double* vx, double* vy, double* vz);
 ORR_API int orr_server_stats_get(orr_server* server, orr_server_stats* out);
+
+/* ---- seeded initial conditions (host only; no device is touched) -------- */
+
+/* The reference's Plummer sphere for a seed, bit for bit: replaces
+ * make_plummer_sphere (src/initial_conditions/plummer.cpp:91-128) over
+ * RandomSource (src/core/random.cpp:12-47; std::mt19937_64, top 53 bits) and
+ * move_to_centre_of_mass_frame, with the position direction drawn before the
+ * velocity direction (the Clang order of plummer.cpp:122). The input side of
+ * the wasm ABI's orrery_create (wasm/orrery_wasm.h:113), which builds its
+ * initial conditions inside the library. Writes `count` values into each host
+ * array. ORR_ERR_INVALID_ARGUMENT unless total_mass > 0, scale_radius > 0 and
+ * 0 < mass_fraction_cutoff < 1 (plummer.cpp:20-36); count 0 writes nothing. */
+ORR_API int orr_plummer_sample(uint64_t count, double total_mass, double scale_radius,
+                               double mass_fraction_cutoff, uint64_t seed, double* x, double* y,
+                               double* z, double* vx, double* vy, double* vz, double* m);
 
 #ifdef __cplusplus
 }

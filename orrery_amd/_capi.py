@@ -43,7 +43,8 @@ class OrrDirectArgs(ctypes.Structure):
 
 class OrrDirectConfig(ctypes.Structure):
     _fields_ = [("targets_per_lane", ctypes.c_int32), ("refine", ctypes.c_int32),
-                ("block_target", ctypes.c_uint32), ("pair_mode", ctypes.c_int32)]
+                ("block_target", ctypes.c_uint32), ("pair_mode", ctypes.c_int32),
+                ("equal_masses", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class OrrTreeArgs(ctypes.Structure):
@@ -62,7 +63,8 @@ class OrrTreeStats(ctypes.Structure):
 
 class OrrTimings(ctypes.Structure):
     _fields_ = [("milliseconds", ctypes.c_double * ORR_PHASES),
-                ("launches", ctypes.c_uint64 * ORR_PHASES)]
+                ("launches", ctypes.c_uint64 * ORR_PHASES),
+                ("graph_launches", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 3)]
 
 
 class OrrStepConfig(ctypes.Structure):
@@ -100,7 +102,8 @@ class OrrServerConfig(ctypes.Structure):
                 ("direct_below", ctypes.c_uint32), ("alone_from", ctypes.c_uint32),
                 ("opening_angle", ctypes.c_double), ("leaf_capacity", ctypes.c_uint32),
                 ("tiles_per_item", ctypes.c_uint32), ("quadrupole", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("fail_rebuilds", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class OrrRequest(ctypes.Structure):
@@ -189,6 +192,8 @@ SIGNATURES = {
     "orr_server_status": [_vp, ctypes.c_uint64, ctypes.POINTER(OrrRequestStatus)],
     "orr_server_result": [_vp, ctypes.c_uint64] + [_vp] * 6,
     "orr_server_stats_get": [_vp, ctypes.POINTER(OrrServerStats)],
+    "orr_plummer_sample": [ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                           ctypes.c_uint64] + [_d] * 7,
 }
 
 _LIB = None

@@ -19,6 +19,12 @@ BUILD = os.path.join(HERE, "_build")
 LIBRARY = os.path.join(HERE, "liborrery_hip.so")
 SOURCES = ["context.hip", "capi.hip", "direct.hip", "morton.hip", "scan.hip", "tree.hip", "batch.hip", "serve.hip", "io.hip",
            "integrate.hip", "probes.hip"]
+# Host-only translation units, compiled by the host compiler: the seeded
+# initial-condition sampler must make the same libm calls, unfused, as the
+# reference's CPU build.
+HOST_SOURCES = ["sample.cpp"]
+HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-fvisibility=hidden", "-Wall", "-ffp-contract=off",
+              "-fno-builtin-sin", "-fno-builtin-cos"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -34,11 +40,14 @@ FLAGS = [
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if src.endswith(".cpp"):
+        cmd = [os.environ.get("CXX", "g++"), *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr}")
+        raise RuntimeError(f"compiling {src} failed:\n{res.stderr}")
     return obj
 
 
@@ -75,8 +84,8 @@ def build_host(verbose: bool = False) -> None:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as pool:
-        objs = list(pool.map(_compile, SOURCES))
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        objs = list(pool.map(_compile, SOURCES + HOST_SOURCES))
     tmp = LIBRARY + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
     res = subprocess.run(cmd, capture_output=True, text=True)

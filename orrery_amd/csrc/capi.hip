@@ -259,6 +259,8 @@ int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* c) {
                 "targets_per_lane must be 1, 2 or 4");
         require(c->refine == 0 || c->refine == 1 || c->refine == -1, "refine must be 0 or 1 (-1 keeps)");
         require(c->pair_mode >= 0 && c->pair_mode <= 3, "pair_mode must be 1, 2 or 3 (0 keeps)");
+        require(c->equal_masses >= 0 && c->equal_masses <= 2, "equal_masses must be 1 or 2 (0 keeps)");
+        if (c->equal_masses != 0) ctx->direct_equal_masses = c->equal_masses;
         if (c->targets_per_lane != 0) ctx->direct_targets_per_lane = c->targets_per_lane;
         if (c->pair_mode != 0) ctx->direct_pair_mode = c->pair_mode;
         if (c->refine >= 0) ctx->direct_refine = c->refine;
@@ -292,6 +294,8 @@ int orr_ctx_get_direct_config(orr_ctx* ctx, orr_direct_config* c) {
         c->refine = ctx->direct_refine;
         c->block_target = ctx->direct_block_target;
         c->pair_mode = ctx->direct_pair_mode;
+        c->equal_masses = ctx->direct_equal_masses;
+        c->reserved = 0;
     });
 }
 

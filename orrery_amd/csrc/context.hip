@@ -103,6 +103,7 @@ void run_captured(orr_ctx* ctx, orr_ctx::Captured& g, std::vector<unsigned char>
         }
     }
     ORR_HIP(hipGraphLaunch(g.exec, ctx->stream));
+    ++ctx->totals.graph_launches;
 }
 
 orr_ctx* make_context(int device) {

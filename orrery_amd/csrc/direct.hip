@@ -39,6 +39,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "close_pair.hpp"
 #include "engine.hpp"
 
 namespace orr {
@@ -67,7 +68,8 @@ __device__ __forceinline__ void pack_tile(double px, double py, double pz, doubl
                                           float4* __restrict__ packed,
                                           float4* __restrict__ packed_lo,
                                           double4* __restrict__ centres,
-                                          float4* __restrict__ extents) {
+                                          float4* __restrict__ extents,
+                                          float2* __restrict__ mass_range = nullptr) {
     const double inf = __builtin_huge_val();
     const double lx = wave_min(valid ? px : inf), hx = wave_max(valid ? px : -inf);
     const double ly = wave_min(valid ? py : inf), hy = wave_max(valid ? py : -inf);
@@ -83,6 +85,15 @@ __device__ __forceinline__ void pack_tile(double px, double py, double pz, doubl
     } else {
         packed[k] = make_float4(0.f, 0.f, 0.f, 0.f);
         packed_lo[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (mass_range != nullptr) {
+        // The tile's smallest and largest packed mass, its massless pads
+        // included: a system with pads never counts as one of equal masses.
+        const double fm = valid ? static_cast<double>(static_cast<float>(pm)) : 0.0;
+        const double lo = wave_min(fm), hi = wave_max(fm);
+        if ((threadIdx.x & (kTile - 1)) == 0) {
+            mass_range[tile] = make_float2(static_cast<float>(lo), static_cast<float>(hi));
+        }
     }
     if ((threadIdx.x & (kTile - 1)) == 0) {
         centres[tile] = make_double4(cx, cy, cz, w);
@@ -100,12 +111,39 @@ __global__ __launch_bounds__(64) void direct_prepare(const double* __restrict__ 
                                                      std::uint64_t n, float4* __restrict__ packed,
                                                      float4* __restrict__ packed_lo,
                                                      double4* __restrict__ centres,
-                                                     float4* __restrict__ extents) {
+                                                     float4* __restrict__ extents,
+                                                     float2* __restrict__ mass_range) {
     const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * kTile + threadIdx.x;
     const bool valid = k < n;
     const std::uint64_t src = valid ? (order != nullptr ? order[k] : k) : 0;
     pack_tile(valid ? x[src] : 0.0, valid ? y[src] : 0.0, valid ? z[src] : 0.0,
-              valid ? m[src] : 0.0, valid, k, blockIdx.x, packed, packed_lo, centres, extents);
+              valid ? m[src] : 0.0, valid, k, blockIdx.x, packed, packed_lo, centres, extents,
+              mass_range);
+}
+
+// uniform[0] = the common packed mass when every particle has the same
+// positive mass, else 0 (one block over the tiles' ranges).
+__global__ __launch_bounds__(256) void mass_uniform(const float2* __restrict__ mass_range,
+                                                    std::uint32_t tiles, int enabled,
+                                                    float* __restrict__ uniform) {
+    __shared__ float lo_s[256], hi_s[256];
+    float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+    for (std::uint32_t t = threadIdx.x; t < tiles; t += blockDim.x) {
+        const float2 r = mass_range[t];
+        lo = fminf(lo, r.x);
+        hi = fmaxf(hi, r.y);
+    }
+    lo_s[threadIdx.x] = lo;
+    hi_s[threadIdx.x] = hi;
+    __syncthreads();
+    for (unsigned half = 128; half > 0; half >>= 1) {
+        if (threadIdx.x < half) {
+            lo_s[threadIdx.x] = fminf(lo_s[threadIdx.x], lo_s[threadIdx.x + half]);
+            hi_s[threadIdx.x] = fmaxf(hi_s[threadIdx.x], hi_s[threadIdx.x + half]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) uniform[0] = (enabled && lo_s[0] == hi_s[0] && lo_s[0] > 0.f) ? lo_s[0] : 0.f;
 }
 
 // One pairwise term accumulated into (px, py, pz). kSafe masks r^2 == 0 (zero
@@ -115,18 +153,28 @@ __global__ __launch_bounds__(64) void direct_prepare(const double* __restrict__ 
 // the hardware reciprocal square root y0 before cubing: with e = 1 - x y0^2
 // (one rounding, via FMA), y^3 = y0^3 (1 + 3e/2) to second order in e.
 template<bool kSafe, bool kRefine>
+__device__ __forceinline__ float inv_cubed(float r2) {
+    if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
+    if constexpr (!kSafe && !kRefine) {
+        return __builtin_amdgcn_rsqf((r2 * r2) * r2);
+    } else {
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        float inv3 = (inv * inv) * inv;
+        if constexpr (kRefine) {
+            const float e = __builtin_fmaf(-(r2 * inv), inv, 1.f);
+            inv3 = inv3 * __builtin_fmaf(1.5f, e, 1.f);
+        }
+        return inv3;
+    }
+}
+
+template<bool kSafe, bool kRefine>
 __device__ __forceinline__ void term(float dx, float dy, float dz, float m, float eps2, float& px,
                                      float& py, float& pz) {
     float r2 = __builtin_fmaf(dx, dx, eps2);
     r2 = __builtin_fmaf(dy, dy, r2);
     r2 = __builtin_fmaf(dz, dz, r2);
-    if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
-    const float inv = __builtin_amdgcn_rsqf(r2);
-    float f = (m * inv) * (inv * inv);
-    if constexpr (kRefine) {
-        const float e = __builtin_fmaf(-(r2 * inv), inv, 1.f);
-        f = f * __builtin_fmaf(1.5f, e, 1.f);
-    }
+    const float f = m * inv_cubed<kSafe, kRefine>(r2);
     px = __builtin_fmaf(f, dx, px);
     py = __builtin_fmaf(f, dy, py);
     pz = __builtin_fmaf(f, dz, pz);
@@ -154,9 +202,40 @@ __device__ __forceinline__ bool near_tiles(const double4& C, const float4& E, do
     return gx * gx + gy * gy + gz * gz <= reach * reach;
 }
 
-// Targets' sums against one 64-source tile J: the tile's terms summed plainly
-// in FP32, then Kahan-added per component into (s, k). kUnroll sets how many
-// far-path sources are in flight (their SGPR loads included).
+// Close pairs: closer than kClose times the wider tile's largest half-extent.
+// Their terms are evaluated in double-float arithmetic (close_pair.hpp); about
+// one step in twenty of a near pass has such a pair on some lane.
+constexpr float kClose = 0.25f;
+
+// The FP32 r^2 below which a near pair is close (r^2 includes eps^2).
+__device__ __forceinline__ float close_limit(double w_target, double w_source, float eps2) {
+    const float rc = kClose * static_cast<float>(fmax(w_target, w_source));
+    return __builtin_fmaf(rc, rc, eps2);
+}
+
+// Adds the double-float value (h, l) to the Kahan pair (s, c) (value s - c).
+__device__ __forceinline__ void kahan_df(float h, float l, float& s, float& c) {
+    kahan(h, s, c);
+    c = c - l;
+}
+
+// Targets' sums against one 64-source tile J.
+//
+//   * Far tiles: each target is re-expressed against the source tile's centre
+//     with one rounding (the FP64 difference of its (hi, lo) offset and the
+//     centres, rounded once); the terms are summed plainly in FP32 in runs of
+//     kRun and each run is Kahan-added per component into (s, k).
+//   * Near tiles: double-float separations, runs of kNearRun terms summed in
+//     FP32 and Kahan-added into the tile's own pair, close pairs in
+//     double-float arithmetic added to it directly; the tile's pair then
+//     joins (s, k).
+//
+// kUnroll sets how many far-path sources are in flight (their SGPR loads
+// included).
+constexpr int kRun = 16;
+// Terms of a near tile summed plainly in FP32 before each Kahan addition.
+constexpr int kNearRun = 8;
+
 template<int R, bool kSafe, bool kRefine, bool kGivenNear = false, int kUnroll = 16>
 __device__ __forceinline__ void source_tile(
     const float4* __restrict__ packed, const float4* __restrict__ packed_lo,
@@ -177,58 +256,109 @@ __device__ __forceinline__ void source_tile(
 #pragma unroll
         for (int r = 0; r < R; ++r) near = near || near_tiles(C, E, cx[r], cy[r], cz[r], ce[r]);
     }
-    float px[R], py[R], pz[R];
+    // Each target's position relative to the source tile's centre: FP64 from
+    // its (hi, lo) offset, split into (hi, lo) floats.
+    float th[R][3], tl[R][3];
 #pragma unroll
-    for (int r = 0; r < R; ++r) px[r] = py[r] = pz[r] = 0.f;
+    for (int r = 0; r < R; ++r) {
+        const double t[3] = {(static_cast<double>(ox[r]) + lx[r]) - (C.x - cx[r]),
+                             (static_cast<double>(oy[r]) + ly[r]) - (C.y - cy[r]),
+                             (static_cast<double>(oz[r]) + lz[r]) - (C.z - cz[r])};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            th[r][c] = static_cast<float>(t[c]);
+            tl[r][c] = static_cast<float>(t[c] - th[r][c]);
+        }
+    }
     if (!near) {
-        float ux[R], uy[R], uz[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            // Target offset re-expressed relative to the source tile's centre.
-            ux[r] = ox[r] - static_cast<float>(C.x - cx[r]);
-            uy[r] = oy[r] - static_cast<float>(C.y - cy[r]);
-            uz[r] = oz[r] - static_cast<float>(C.z - cz[r]);
-        }
-#pragma unroll kUnroll
-        for (int j = 0; j < kTile; ++j) {
-            const float4 q = src[j];
+        for (int run = 0; run < kTile / kRun; ++run) {
+            float px[R], py[R], pz[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                term<kSafe, kRefine>(q.x - ux[r], q.y - uy[r], q.z - uz[r], q.w, eps2, px[r],
-                                     py[r], pz[r]);
-        }
-    } else {
-        // Double-float target position relative to the source tile's centre.
-        float th[R][3], tl[R][3];
+            for (int r = 0; r < R; ++r) px[r] = py[r] = pz[r] = 0.f;
+#pragma unroll (kUnroll < kRun ? kUnroll : kRun)
+            for (int j = run * kRun; j < (run + 1) * kRun; ++j) {
+                const float4 q = src[j];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const double t[3] = {(static_cast<double>(ox[r]) + lx[r]) - (C.x - cx[r]),
-                                 (static_cast<double>(oy[r]) + ly[r]) - (C.y - cy[r]),
-                                 (static_cast<double>(oz[r]) + lz[r]) - (C.z - cz[r])};
+                for (int r = 0; r < R; ++r)
+                    term<kSafe, kRefine>(q.x - th[r][0], q.y - th[r][1], q.z - th[r][2], q.w, eps2,
+                                         px[r], py[r], pz[r]);
+            }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                th[r][c] = static_cast<float>(t[c]);
-                tl[r][c] = static_cast<float>(t[c] - th[r][c]);
+            for (int r = 0; r < R; ++r) {
+                kahan(px[r], sx[r], kx[r]);
+                kahan(py[r], sy[r], ky[r]);
+                kahan(pz[r], sz[r], kz[r]);
             }
         }
-        const float4* __restrict__ src_lo = packed_lo + static_cast<std::size_t>(J) * kTile;
-#pragma unroll 8
-        for (int j = 0; j < kTile; ++j) {
+        return;
+    }
+    const float4* __restrict__ src_lo = packed_lo + static_cast<std::size_t>(J) * kTile;
+    float px[R], py[R], pz[R], pkx[R], pky[R], pkz[R], lim[R];
+    std::uint64_t close[R];  // per lane: bit j set when source j is a close pair
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        px[r] = py[r] = pz[r] = pkx[r] = pky[r] = pkz[r] = 0.f;
+        lim[r] = close_limit(ce[r].w, E.w, eps2);
+        close[r] = 0;
+    }
+#pragma unroll 1
+    for (int j0 = 0; j0 < kTile; j0 += kNearRun) {
+        float qx[R], qy[R], qz[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) qx[r] = qy[r] = qz[r] = 0.f;
+#pragma unroll
+        for (int j = j0; j < j0 + kNearRun; ++j) {
             const float4 q = src[j];
             const float4 ql = src_lo[j];
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                term<kSafe, kRefine>((q.x - th[r][0]) + (ql.x - tl[r][0]),
-                                     (q.y - th[r][1]) + (ql.y - tl[r][1]),
-                                     (q.z - th[r][2]) + (ql.z - tl[r][2]), q.w, eps2, px[r],
-                                     py[r], pz[r]);
+            for (int r = 0; r < R; ++r) {
+                const float dx = (q.x - th[r][0]) + (ql.x - tl[r][0]);
+                const float dy = (q.y - th[r][1]) + (ql.y - tl[r][1]);
+                const float dz = (q.z - th[r][2]) + (ql.z - tl[r][2]);
+                float r2 = __builtin_fmaf(dx, dx, eps2);
+                r2 = __builtin_fmaf(dy, dy, r2);
+                r2 = __builtin_fmaf(dz, dz, r2);
+                // A close pair's FP32 term is left out here and added in
+                // double-float below.
+                const bool near_pair = r2 < lim[r] && r2 > 0.f;
+                close[r] |= near_pair ? std::uint64_t{1} << j : std::uint64_t{0};
+                const float f = near_pair ? 0.f : q.w * inv_cubed<kSafe, kRefine>(r2);
+                qx[r] = __builtin_fmaf(f, dx, qx[r]);
+                qy[r] = __builtin_fmaf(f, dy, qy[r]);
+                qz[r] = __builtin_fmaf(f, dz, qz[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            kahan(qx[r], px[r], pkx[r]);
+            kahan(qy[r], py[r], pky[r]);
+            kahan(qz[r], pz[r], pkz[r]);
+        }
+    }
+    // The close pairs, each lane over its own (few): sources read by index.
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        std::uint64_t bits = close[r];
+        while (bits != 0) {
+            const int j = __builtin_ctzll(bits);
+            bits &= bits - 1;
+            const float4 q = src[j];
+            const float4 ql = src_lo[j];
+            const float bh[3] = {q.x, q.y, q.z}, bl[3] = {ql.x, ql.y, ql.z};
+            DF D[3], G, T[3];
+            close_pair(th[r], tl[r], bh, bl, eps2, D, G);
+            close_term(q.w, G, D, T);
+            kahan_df(T[0].hi, T[0].lo, px[r], pkx[r]);
+            kahan_df(T[1].hi, T[1].lo, py[r], pky[r]);
+            kahan_df(T[2].hi, T[2].lo, pz[r], pkz[r]);
         }
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        kahan(px[r], sx[r], kx[r]);
-        kahan(py[r], sy[r], ky[r]);
-        kahan(pz[r], sz[r], kz[r]);
+        kahan_df(px[r], -pkx[r], sx[r], kx[r]);
+        kahan_df(py[r], -pky[r], sy[r], ky[r]);
+        kahan_df(pz[r], -pkz[r], sz[r], kz[r]);
     }
 }
 
@@ -520,6 +650,8 @@ __global__ __launch_bounds__(256) void batch_reduce(const float* __restrict__ pa
 // ceil(Q / s) chunks); items are numbered block by block, items_before(Q) of
 // them before block Q.
 constexpr int kStay = 8;
+// Steps between flushes of the streamed side's travelling sums (to LDS).
+constexpr int kFlush = 8;
 
 __host__ __device__ inline std::uint32_t pair_chunks(std::uint32_t Q, std::uint32_t S) {
     return (kStay * Q + S - 1) / S;
@@ -554,19 +686,23 @@ __device__ __forceinline__ float rotate_add(float a, float t) {
 // Four waves per SIMD (128 VGPRs): the stationary side holds 13 values per
 // tile per lane (offset re-expressed against the streamed tile, mass, the
 // streamed tile's partial sums, Kahan sums and compensations), 104 in all.
-template<bool kSafe, bool kRefine>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void direct_pairs(const float4* __restrict__ packed,
-                                                       const float4* __restrict__ packed_lo,
-                                                       const double4* __restrict__ centres,
-                                                       const std::uint32_t* __restrict__ mask,
-                                                       std::uint32_t tiles, std::uint32_t words,
-                                                       std::uint32_t blocks, std::uint32_t S,
-                                                       std::uint32_t items, float eps2,
-                                                       float* __restrict__ jpart,
-                                                       double* __restrict__ ipart) {
-    const std::uint32_t item =
-        __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kTile) + threadIdx.x / kTile);
-    if (item >= items) return;
+//
+// kEqual: every particle has one mass (the flag the mass pass leaves). The
+// masses then leave the pair term -- both sides add inv^3 d, two multiplies
+// fewer per pair and no travelling mass -- and the fold multiplies the pairs'
+// slots by the common mass.
+template<bool kSafe, bool kRefine, bool kEqual>
+__device__ __forceinline__ void pairs_item(const float4* __restrict__ packed,
+                                           const float4* __restrict__ packed_lo,
+                                           const double4* __restrict__ centres,
+                                           const std::uint32_t* __restrict__ mask,
+                                           std::uint32_t tiles, std::uint32_t words,
+                                           std::uint32_t blocks, std::uint32_t S,
+                                           std::uint32_t item, float eps2,
+                                           float* __restrict__ jpart,
+                                           double* __restrict__ ipart,
+                                           double* __restrict__ home,
+                                           float* __restrict__ comp) {
     // The item's block: the largest Q with pair_items_before(Q) <= item.
     std::uint32_t lo = 1, hi = blocks - 1;
     while (lo < hi) {
@@ -583,19 +719,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
 
     // Stationary tiles past the last (a partial last block) repeat the last
     // tile massless: they pull nothing and their sums are never read.
-    std::uint32_t stay[kStay];
-    float m[kStay], sx[kStay], sy[kStay], sz[kStay], kx[kStay], ky[kStay], kz[kStay];
+    // home: the wave's streamed-side sums by particle, in FP64 (LDS); the
+    // travelling accumulators are added there every kFlush steps (each lane to
+    // the slot of the particle it holds then), so no FP32 sum runs longer than
+    // kFlush steps of kStay terms. comp: this lane's stationary-side Kahan
+    // compensations (LDS, [(3 r + c) kTile]), touched once per streamed tile,
+    // so that the far loop keeps its registers.
+
+    float m[kStay], sx[kStay], sy[kStay], sz[kStay];
 #pragma unroll
     for (int r = 0; r < kStay; ++r) {
-        stay[r] = min(t0 + r, tiles - 1);
-        m[r] = t0 + r < tiles ? packed[static_cast<std::size_t>(stay[r]) * kTile + lane].w : 0.f;
-        sx[r] = sy[r] = sz[r] = kx[r] = ky[r] = kz[r] = 0.f;
+        if constexpr (kEqual) {
+            m[r] = 1.f;
+        } else {
+            m[r] = t0 + r < tiles ? packed[static_cast<std::size_t>(min(t0 + r, tiles - 1)) * kTile + lane].w : 0.f;
+        }
+        sx[r] = sy[r] = sz[r] = 0.f;
+        comp[(3 * r) * kTile] = comp[(3 * r + 1) * kTile] = comp[(3 * r + 2) * kTile] = 0.f;
     }
+    // Adds (h, l) to stationary sum (s, compensation of component c of tile r).
+    auto stay_add = [&](int r, int c, float h, float l, float& sum) {
+        float k = comp[(3 * r + c) * kTile];
+        kahan_df(h, l, sum, k);
+        comp[(3 * r + c) * kTile] = k;
+    };
 
     for (std::uint32_t I = i0; I < i1; ++I) {
         const double4 CI = centres[I];
         const float4 qi = packed[static_cast<std::size_t>(I) * kTile + lane];
-        float xi = qi.x, yi = qi.y, zi = qi.z, mi = qi.w;
+        float xi = qi.x, yi = qi.y, zi = qi.z, mi = kEqual ? 1.f : qi.w;
         std::uint32_t near = 0;
         float wx[kStay], wy[kStay], wz[kStay];
         // The stationary offsets and centres are re-read for every streamed
@@ -606,32 +758,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
 #pragma unroll
         for (int r = 0; r < kStay; ++r) {
             const std::uint32_t tile = min(first + r, tiles - 1);
-            near |= ((mask[static_cast<std::size_t>(stay[r]) * words + I / 32] >> (I & 31u)) & 1u) << r;
+            near |= ((mask[static_cast<std::size_t>(tile) * words + I / 32] >> (I & 31u)) & 1u) << r;
             const double4 C = centres[tile];
             const float4 o = packed[static_cast<std::size_t>(tile) * kTile + lane];
-            // The stationary offset re-expressed relative to the streamed tile's centre.
-            wx[r] = o.x - static_cast<float>(CI.x - C.x);
-            wy[r] = o.y - static_cast<float>(CI.y - C.y);
-            wz[r] = o.z - static_cast<float>(CI.z - C.z);
+            // The stationary offset re-expressed relative to the streamed
+            // tile's centre: the FP64 difference rounded once.
+            wx[r] = static_cast<float>(static_cast<double>(o.x) - (CI.x - C.x));
+            wy[r] = static_cast<float>(static_cast<double>(o.y) - (CI.y - C.y));
+            wz[r] = static_cast<float>(static_cast<double>(o.z) - (CI.z - C.z));
         }
         float px[kStay], py[kStay], pz[kStay];
 #pragma unroll
         for (int r = 0; r < kStay; ++r) px[r] = py[r] = pz[r] = 0.f;
-        float ax = 0.f, ay = 0.f, az = 0.f, cax = 0.f, cay = 0.f, caz = 0.f;
+        float ax = 0.f, ay = 0.f, az = 0.f;
         // d = x_i - x_j; j gains m_i d / r^3, i gains -m_j d / r^3.
         auto pair = [&](int r, float& tx, float& ty, float& tz) {
             const float dx = xi - wx[r], dy = yi - wy[r], dz = zi - wz[r];
             float r2 = __builtin_fmaf(dx, dx, eps2);
             r2 = __builtin_fmaf(dy, dy, r2);
             r2 = __builtin_fmaf(dz, dz, r2);
-            if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            float inv3 = (inv * inv) * inv;
-            if constexpr (kRefine) {
-                const float e = __builtin_fmaf(-(r2 * inv), inv, 1.f);
-                inv3 = inv3 * __builtin_fmaf(1.5f, e, 1.f);
-            }
-            const float fj = mi * inv3, fi = m[r] * inv3;
+            const float inv3 = inv_cubed<kSafe, kRefine>(r2);
+            const float fj = kEqual ? inv3 : mi * inv3, fi = kEqual ? inv3 : m[r] * inv3;
             px[r] = __builtin_fmaf(fj, dx, px[r]);
             py[r] = __builtin_fmaf(fj, dy, py[r]);
             pz[r] = __builtin_fmaf(fj, dz, pz[r]);
@@ -646,36 +793,61 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
             xi = wave_rotate(xi);
             yi = wave_rotate(yi);
             zi = wave_rotate(zi);
-            mi = wave_rotate(mi);
+            if constexpr (!kEqual) mi = wave_rotate(mi);
         };
-        if (near == 0) {
-#pragma unroll 1
-            for (int step = 0; step < kTile; ++step) {
-                float tx = 0.f, ty = 0.f, tz = 0.f;
-#pragma unroll
-                for (int r = 0; r < kStay; ++r) pair(r, tx, ty, tz);
-                advance(tx, ty, tz);
+        // After step s the accumulator on lane l belongs to streamed particle
+        // (l - s) mod 64. Flushes after steps kFlush - 1, 2 kFlush - 1, ...
+        auto flush_streamed = [&](std::uint32_t last_step) {
+            const unsigned slot = (lane - last_step) & (kTile - 1);
+            if (last_step == kFlush - 1) {
+                home[slot] = ax;
+                home[kTile + slot] = ay;
+                home[2 * kTile + slot] = az;
+            } else {
+                atomicAdd(&home[slot], static_cast<double>(ax));
+                atomicAdd(&home[kTile + slot], static_cast<double>(ay));
+                atomicAdd(&home[2 * kTile + slot], static_cast<double>(az));
             }
-        } else {
+            ax = ay = az = 0.f;
+        };
+        // Runs of kFlush steps, each followed by a flush of the streamed sums.
 #pragma unroll 1
-            for (int step = 0; step < kTile; ++step) {
-                float tx = 0.f, ty = 0.f, tz = 0.f;
+        for (std::uint32_t run = 0; run < kTile; run += kFlush) {
+            if (near == 0) {
+#pragma unroll 4
+                for (int u = 0; u < kFlush; ++u) {
+                    float tx = 0.f, ty = 0.f, tz = 0.f;
 #pragma unroll
-                for (int r = 0; r < kStay; ++r) {
-                    if (((near >> r) & 1u) == 0) pair(r, tx, ty, tz);
+                    for (int r = 0; r < kStay; ++r) pair(r, tx, ty, tz);
+                    advance(tx, ty, tz);
                 }
-                advance(tx, ty, tz);
+            } else {
+#pragma unroll 4
+                for (int u = 0; u < kFlush; ++u) {
+                    float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+                    for (int r = 0; r < kStay; ++r) {
+                        if (((near >> r) & 1u) == 0) pair(r, tx, ty, tz);
+                    }
+                    advance(tx, ty, tz);
+                }
             }
+            flush_streamed(run + kFlush - 1);
         }
-        // After a pass of 64 steps the streamed particles are home and each
-        // accumulator sits one lane short of home, which is where the next
-        // pass's first rotate_add expects it; one more rotation before the
-        // store brings it home.
+#pragma unroll
+        for (int r = 0; r < kStay; ++r) {
+            stay_add(r, 0, px[r], 0.f, sx[r]);
+            stay_add(r, 1, py[r], 0.f, sy[r]);
+            stay_add(r, 2, pz[r], 0.f, sz[r]);
+        }
         if (near != 0) {
-            // Near stationary tiles, one pass of 64 steps each, with the
-            // one-sided kernel's double-float separation: the streamed offset
-            // (hi, lo) against the stationary position taken relative to the
-            // streamed tile's centre in FP64 and split into (hi, lo).
+            // Near stationary tiles, one pass of 64 steps each: the one-sided
+            // kernel's double-float separation (the streamed offset (hi, lo)
+            // against the stationary position taken relative to the streamed
+            // tile's centre in FP64 and split into (hi, lo)); both sides sum
+            // runs of kFlush terms, the stationary runs Kahan-added, the
+            // streamed runs added to the FP64 slots. Close pairs are marked
+            // and evaluated afterwards in double-float arithmetic.
             const float4 il = packed_lo[static_cast<std::size_t>(I) * kTile + lane];
             float xl = il.x, yl = il.y, zl = il.z;
 #pragma unroll
@@ -694,56 +866,79 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
                     th[c] = static_cast<float>(t[c]);
                     tl[c] = static_cast<float>(t[c] - th[c]);
                 }
-                // The tile's 64 terms are summed on their own (as the one-sided
-                // kernel sums a tile) and added once.
-                float nx = 0.f, ny = 0.f, nz = 0.f;
+                const float lim = close_limit(C.w, CI.w, eps2);
+                const float mr = m[r];
+                std::uint64_t close = 0;  // bit s: this lane's pair at step s is close
+                float qx = 0.f, qy = 0.f, qz = 0.f;  // stationary run
+                ax = ay = az = 0.f;                  // streamed run (travelling)
 #pragma unroll 1
-                for (int step = 0; step < kTile; ++step) {
-                    const float dx = (xi - th[0]) + (xl - tl[0]);
-                    const float dy = (yi - th[1]) + (yl - tl[1]);
-                    const float dz = (zi - th[2]) + (zl - tl[2]);
-                    float r2 = __builtin_fmaf(dx, dx, eps2);
-                    r2 = __builtin_fmaf(dy, dy, r2);
-                    r2 = __builtin_fmaf(dz, dz, r2);
-                    if constexpr (kSafe) r2 = r2 > 0.f ? r2 : 1.f;
-                    const float inv = __builtin_amdgcn_rsqf(r2);
-                    float inv3 = (inv * inv) * inv;
-                    if constexpr (kRefine) {
-                        const float e = __builtin_fmaf(-(r2 * inv), inv, 1.f);
-                        inv3 = inv3 * __builtin_fmaf(1.5f, e, 1.f);
+                for (std::uint32_t run = 0; run < kTile; run += kFlush) {
+#pragma unroll 2
+                    for (std::uint32_t u = 0; u < kFlush; ++u) {
+                        const std::uint32_t step = run + u;
+                        const float dx = (xi - th[0]) + (xl - tl[0]);
+                        const float dy = (yi - th[1]) + (yl - tl[1]);
+                        const float dz = (zi - th[2]) + (zl - tl[2]);
+                        float r2 = __builtin_fmaf(dx, dx, eps2);
+                        r2 = __builtin_fmaf(dy, dy, r2);
+                        r2 = __builtin_fmaf(dz, dz, r2);
+                        const bool near_pair = r2 < lim && r2 > 0.f;
+                        close |= near_pair ? std::uint64_t{1} << step : std::uint64_t{0};
+                        const float inv3 = near_pair ? 0.f : inv_cubed<kSafe, kRefine>(r2);
+                        const float fj = kEqual ? inv3 : mi * inv3, fi = kEqual ? inv3 : mr * inv3;
+                        qx = __builtin_fmaf(fj, dx, qx);
+                        qy = __builtin_fmaf(fj, dy, qy);
+                        qz = __builtin_fmaf(fj, dz, qz);
+                        ax = rotate_add(ax, -fi * dx);
+                        ay = rotate_add(ay, -fi * dy);
+                        az = rotate_add(az, -fi * dz);
+                        xi = wave_rotate(xi);
+                        yi = wave_rotate(yi);
+                        zi = wave_rotate(zi);
+                        if constexpr (!kEqual) mi = wave_rotate(mi);
+                        xl = wave_rotate(xl);
+                        yl = wave_rotate(yl);
+                        zl = wave_rotate(zl);
                     }
-                    const float fj = mi * inv3, fi = m[r] * inv3;
-                    px[r] = __builtin_fmaf(fj, dx, px[r]);
-                    py[r] = __builtin_fmaf(fj, dy, py[r]);
-                    pz[r] = __builtin_fmaf(fj, dz, pz[r]);
-                    nx = rotate_add(nx, -fi * dx);
-                    ny = rotate_add(ny, -fi * dy);
-                    nz = rotate_add(nz, -fi * dz);
-                    xi = wave_rotate(xi);
-                    yi = wave_rotate(yi);
-                    zi = wave_rotate(zi);
-                    mi = wave_rotate(mi);
-                    xl = wave_rotate(xl);
-                    yl = wave_rotate(yl);
-                    zl = wave_rotate(zl);
+                    const unsigned slot = (lane - (run + kFlush - 1)) & (kTile - 1);
+                    atomicAdd(&home[slot], static_cast<double>(ax));
+                    atomicAdd(&home[kTile + slot], static_cast<double>(ay));
+                    atomicAdd(&home[2 * kTile + slot], static_cast<double>(az));
+                    ax = ay = az = 0.f;
+                    stay_add(r, 0, qx, 0.f, sx[r]);
+                    stay_add(r, 1, qy, 0.f, sy[r]);
+                    stay_add(r, 2, qz, 0.f, sz[r]);
+                    qx = qy = qz = 0.f;
                 }
-                // Both sums are one lane short of home: Kahan-added lane by lane.
-                kahan(nx, ax, cax);
-                kahan(ny, ay, cay);
-                kahan(nz, az, caz);
+                // Close pairs: at step s this lane met streamed particle
+                // (lane - s) mod 64, read back here by index.
+                while (close != 0) {
+                    const unsigned step = __builtin_ctzll(close);
+                    close &= close - 1;
+                    const unsigned p = (lane - step) & (kTile - 1);
+                    const std::size_t k = static_cast<std::size_t>(I) * kTile + p;
+                    const float4 b = packed[k];
+                    const float4 bl4 = packed_lo[k];
+                    const float bh[3] = {b.x, b.y, b.z}, bl[3] = {bl4.x, bl4.y, bl4.z};
+                    DF D[3], G, T[3];
+                    close_pair(th, tl, bh, bl, eps2, D, G);
+                    close_term(kEqual ? 1.f : b.w, G, D, T);
+                    stay_add(r, 0, T[0].hi, T[0].lo, sx[r]);
+                    stay_add(r, 1, T[1].hi, T[1].lo, sy[r]);
+                    stay_add(r, 2, T[2].hi, T[2].lo, sz[r]);
+                    close_term(mr, G, D, T);
+                    atomicAdd(&home[p], -(static_cast<double>(T[0].hi) + T[0].lo));
+                    atomicAdd(&home[kTile + p], -(static_cast<double>(T[1].hi) + T[1].lo));
+                    atomicAdd(&home[2 * kTile + p], -(static_cast<double>(T[2].hi) + T[2].lo));
+                }
             }
         }
-        // The slot holds sum - compensation in FP64.
+        // The streamed particles are home: each lane's sum over the block is in
+        // its FP64 slot.
         double* out = ipart + pair_slot(Q, I) * 3 * kTile;
-        out[lane] = static_cast<double>(wave_rotate(ax)) - static_cast<double>(wave_rotate(cax));
-        out[kTile + lane] = static_cast<double>(wave_rotate(ay)) - static_cast<double>(wave_rotate(cay));
-        out[2 * kTile + lane] = static_cast<double>(wave_rotate(az)) - static_cast<double>(wave_rotate(caz));
-#pragma unroll
-        for (int r = 0; r < kStay; ++r) {
-            kahan(px[r], sx[r], kx[r]);
-            kahan(py[r], sy[r], ky[r]);
-            kahan(pz[r], sz[r], kz[r]);
-        }
+        out[lane] = home[lane];
+        out[kTile + lane] = home[kTile + lane];
+        out[2 * kTile + lane] = home[2 * kTile + lane];
     }
     float* out = jpart + static_cast<std::size_t>(item) * 6 * kStay * kTile;
     constexpr int kRow = kStay * kTile;
@@ -753,10 +948,37 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
         out[k] = sx[r];
         out[kRow + k] = sy[r];
         out[2 * kRow + k] = sz[r];
-        out[3 * kRow + k] = kx[r];
-        out[4 * kRow + k] = ky[r];
-        out[5 * kRow + k] = kz[r];
+        out[3 * kRow + k] = comp[(3 * r) * kTile];
+        out[4 * kRow + k] = comp[(3 * r + 1) * kTile];
+        out[5 * kRow + k] = comp[(3 * r + 2) * kTile];
     }
+}
+
+// One launch per mass case, both enqueued; the mass pass's flag (uniform[0] >
+// 0: every particle's mass) lets one of them run and the other exit at once,
+// so the host never waits for the flag and each case keeps its own registers.
+template<bool kSafe, bool kRefine, bool kEqual>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void direct_pairs(const float4* __restrict__ packed,
+                                                       const float4* __restrict__ packed_lo,
+                                                       const double4* __restrict__ centres,
+                                                       const std::uint32_t* __restrict__ mask,
+                                                       std::uint32_t tiles, std::uint32_t words,
+                                                       std::uint32_t blocks, std::uint32_t S,
+                                                       std::uint32_t items, float eps2,
+                                                       const float* __restrict__ uniform,
+                                                       float* __restrict__ jpart,
+                                                       double* __restrict__ ipart) {
+    if ((uniform[0] > 0.f) != kEqual) return;
+    const std::uint32_t item =
+        __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kTile) + threadIdx.x / kTile);
+    if (item >= items) return;
+    __shared__ double flushed[kBlock / kTile][3][kTile];
+    __shared__ float compensation[kBlock / kTile][3 * kStay][kTile];
+    const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);
+    double* const home = &flushed[wave][0][0];
+    float* const comp = &compensation[wave][0][threadIdx.x & (kTile - 1)];
+    pairs_item<kSafe, kRefine, kEqual>(packed, packed_lo, centres, mask, tiles, words, blocks, S,
+                                       item, eps2, jpart, ipart, home, comp);
 }
 
 // What the pairs kernel leaves out: the pairs inside each block, one-sided.
@@ -805,6 +1027,7 @@ __global__ __launch_bounds__(kBlock) void direct_complement(const float4* __rest
 __global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jpart,
                                                     const double* __restrict__ ipart,
                                                     const float* __restrict__ cpart,
+                                                    const float* __restrict__ uniform,
                                                     std::uint32_t tiles, std::uint32_t blocks,
                                                     std::uint32_t S, std::uint64_t n,
                                                     const std::uint32_t* __restrict__ order,
@@ -833,6 +1056,11 @@ __global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jp
         a1 += p[kTile];
         a2 += p[2 * kTile];
     }
+    // The pairs kernel's slots leave out a common mass; the complement's not.
+    const double scale = uniform[0] > 0.f ? static_cast<double>(uniform[0]) : 1.0;
+    a0 *= scale;
+    a1 *= scale;
+    a2 *= scale;
     const std::size_t np = static_cast<std::size_t>(tiles) * kTile;
     a0 += static_cast<double>(cpart[k]) - static_cast<double>(cpart[3 * np + k]);
     a1 += static_cast<double>(cpart[np + k]) - static_cast<double>(cpart[4 * np + k]);
@@ -920,13 +1148,17 @@ PairPlan pair_plan(std::uint32_t tiles, bool masked, int mode) {
 template<bool kSafe, bool kRefine>
 void launch_pairs(orr_ctx* ctx, const PairPlan& plan, const float4* packed, const float4* packed_lo,
                   const double4* centres, const float4* extents, const std::uint32_t* mask,
-                  std::uint32_t tiles, std::uint32_t words, float eps2, float* jpart, double* ipart,
-                  float* cpart) {
+                  std::uint32_t tiles, std::uint32_t words, float eps2, const float* uniform,
+                  float* jpart, double* ipart, float* cpart) {
     constexpr std::uint32_t kWaves = kBlock / kTile;
     const std::uint32_t items = static_cast<std::uint32_t>(plan.items);
-    hipLaunchKernelGGL((direct_pairs<kSafe, kRefine>), dim3((items + kWaves - 1) / kWaves),
+    hipLaunchKernelGGL((direct_pairs<kSafe, kRefine, true>), dim3((items + kWaves - 1) / kWaves),
                        dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, mask, tiles, words, plan.blocks,
-                       plan.S, items, eps2, jpart, ipart);
+                       plan.S, items, eps2, uniform, jpart, ipart);
+    ORR_LAUNCHED("direct_pairs");
+    hipLaunchKernelGGL((direct_pairs<kSafe, kRefine, false>), dim3((items + kWaves - 1) / kWaves),
+                       dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, mask, tiles, words, plan.blocks,
+                       plan.S, items, eps2, uniform, jpart, ipart);
     ORR_LAUNCHED("direct_pairs");
     hipLaunchKernelGGL((direct_complement<kSafe, kRefine>), dim3((tiles + kWaves - 1) / kWaves),
                        dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, extents, mask, tiles,
@@ -955,7 +1187,7 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
     const PairPlan plan = pair_plan(tiles, masked, ctx->direct_pair_mode);
     const std::size_t cpart_floats = 6 * static_cast<std::size_t>(npad);
     ctx->direct_packed.reserve(2 * sizeof(float4) * npad);
-    ctx->direct_centres.reserve((sizeof(double4) + sizeof(float4)) * tiles);
+    ctx->direct_centres.reserve((sizeof(double4) + sizeof(float4) + sizeof(float2)) * tiles + 16);
     ctx->direct_partials.reserve(
         plan.use ? sizeof(double) * plan.ipart_doubles + sizeof(float) * (plan.jpart_floats + cpart_floats)
                  : sizeof(float) * 6 * static_cast<std::size_t>(npad) * splits);
@@ -963,6 +1195,8 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
     float4* packed_lo = packed + npad;
     double4* centres = ctx->direct_centres.as<double4>();
     float4* extents = reinterpret_cast<float4*>(centres + tiles);
+    float2* mass_range = reinterpret_cast<float2*>(extents + tiles);
+    float* uniform = reinterpret_cast<float*>(mass_range + tiles);
     float* partials = ctx->direct_partials.as<float>();
 
     const std::uint32_t words = (tiles + 31) / 32;
@@ -974,8 +1208,13 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
     {
         PhaseTimer t(ctx, kDirectPrepare);
         hipLaunchKernelGGL(direct_prepare, dim3(tiles), dim3(kTile), 0, ctx->stream, a.x, a.y, a.z,
-                           a.m, order, n, packed, packed_lo, centres, extents);
+                           a.m, order, n, packed, packed_lo, centres, extents, mass_range);
         ORR_LAUNCHED("direct_prepare");
+        if (plan.use) {
+            hipLaunchKernelGGL(mass_uniform, dim3(1), dim3(256), 0, ctx->stream, mass_range, tiles,
+                               ctx->direct_equal_masses == 1 ? 1 : 0, uniform);
+            ORR_LAUNCHED("mass_uniform");
+        }
         if (masked) {
             hipLaunchKernelGGL(near_mask, dim3((tiles + kMaskRows - 1) / kMaskRows), dim3(256), 0,
                                ctx->stream, centres, extents, tiles, words, mask);
@@ -994,14 +1233,14 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
             PhaseTimer t(ctx, kDirectKernel);
             auto launch = safe ? (refine ? launch_pairs<true, true> : launch_pairs<true, false>)
                                : (refine ? launch_pairs<false, true> : launch_pairs<false, false>);
-            launch(ctx, plan, packed, packed_lo, centres, extents, mask, tiles, words, eps2, jpart,
-                   ipart, cpart);
+            launch(ctx, plan, packed, packed_lo, centres, extents, mask, tiles, words, eps2, uniform,
+                   jpart, ipart, cpart);
         }
         {
             PhaseTimer t(ctx, kDirectReduce);
             const std::uint32_t blocks = static_cast<std::uint32_t>((n + 255) / 256);
             hipLaunchKernelGGL(pairs_reduce, dim3(blocks), dim3(256), 0, ctx->stream, jpart, ipart,
-                               cpart, tiles, plan.blocks, plan.S, n, order,
+                               cpart, uniform, tiles, plan.blocks, plan.S, n, order,
                                a.gravitational_constant, a.ax, a.ay, a.az);
             ORR_LAUNCHED("pairs_reduce");
         }

@@ -125,6 +125,8 @@ struct orr_ctx {
     // 1: far tile pairs once for both tiles where that is faster and fits,
     // 2: one-sided, 3: pairs wherever they fit.
     int direct_pair_mode = 1;
+    // 1: the pairs path's equal-mass form where the masses allow it, 2: never.
+    int direct_equal_masses = 1;
     // Barnes-Hut walk configuration (orr_ctx_set_tree_config); 0 chooses by size.
     int walk_targets_per_lane = 0;
 

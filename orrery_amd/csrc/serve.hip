@@ -111,6 +111,7 @@ struct orr_server {
     std::map<GroupKey, Group> groups;
     std::uint64_t resident = 0, peak = 0, budget = 0;
     std::uint64_t rounds = 0, builds = 0, particle_steps = 0, completed = 0;
+    std::uint32_t fail_rebuilds = 0;  // testing aid: rebuilds still to fail
     double busy = 0;
 
     double now() const {
@@ -144,66 +145,76 @@ std::uint64_t request_bytes(const orr_server* s, std::uint64_t count) {
            orr::grown_bytes(orr::diagnostics_scratch_bytes(count)) + kBytesPerRequest;
 }
 
+// Builds the group's batch from its members. Nothing the group or its
+// requests hold changes until the new batch is complete: a failure on the way
+// (an allocation, an upload, a device-to-device copy) leaves the old batch, the
+// slots and every new request's host copy as they were, so a later round can
+// try again.
 void rebuild(orr_server* s, const GroupKey& key, Group& g) {
     orr_batch* old = g.batch;
-    g.batch = nullptr;
-    if (!g.members.empty()) {
-        std::vector<orr_batch_system> sys;
-        std::uint64_t particles = 0;
-        for (std::uint64_t id : g.members) {
-            const orr_request& q = s->requests.at(id).spec;
-            sys.push_back(orr_batch_system{q.count, -1, 0, q.softening_squared,
-                                           q.gravitational_constant, q.timestep});
-            particles += q.count;
-        }
-        const orr_batch_config bc = batch_config(s, key.integrator);
-        orr_batch* nb = nullptr;
-        call(orr_batch_create(s->ctx, sys.data(), static_cast<std::uint32_t>(sys.size()), &bc, &nb));
-        try {
-            std::map<std::uint64_t, std::uint32_t> slot;
-            for (std::uint32_t k = 0; k < g.members.size(); ++k) {
-                const std::uint64_t id = g.members[k];
-                Request& r = s->requests.at(id);
-                slot[id] = k;
-                if (r.fresh) {
-                    call(orr_batch_upload(nb, k, r.host[0].data(), r.host[1].data(), r.host[2].data(),
-                                          r.host[3].data(), r.host[4].data(), r.host[5].data(),
-                                          r.host[6].data()));
-                    for (auto& v : r.host) std::vector<double>().swap(v);
-                    r.fresh = false;
-                    // Start diagnostics on the batch's stream, ahead of its steps.
-                    orr::batch_diagnostics_enqueue(nb, k, r.diag);
-                } else {
-                    call(orr_batch_copy_system(nb, k, old, g.slot.at(id)));
-                }
-            }
-            g.slot = std::move(slot);
-        } catch (...) {
-            orr_batch_destroy(nb);
-            g.batch = old;
-            throw;
-        }
-        g.batch = nb;
-        // The copies are done (they synchronise): the old batch goes before the
-        // new one allocates its scratch, so the two never hold scratch at once.
-        if (old != nullptr) {
-            orr_batch* gone = old;
-            old = nullptr;
-            call(orr_batch_destroy(gone));
-        }
-        // Accelerations for the current positions (a continuing request's
-        // come out as the bits it already had), enqueued before the steps.
-        orr::batch_refresh_enqueue(nb);
-        if (g.particles > 0 && g.ms_per_step > 0) {
-            g.ms_per_step *= static_cast<double>(particles) / static_cast<double>(g.particles);
-        }
-        g.particles = particles;
-        ++s->builds;
-    } else {
+    if (g.members.empty()) {
+        g.batch = nullptr;
         g.slot.clear();
         g.particles = 0;
+        g.dirty = false;
+        if (old != nullptr) call(orr_batch_destroy(old));
+        return;
     }
+    std::vector<orr_batch_system> sys;
+    std::uint64_t particles = 0;
+    for (std::uint64_t id : g.members) {
+        const orr_request& q = s->requests.at(id).spec;
+        sys.push_back(orr_batch_system{q.count, -1, 0, q.softening_squared, q.gravitational_constant,
+                                       q.timestep});
+        particles += q.count;
+    }
+    const orr_batch_config bc = batch_config(s, key.integrator);
+    orr_batch* nb = nullptr;
+    call(orr_batch_create(s->ctx, sys.data(), static_cast<std::uint32_t>(sys.size()), &bc, &nb));
+    std::map<std::uint64_t, std::uint32_t> slot;
+    try {
+        for (std::uint32_t k = 0; k < g.members.size(); ++k) {
+            const std::uint64_t id = g.members[k];
+            Request& r = s->requests.at(id);
+            slot[id] = k;
+            if (r.fresh) {
+                call(orr_batch_upload(nb, k, r.host[0].data(), r.host[1].data(), r.host[2].data(),
+                                      r.host[3].data(), r.host[4].data(), r.host[5].data(),
+                                      r.host[6].data()));
+                // Start diagnostics on the batch's stream, ahead of its steps.
+                orr::batch_diagnostics_enqueue(nb, k, r.diag);
+            } else {
+                call(orr_batch_copy_system(nb, k, old, g.slot.at(id)));
+            }
+        }
+        if (s->fail_rebuilds > 0) {
+            --s->fail_rebuilds;
+            orr::fail(ORR_ERR_RUNTIME, "injected failure of a batch rebuild (orr_server_config.fail_rebuilds)");
+        }
+    } catch (...) {
+        (void)orr_batch_destroy(nb);
+        throw;
+    }
+    // Commit: the new batch holds every member.
+    g.batch = nb;
+    g.slot = std::move(slot);
+    for (std::uint64_t id : g.members) {
+        Request& r = s->requests.at(id);
+        if (!r.fresh) continue;
+        for (auto& v : r.host) std::vector<double>().swap(v);
+        r.fresh = false;
+    }
+    if (g.particles > 0 && g.ms_per_step > 0) {
+        g.ms_per_step *= static_cast<double>(particles) / static_cast<double>(g.particles);
+    }
+    g.particles = particles;
+    ++s->builds;
+    // The copies are done: the old batch goes before the new one allocates its
+    // scratch, so the two never hold scratch at once.
     if (old != nullptr) call(orr_batch_destroy(old));
+    // Accelerations for the current positions (a continuing request's come
+    // out as the bits it already had), enqueued before the steps.
+    orr::batch_refresh_enqueue(nb);
     g.dirty = false;
 }
 
@@ -364,6 +375,7 @@ int orr_server_create(orr_ctx* ctx, const orr_server_config* config, orr_server*
             budget = static_cast<std::uint64_t>(std::min(kMaxBudget, 0.9 * static_cast<double>(free_b)));
         }
         s->budget = budget;
+        s->fail_rebuilds = config->fail_rebuilds;
         s->t0 = std::chrono::steady_clock::now();
         *out = s.release();
     });

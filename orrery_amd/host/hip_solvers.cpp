@@ -40,7 +40,6 @@ public:
         if (orr_device_count(&count) != ORR_OK || count <= device || device < 0) return nullptr;
         orr_ctx* ctx = nullptr;
         if (orr_ctx_create(device, &ctx) != ORR_OK) return nullptr;
-        (void)orr_ctx_set_timing(ctx, 1);
         return std::unique_ptr<Device>(new Device(ctx));
     }
     ~Device() {
@@ -94,9 +93,11 @@ public:
         h.gathering = t.milliseconds[6];
         h.construction = t.milliseconds[7] + t.milliseconds[8];
         h.traversal = t.milliseconds[9];
+        h.graph_launches = t.graph_launches;
         return h;
     }
     void reset_timings() { check(orr_ctx_reset_timings(ctx_)); }
+    void set_timing(bool enabled) { check(orr_ctx_set_timing(ctx_, enabled ? 1 : 0)); }
 
 private:
     explicit Device(orr_ctx* ctx) : ctx_(ctx) {}
@@ -183,6 +184,7 @@ InteractionCount HipDirectSolver::interaction_count() const noexcept { return im
 void HipDirectSolver::reset_interaction_count() noexcept { impl_->count = InteractionCount{}; }
 HipTimings HipDirectSolver::timings() const { return impl_->device->timings(); }
 void HipDirectSolver::reset_timings() { impl_->device->reset_timings(); }
+void HipDirectSolver::set_timing(bool enabled) { impl_->device->set_timing(enabled); }
 
 // ---- HipTreeSolver -----------------------------------------------------------
 
@@ -253,5 +255,6 @@ HipTreeSummary HipTreeSolver::tree() const noexcept { return impl_->summary; }
 std::uint64_t HipTreeSolver::node_visits() const noexcept { return impl_->visits; }
 HipTimings HipTreeSolver::timings() const { return impl_->device->timings(); }
 void HipTreeSolver::reset_timings() { impl_->device->reset_timings(); }
+void HipTreeSolver::set_timing(bool enabled) { impl_->device->set_timing(enabled); }
 
 }  // namespace orrery::solvers

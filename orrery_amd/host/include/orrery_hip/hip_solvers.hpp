@@ -41,7 +41,12 @@ private:
 };
 
 /// Per-phase device time of the evaluations since the last reset, in
-/// milliseconds, measured with hipEvents on the engine's stream.
+/// milliseconds, measured with hipEvents on the engine's stream while timing
+/// is on (`set_timing(true)`). Timing is off by default: then repeated
+/// evaluations of one size replay a captured hipGraph (counted in
+/// `graph_launches`) and the phase times stay zero; with timing on every
+/// evaluation runs eagerly so that its phases can be timed. The results are
+/// the same bits either way.
 struct HipTimings {
     double prepare{};
     double kernel{};
@@ -50,6 +55,7 @@ struct HipTimings {
     double construction{}; // tree build and moments
     double gathering{};
     double traversal{};
+    std::uint64_t graph_launches{};
 };
 
 /// A summary of the last tree (Octree::leaf_count / depth, octree.hpp:229-236).
@@ -79,6 +85,7 @@ public:
 
     [[nodiscard]] HipTimings timings() const;
     void reset_timings();
+    void set_timing(bool enabled);
 
 private:
     struct Impl;
@@ -110,6 +117,7 @@ public:
     [[nodiscard]] std::uint64_t node_visits() const noexcept;
     [[nodiscard]] HipTimings timings() const;
     void reset_timings();
+    void set_timing(bool enabled);
 
 private:
     struct Impl;

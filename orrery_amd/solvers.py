@@ -118,15 +118,15 @@ class Context:
 
     # -- direct kernel configuration (autotuner knob) -----------------------
     def set_direct_config(self, targets_per_lane: int = 0, refine: int = -1, block_target: int = 0,
-                          pair_mode: int = 0) -> None:
-        c = C.OrrDirectConfig(targets_per_lane, refine, block_target, pair_mode)
+                          pair_mode: int = 0, equal_masses: int = 0) -> None:
+        c = C.OrrDirectConfig(targets_per_lane, refine, block_target, pair_mode, equal_masses, 0)
         C.check(self._lib.orr_ctx_set_direct_config(self.handle, ctypes.byref(c)))
 
     def direct_config(self) -> dict:
         c = C.OrrDirectConfig()
         C.check(self._lib.orr_ctx_get_direct_config(self.handle, ctypes.byref(c)))
         return dict(targets_per_lane=c.targets_per_lane, refine=c.refine, block_target=c.block_target,
-                    pair_mode=c.pair_mode)
+                    pair_mode=c.pair_mode, equal_masses=c.equal_masses)
 
     def set_tree_config(self, targets_per_lane: int = 0) -> None:
         c = C.OrrTreeConfig(targets_per_lane)
@@ -160,6 +160,12 @@ class Context:
 
     def reset_timings(self) -> None:
         C.check(self._lib.orr_ctx_reset_timings(self.handle))
+
+    def graph_launches(self) -> int:
+        """Evaluations replayed as one captured hipGraph since the last reset."""
+        t = C.OrrTimings()
+        C.check(self._lib.orr_ctx_timings(self.handle, ctypes.byref(t)))
+        return int(t.graph_launches)
 
 
 class DeviceArray:
@@ -688,12 +694,14 @@ class ServerConfig:
     tiles_per_item: int = 16
     quadrupole: bool = False
     use_graph: bool = True
+    fail_rebuilds: int = 0  # testing aid: the first n batch rebuilds fail
 
     def native(self) -> C.OrrServerConfig:
         return C.OrrServerConfig(int(self.memory_budget), float(self.round_ms), int(self.direct_below),
                                  int(self.alone_from), float(self.opening_angle),
                                  int(self.leaf_capacity), int(self.tiles_per_item),
-                                 int(bool(self.quadrupole)), int(bool(self.use_graph)))
+                                 int(bool(self.quadrupole)), int(bool(self.use_graph)),
+                                 int(self.fail_rebuilds), 0)
 
 
 class Server:

@@ -1,27 +1,35 @@
 """Synthetic workloads for the benchmark and the serving loop (numpy, seeded).
 
-* ``plummer`` -- the model of initial_conditions/plummer.cpp:43-87: radii by
-  the inverse mass CDF a / sqrt(u^(-2/3) - 1) truncated at a mass fraction,
-  speeds q v_esc with q drawn by rejection under the envelope 0.1 on
-  q^2 (1 - q^2)^(7/2), isotropic directions, then the centre-of-mass frame
-  (centre_of_mass_frame.cpp). Drawn from numpy's PCG64, so the bits differ
-  from the reference's mt19937_64 stream; nothing here is compared bit for bit
-  with the reference generator.
+* ``plummer`` -- the reference's own seeded sample
+  (initial_conditions/plummer.cpp:91-128 over core/random.cpp:12-47): the
+  engine's host-side ``orr_plummer_sample`` draws the same mt19937_64 stream
+  in the same order, so seed s gives the particles the reference's CPU path
+  gives for seed s, bit for bit. Radii by the inverse mass CDF
+  a / sqrt(u^(-2/3) - 1) truncated at a mass fraction, speeds q v_esc with q
+  drawn by rejection under the envelope 0.1 on q^2 (1 - q^2)^(7/2), isotropic
+  directions, then the centre-of-mass frame.
 * ``disc_collision`` -- two exponential discs (sech^2 vertical profile,
   truncated at ``truncation`` scale lengths) with Hernquist bulges, circular
   speeds from the enclosed mass plus Gaussian dispersion, a fraction of the
   disc particles inclined, the pair on a parabolic orbit of given pericentre
   and initial separation.
-* ``batched_spheres`` -- the uneven batch: sizes log-uniform in [lo, hi].
+* ``batched_spheres`` -- the uneven batch: sizes log-uniform in [lo, hi]
+  (numpy), each sphere the reference's seeded Plummer sample.
+
+The disc model is this project's own (the two-disc configuration differs from
+the reference's galaxy generator) and draws from numpy's PCG64.
 
 These are workload generators only: the engine consumes whatever positions and
 masses it is given.
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
+
+from . import _capi as C
 
 PLUMMER_CUTOFF = (100.0 / 101.0) ** 1.5  # r < 10 a for a = 1
 
@@ -69,24 +77,13 @@ def _com_frame(pos: np.ndarray, vel: np.ndarray, m: np.ndarray) -> Workload:
 
 def plummer(n: int, seed: int, total_mass: float = 1.0, scale_radius: float = 1.0,
             cutoff: float = PLUMMER_CUTOFF) -> Workload:
+    """The reference's Plummer sphere for ``seed`` (a = 1, r < 10 a by default)."""
     n = int(n)
-    rng = _rng(seed)
-    a, M = float(scale_radius), float(total_mass)
-    mf = rng.uniform(0.0, cutoff, n)
-    r = a / np.sqrt(mf ** (-2.0 / 3.0) - 1.0)
-    q = np.empty(n)
-    todo = np.arange(n)
-    while todo.size:
-        qq = rng.uniform(size=todo.size)
-        yy = 0.1 * rng.uniform(size=todo.size)
-        c = 1.0 - qq * qq
-        ok = yy < qq * qq * c * c * c * np.sqrt(c)
-        q[todo[ok]] = qq[ok]
-        todo = todo[~ok]
-    speed = q * np.sqrt(2.0 * M / np.sqrt(r * r + a * a))
-    pos = r * _unit_vectors(rng, n)
-    vel = speed * _unit_vectors(rng, n)
-    return _com_frame(pos, vel, np.full(n, M / n))
+    arrays = [np.zeros(n) for _ in range(7)]
+    ptr = ctypes.POINTER(ctypes.c_double)
+    C.check(C.load().orr_plummer_sample(n, float(total_mass), float(scale_radius), float(cutoff),
+                                        int(seed), *(a.ctypes.data_as(ptr) for a in arrays)))
+    return Workload(*arrays)
 
 
 class _Disc:

@@ -136,6 +136,7 @@ void direct_cases() {
     // Agreement with FP64 on a random cluster.
     System c = random_cluster(3000, 7);
     auto s2 = HipDirectSolver::try_create(Softening{0.05});
+    s2->set_timing(true);
     s2->evaluate(c.positions(), c.masses(), c.accelerations());
     std::vector<double> rx, ry, rz;
     fp64_direct(c, 0.05 * 0.05, rx, ry, rz);
@@ -145,6 +146,17 @@ void direct_cases() {
     CHECK(s2->timings().kernel > 0);
     // Ordered pairs, N(N-1).
     CHECK(s2->interaction_count().particle_particle == 3000ull * 2999ull);
+    // Timing off (the default): later evaluations of one size replay the
+    // captured ordering graph and give the same bits as the timed, eager one.
+    s2->set_timing(false);
+    s2->reset_timings();
+    const std::vector<double> eager_x = c.ax, eager_y = c.ay, eager_z = c.az;
+    for (int k = 0; k < 4; ++k) {
+        s2->evaluate(c.positions(), c.masses(), c.accelerations());
+        CHECK(c.ax == eager_x && c.ay == eager_y && c.az == eager_z);
+    }
+    CHECK(s2->timings().graph_launches >= 2);
+    CHECK(s2->timings().kernel == 0);
     // Mismatched spans are a usage error.
     System bad(3);
     bad.m.resize(2);
@@ -176,6 +188,22 @@ void tree_cases() {
     CHECK(solver->interaction_count().particle_particle > 0);
     CHECK(solver->interaction_count().particle_cell > 0);
     CHECK(solver->tree().leaf_count > 0 && solver->tree().leaf_count <= solver->tree().node_count);
+    // Untimed by default: the second evaluation of a size is captured and the
+    // later ones replay the graph, bit for bit the eager result; with timing
+    // on the evaluation runs eagerly, again with the same bits.
+    const std::vector<double> first_x = c.ax, first_y = c.ay, first_z = c.az;
+    solver->reset_timings();
+    for (int k = 0; k < 4; ++k) {
+        solver->evaluate(c.positions(), c.masses(), c.accelerations());
+        CHECK(c.ax == first_x && c.ay == first_y && c.az == first_z);
+    }
+    CHECK(solver->timings().graph_launches >= 2);
+    CHECK(solver->timings().traversal == 0);
+    solver->set_timing(true);
+    solver->evaluate(c.positions(), c.masses(), c.accelerations());
+    CHECK(c.ax == first_x && c.ay == first_y && c.az == first_z);
+    CHECK(solver->timings().traversal > 0);
+    solver->set_timing(false);
     // theta = 0: the direct sum, N(N-1) pairs, no cells (barnes_hut_test.cpp:111-152).
     p.opening_angle = 0;
     auto exact = HipTreeSolver::try_create(p, Softening{0.05});

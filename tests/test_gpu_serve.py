@@ -143,3 +143,37 @@ def test_invalid_requests(ctx):
     with pytest.raises(E.solvers.C.EngineError):
         s.result(rid)
     s.close()
+
+
+def test_failed_rebuild_keeps_requests_and_is_retried(ctx):
+    # The first two batch rebuilds fail after their uploads and copies (as an
+    # allocation or copy failure would). The round reports the failure; the
+    # requests and their host copies stay, later rounds rebuild, and every
+    # result equals the undisturbed run's bit for bit.
+    reqs = requests()[:4]
+    clean = E.Server(ctx, config())
+    ids = [clean.submit(w, integ, EPS, 1e-3, steps) for w, integ, steps in reqs]
+    clean.drain()
+    want = [clean.result(i) for i in ids]
+    clean.close()
+
+    s = E.Server(ctx, config(fail_rebuilds=2))
+    ids = [s.submit(w, integ, EPS, 1e-3, steps) for w, integ, steps in reqs[:2]]
+    failures = 0
+    for _ in range(2):
+        with pytest.raises(E.solvers.C.EngineError) as e:
+            s.round()
+        assert "injected" in str(e.value)
+        failures += 1
+    # Arrivals after the failures join the retried batches.
+    ids += [s.submit(w, integ, EPS, 1e-3, steps) for w, integ, steps in reqs[2:]]
+    s.drain()
+    assert failures == 2
+    for rid, (w, integ, steps), ref in zip(ids, reqs, want):
+        st = s.status(rid)
+        assert st["state"] == "done" and st["steps_done"] == steps
+        got = s.result(rid)
+        for key in ("x", "y", "z", "vx", "vy", "vz"):
+            assert np.array_equal(got[key], ref[key]), (w.n, integ, key)
+    assert s.stats()["resident_bytes"] == 0
+    s.close()

@@ -1,8 +1,11 @@
-"""The synthetic workload generators against the models they state.
+"""The synthetic workload generators.
 
-The generators draw from numpy's PCG64, so they are checked by what fixes
-the models -- closed forms, invariants and distributions -- and, for the two-disc
-model, against the oracle's C version of the same model by distribution.
+The Plummer sampler is the reference's own seeded sample, so it is pinned bit
+for bit: against the oracle's restatement of plummer.cpp / random.cpp and,
+where it is built, against the reference library compiled from its sources.
+The two-disc model is this project's own and draws from numpy's PCG64, so it is
+checked by what fixes the model -- closed forms, invariants and distributions --
+and against the oracle's C version of the same model by distribution.
 """
 import numpy as np
 import pytest
@@ -19,21 +22,42 @@ def ks_two_sample(a, b):
     return np.max(np.abs(fa - fb))
 
 
-def test_plummer_invariants_and_distribution():
+def same_bits(p, q):
+    return all(np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+               for a, b in zip(p.arrays(), q.arrays()))
+
+
+@pytest.mark.parametrize("n,seed", [(1, 1), (7, 2), (4096, 1), (65536, 1), (65536, 2), (65536, 3),
+                                    (262144, 3)])
+def test_plummer_is_the_reference_sample(n, seed):
+    # mt19937_64(seed), top-53-bit uniforms, position direction before velocity
+    # direction, compensated centre-of-mass shift: every bit as the reference.
+    p = W.plummer(n, seed)
+    assert same_bits(p, oracle.plummer(n, seed))
+    ref = oracle.load_reference()
+    if ref is not None:
+        assert same_bits(p, oracle.plummer(n, seed, lib=ref))
+
+
+def test_plummer_other_parameters_and_errors():
+    p = W.plummer(1000, 9, total_mass=2.5, scale_radius=3 * np.pi / 16, cutoff=0.999)
+    assert same_bits(p, oracle.plummer(1000, 9, total_mass=2.5, scale_radius=3 * np.pi / 16,
+                                       cutoff=0.999))
+    assert W.plummer(0, 1).n == 0
+    from orrery_amd import _capi as C
+    for kwargs in (dict(total_mass=0.0), dict(scale_radius=-1.0), dict(cutoff=1.0),
+                   dict(cutoff=float("nan"))):
+        with pytest.raises(C.EngineError):
+            W.plummer(10, 1, **kwargs)
+
+
+def test_plummer_invariants():
     p = W.plummer(20000, 3)
     assert p.n == 20000 and np.allclose(p.m, 1 / 20000)
     for pos, vel in ((p.x, p.vx), (p.y, p.vy), (p.z, p.vz)):
         assert abs(np.sum(pos * p.m)) < 1e-14 and abs(np.sum(vel * p.m)) < 1e-14
     r = np.sqrt(p.x ** 2 + p.y ** 2 + p.z ** 2)
     assert r.max() < 10.2  # truncated at r < 10 a before the centre-of-mass shift
-    # Same model as plummer.cpp (the oracle's restatement of it): radii and
-    # speeds distributed alike.
-    q = oracle.plummer(20000, 3)
-    rq = np.sqrt(q.x ** 2 + q.y ** 2 + q.z ** 2)
-    assert ks_two_sample(r, rq) < 0.025
-    v = np.sqrt(p.vx ** 2 + p.vy ** 2 + p.vz ** 2)
-    vq = np.sqrt(q.vx ** 2 + q.vy ** 2 + q.vz ** 2)
-    assert ks_two_sample(v, vq) < 0.025
     # Virial equilibrium: 2K/|U| = 1 within sampling noise.
     d = oracle.diagnostics(oracle.Particles(*p.arrays()), 1e-6)
     assert abs(2 * d["kinetic"] / abs(d["potential"]) - 1) < 0.05
