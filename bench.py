@@ -5,7 +5,8 @@
 
 Headline (``value``), direct O(N^2) on one GPU: a Plummer sphere of
 N = 262,144 equal-mass particles (total mass 1, G = 1, scale radius a = 1, radii
-by inverse CDF truncated at r < 10 a, rejection-sampled speeds, seed 1), Plummer
+by inverse CDF truncated at r < 10 a, rejection-sampled speeds, seed 1, drawn
+as the reference's sampler draws it), Plummer
 softening eps = 0.005, Yoshida-4 with dt = 5e-4. A step is one Yoshida-4 step
 on device-resident FP64 state: three force evaluations (FP32 pairwise terms,
 compensated accumulation) and the FP64 kick/drift updates around them.
@@ -16,7 +17,10 @@ direct_complement, the pairs inside each block of 8 tiles) from HIP events
 recorded on the engine's own stream.
 
 Further legs in the same JSON line:
-* ``direct_n65536`` -- the same direct stepping at config 2's other size;
+* ``direct_n65536`` -- the same direct stepping at N = 65,536;
+* ``plummer4096_vv`` -- the plain CPU path's run (Plummer N = 4096, eps 0.01,
+  velocity Verlet dt = 1e-3, 100 steps): time per step and energy drift on the
+  host's reference kernel, beside the device-resident single-system path;
 * ``barnes_hut`` -- two-disc collision at N = 2,097,152, theta 0.5, leaf 16;
 * ``batched``    -- 512 Plummer spheres, N log-uniform in [64, 16384], eps 0.02,
   direct below 4096 particles and Barnes-Hut (theta 0.6) above, velocity Verlet
@@ -28,11 +32,16 @@ Further legs in the same JSON line:
   throughput, per-request energy drift against the CPU reference, and beside
   them the same trace served one request at a time on the single-system path;
 * ``cpu_baseline`` and the ``force_error`` figures -- the CPU leg: the
-  reference's threaded AVX2 FP64 kernel (or the oracle's restatement) timed on
-  this host's cores over a bounded sample of the same input, and the FP64
-  Neumaier direct sums the GPU accelerations are checked against.
+  reference's threaded AVX2 FP64 kernels (or the oracle's restatement) timed on
+  this host's cores under the reference's protocol (2 warm-ups, settling within
+  5 %, 11 trials, median and quartiles; CPU model, AVX2/FMA and thread count
+  stated) over bounded samples of the same inputs, and the FP64 Neumaier direct
+  sums the GPU accelerations are checked against.
 
-Inputs come from orrery_amd.workloads (seeded numpy generators of the same models).
+Inputs come from orrery_amd.workloads: Plummer spheres from the engine's
+sampler, which draws the reference's mt19937_64 sequence in the reference's
+order (src/initial_conditions/plummer.cpp), so a seed gives the reference's own
+sample bit for bit; the two-disc model from a seeded numpy generator.
 """
 from __future__ import annotations
 
@@ -58,25 +67,120 @@ def log(msg: str) -> None:
 
 
 def cpu_threads() -> int:
+    """Every logical CPU in this process's affinity mask, unless OMP_NUM_THREADS
+    names fewer: on a GPU host whose CPUs are shared between GPU jobs that
+    variable states this job's share (it does on the MI355X hosts), and more
+    threads than the share only contend with the neighbours' work."""
     n = os.cpu_count() or 1
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    return max(1, min(16, n))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def cpu_description() -> dict:
+    """CPU model, logical processors and the vector features the reference's
+    AVX2 kernel needs (benchmarks/harness/machine_state.cpp:94-118 prints the same)."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key = key.strip()
+                if key == "model name" and model == "unknown":
+                    model = val.strip()
+                elif key == "flags" and not flags:
+                    flags = set(val.split())
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"model": model, "logical_processors": os.cpu_count() or 1, "affinity": affinity,
+            "avx2": "avx2" in flags, "fma": "fma" in flags,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def run_trials(body, warmup: int = 2, settling_limit: int = 24, settled_within: float = 0.05,
+               trials: int = 11, minimum_trial: float = 0.05) -> dict:
+    """The reference's measurement protocol (benchmarks/harness/protocol.hpp:67-132,
+    protocol.cpp, statistics.cpp): `warmup` discarded calls; one call sizes the
+    trial (calls folded into a trial until it lasts `minimum_trial` seconds, at
+    most 1000); trials are discarded until two consecutive ones agree within
+    `settled_within` twice in a row (at most `settling_limit`); then `trials`
+    timed trials, each recorded per call. Median and quartiles by nearest rank
+    on the sorted trials; drift = the second half's median over the first's, - 1."""
+    for _ in range(warmup):
+        body()
+
+    def timed(repeats):
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            body()
+        return time.perf_counter() - t0
+
+    single = timed(1)
+    repeats = 1 if minimum_trial <= 0 or single <= 0 or single >= minimum_trial else \
+        int(min(minimum_trial / single, 1000))
+    previous, agreements, settle = None, 0, 0
+    for _ in range(settling_limit):
+        current = timed(repeats)
+        settle += 1
+        if previous is not None:
+            change = abs(current - previous) / previous if previous > 0 else 1.0
+            agreements = agreements + 1 if change < settled_within else 0
+            if agreements >= 2:
+                break
+        previous = current
+    in_order = [timed(repeats) / repeats for _ in range(trials)]
+    ordered = sorted(in_order)
+
+    def quantile(f):
+        return ordered[min(int(f * len(ordered)), len(ordered) - 1)]
+
+    drift = 0.0
+    if len(in_order) >= 4:
+        half = len(in_order) // 2
+        early, late = sorted(in_order[:half]), sorted(in_order[-half:])
+        drift = late[len(late) // 2] / early[len(early) // 2] - 1.0
+    med = quantile(0.5)
+    return {"median_s": med, "q1_s": quantile(0.25), "q3_s": quantile(0.75),
+            "relative_spread": (quantile(0.75) - quantile(0.25)) / med if med > 0 else 0.0,
+            "fastest_s": ordered[0], "slowest_s": ordered[-1], "drift": drift,
+            "trials": trials, "warmup": warmup, "settling_trials": settle,
+            "calls_per_trial": repeats}
+
+
+COOLDOWN_S = 0.75  # between configurations (protocol.hpp: cooldown)
 
 
 class CpuLeg:
     """Everything this benchmark runs on the host CPU: the reference's own
-    kernels (oracle/_ref, else the oracle's restatement) timed as the baseline,
-    and the FP64 reference forces the GPU results are checked against. None of
-    it is part of the measured GPU path."""
+    kernels (oracle/_ref, else the oracle's restatement) timed as the baseline
+    under the reference's trial protocol, and the FP64 reference forces the GPU
+    results are checked against. None of it is part of the measured GPU path."""
 
     def __init__(self):
         import oracle
         self.oracle = oracle
         self.ref = oracle.load_reference()
         self.threads = cpu_threads()
+        self.machine = cpu_description()
+        self.kind = "reference" if self.ref is not None else "port"
+        log(f"[bench] CPU: {self.machine['model']}, {self.threads} threads of "
+            f"{self.machine['affinity']} in the affinity mask, avx2 {self.machine['avx2']}, "
+            f"fma {self.machine['fma']}")
+
+    def stamp(self, stats: dict, value: float, unit: str, sample: str, **extra) -> dict:
+        out = {"value": value, "unit": unit, "cores": self.threads, "kind": self.kind,
+               "sample": sample, "cpu": self.machine, "protocol": stats}
+        out.update(extra)
+        return out
 
     def fp64_errors(self, state, eps: float, acc, targets) -> dict:
         o = self.oracle
@@ -87,66 +191,110 @@ class CpuLeg:
         return {"rms_relative": rms, "max_relative": worst, "targets": int(len(targets)),
                 "reference": "FP64 Neumaier direct sum"}
 
-    def direct_baseline(self, state, eps: float, seconds: float = 10.0) -> dict:
+    def direct_baseline(self, state, eps: float, targets: int) -> dict:
+        """Interactions/s of the AVX2 FP64 direct kernel on the work-stealing
+        executor over `targets` targets (all N when targets >= N) x all sources."""
         import ctypes
         from oracle import _ptr, _u64
         n = state.n
+        targets = min(n, targets)
         ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
         if self.ref is not None:
-            kind = "reference"
-
-            def run(tb, te):
+            def run():
                 self.ref.ref_direct_targets(_u64(n), _ptr(state.x), _ptr(state.y), _ptr(state.z),
                                             _ptr(state.m), ctypes.c_double(eps), ctypes.c_int(1),
-                                            ctypes.c_uint(self.threads), _u64(tb), _u64(te),
+                                            ctypes.c_uint(self.threads), _u64(0), _u64(targets),
                                             _ptr(ax), _ptr(ay), _ptr(az))
         else:
-            kind = "port"
             lib = self.oracle.load()
 
-            def run(tb, te):
+            def run():
                 lib.orc_direct(_u64(n), _ptr(state.x), _ptr(state.y), _ptr(state.z), _ptr(state.m),
                                ctypes.c_double(eps * eps), ctypes.c_double(1.0), ctypes.c_int(1),
-                               ctypes.c_uint(self.threads), _u64(tb), _u64(te), _ptr(ax), _ptr(ay),
-                               _ptr(az))
+                               ctypes.c_uint(self.threads), _u64(0), _u64(targets), _ptr(ax),
+                               _ptr(ay), _ptr(az))
 
-        probe = min(n, 256)
-        run(0, probe)  # warm-up
-        t0 = time.perf_counter()
-        run(0, probe)
-        dt = time.perf_counter() - t0
-        rate = probe * (n - 1) / max(dt, 1e-9)
-        targets = int(min(n, max(probe, seconds * rate / max(1, n - 1))))
-        t0 = time.perf_counter()
-        run(0, targets)
-        dt = time.perf_counter() - t0
+        time.sleep(COOLDOWN_S)
+        st = run_trials(run)
         inter = targets * (n - 1)
-        return {"value": inter / dt, "unit": "interactions/s", "cores": self.threads, "kind": kind,
-                "sample": f"{targets} of {n} targets x all {n} sources, FP64 AVX2 kernel with the "
-                          f"work-stealing executor, {dt:.1f} s"}
+        what = "every target" if targets == n else f"{targets} of {n} targets"
+        return self.stamp(st, inter / st["median_s"], "interactions/s",
+                          f"{what} x all {n} sources per call, FP64 AVX2 kernel on the "
+                          f"work-stealing executor; value = interactions per call / median call time",
+                          interactions_per_call=inter,
+                          iqr_interactions_per_s=[inter / st["q3_s"], inter / st["q1_s"]])
 
-    def barnes_hut_baseline(self, state, eps: float, theta: float, cap: int) -> dict:
+    def barnes_hut_baseline(self, state, eps: float, theta: float, cap: int, quadrupole: bool,
+                            trials: int = 11, warmup: int = 2, settling_limit: int = 24) -> dict:
         import ctypes
         from oracle import _ptr, _u64, _u64p
         n = state.n
         ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
         counts = np.zeros(2, dtype=np.uint64)
-        t0 = time.perf_counter()
         if self.ref is not None:
-            kind = "reference"
-            self.ref.ref_barnes_hut(_u64(n), _ptr(state.x), _ptr(state.y), _ptr(state.z),
-                                    _ptr(state.m), ctypes.c_double(eps), ctypes.c_double(theta),
-                                    _u64(cap), ctypes.c_int(0), ctypes.c_int(1),
-                                    ctypes.c_uint(self.threads), _ptr(ax), _ptr(ay), _ptr(az),
-                                    _ptr(counts, _u64p))
+            def run():
+                self.ref.ref_barnes_hut(_u64(n), _ptr(state.x), _ptr(state.y), _ptr(state.z),
+                                        _ptr(state.m), ctypes.c_double(eps), ctypes.c_double(theta),
+                                        _u64(cap), ctypes.c_int(1 if quadrupole else 0), ctypes.c_int(1),
+                                        ctypes.c_uint(self.threads), _ptr(ax), _ptr(ay), _ptr(az),
+                                        _ptr(counts, _u64p))
         else:
-            kind = "port"
-            self.oracle.barnes_hut(state.x, state.y, state.z, state.m, eps * eps, theta, cap,
-                                   threads=self.threads)
-        dt = time.perf_counter() - t0
-        return {"value": dt * 1e3, "unit": "ms/evaluation", "cores": self.threads, "kind": kind,
-                "sample": f"N={n} two-disc collision (same generator), theta {theta}, leaf {cap}, "
-                          f"the reference's threaded BarnesHutSolver"}
+            def run():
+                self.oracle.barnes_hut(state.x, state.y, state.z, state.m, eps * eps, theta, cap,
+                                       quadrupole=quadrupole, threads=self.threads)
+
+        time.sleep(COOLDOWN_S)
+        st = run_trials(run, warmup=warmup, settling_limit=settling_limit, trials=trials)
+        return self.stamp(st, st["median_s"] * 1e3, "ms/evaluation",
+                          f"N={n} two-disc collision (the GPU leg's arrays), theta {theta}, leaf "
+                          f"{cap}, {'quadrupole' if quadrupole else 'monopole'}, the reference's "
+                          f"BarnesHutSolver on the work-stealing executor, AVX2 leaf kernel",
+                          iqr_ms=[st["q1_s"] * 1e3, st["q3_s"] * 1e3],
+                          interactions={"particle_particle": int(counts[0]),
+                                        "particle_cell": int(counts[1])} if self.ref is not None else None)
+
+    def plummer_vv(self, state, eps: float, dt: float, steps: int) -> dict:
+        """The plain CPU run: `steps` velocity Verlet steps of the direct solver
+        (AVX2 kernel, work-stealing executor) from `state`, timed per run under
+        the protocol; FP64 energy drift of one run."""
+        import ctypes
+        from oracle import _ptr, _u64
+        o = self.oracle
+        n = state.n
+        f = o.make_field("vv", "direct", eps2=eps * eps, threads=self.threads)
+        start = o.Particles(*(a.copy() for a in state.arrays()))
+        acc0 = o.refresh(start, f)
+        work = o.Particles(*(a.copy() for a in state.arrays()))
+        acc = [a.copy() for a in acc0]
+
+        def reset():
+            for a, b in zip(work.arrays(), start.arrays()):
+                a[:] = b
+            for a, b in zip(acc, acc0):
+                a[:] = b
+
+        if self.ref is not None:
+            def run():
+                reset()
+                self.ref.ref_steps_threaded(_u64(n), *(_ptr(a) for a in work.arrays()[:6]),
+                                            *(_ptr(a) for a in acc), _ptr(work.m), ctypes.c_int(0),
+                                            ctypes.c_int(2), ctypes.c_double(eps), ctypes.c_double(0.5),
+                                            _u64(16), ctypes.c_double(dt), _u64(steps),
+                                            ctypes.c_uint(self.threads))
+        else:
+            def run():
+                reset()
+                o.steps(work, acc, f, dt, steps)
+
+        time.sleep(COOLDOWN_S)
+        st = run_trials(run)
+        e0 = o.diagnostics(start, eps * eps)["total"]
+        e1 = o.diagnostics(work, eps * eps)["total"]
+        return self.stamp(st, st["median_s"] * 1e3 / steps, "ms/step",
+                          f"{steps} velocity Verlet steps per call from the same initial state, "
+                          f"direct FP64 AVX2 kernel on the work-stealing executor",
+                          iqr_ms_per_step=[st["q1_s"] * 1e3 / steps, st["q3_s"] * 1e3 / steps],
+                          energy_drift=(e1 - e0) / abs(e0))
 
 
 def measured_traffic(n: int):
@@ -224,6 +372,45 @@ def direct_rate(E, W, ctx, n: int, steps: int, seed: int = 1) -> dict:
             / FP32_PEAK_TFLOPS}
 
 
+def bench_plummer_vv(E, W, ctx, cpu) -> dict:
+    """The plain CPU path's configuration: Plummer N = 4096 (seed 1), eps 0.01,
+    velocity Verlet dt = 1e-3 for 100 steps. CPU: time per step under the
+    reference's protocol and FP64 energy drift; beside it the device-resident
+    single-system path on the same state (time per step over five runs of the
+    100 steps, drift, and the first force evaluation against FP64)."""
+    n, eps, dt, steps, runs = 4096, 0.01, 1e-3, 100, 5
+    state = W.plummer(n, 1)
+    cfg = E.StepConfig(integrator="velocity-verlet", solver="direct", softening=eps)
+    s = E.DeviceSystem(ctx, n)
+    times, drift, force = [], None, None
+    for r in range(runs + 1):
+        s.upload(*state.arrays())
+        e0 = s.diagnostics(eps)["total"]
+        s.refresh(cfg)
+        if r == 1 and cpu is not None:
+            a = s.download()
+            force = cpu.fp64_errors(state, eps, (a["ax"], a["ay"], a["az"]), np.arange(n))
+        t0 = time.perf_counter()
+        s.step(cfg, dt, steps)
+        elapsed = time.perf_counter() - t0
+        e1 = s.diagnostics(eps)["total"]
+        if r > 0:  # the first run warms up
+            times.append(elapsed * 1e3 / steps)
+            drift = (e1 - e0) / abs(e0)
+    s.close()
+    times.sort()
+    out = {"workload": f"Plummer N={n} seed 1, eps {eps}, velocity Verlet dt {dt}, {steps} steps, "
+                       f"direct", "n": n, "steps": steps,
+           "gpu": {"ms_per_step": times[len(times) // 2], "runs": runs,
+                   "range_ms_per_step": [times[0], times[-1]], "energy_drift": drift,
+                   "force_error": force,
+                   "path": "device-resident single-system path (orr_system), eager launches"}}
+    if cpu is not None:
+        log("[bench] CPU: Plummer N=4096 velocity Verlet, 100 steps per trial")
+        out["cpu_baseline"] = cpu.plummer_vv(state, eps, dt, steps)
+    return out
+
+
 def bench_barnes_hut(E, W, ctx, n: int, evals: int, error_targets: int, cpu) -> dict:
     """Barnes-Hut on the two-disc collision: N particles, theta 0.5, leaf 16."""
     eps, theta, cap = 0.01, 0.5, 16
@@ -295,7 +482,16 @@ def bench_barnes_hut(E, W, ctx, n: int, evals: int, error_targets: int, cpu) -> 
     for a in d + out:
         a.release()
     if cpu is not None:
-        out_d["cpu_baseline"] = cpu.barnes_hut_baseline(W.disc_collision(262144, 1), eps, theta, cap)
+        # The GPU headline's settings (quadrupole, theta 0.5, leaf 16) on the
+        # same arrays; at 2M one evaluation takes seconds on the host, so that
+        # size runs one warm-up and three timed trials (stated in "protocol").
+        log("[bench] CPU: Barnes-Hut N=262,144 and the GPU leg's arrays")
+        small = W.disc_collision(262144, 1) if n != 262144 else p
+        out_d["cpu_baseline"] = {
+            "n262144": cpu.barnes_hut_baseline(small, eps, theta, cap, True),
+            f"n{n}": cpu.barnes_hut_baseline(p, eps, theta, cap, True, trials=3, warmup=1,
+                                              settling_limit=0),
+        }
     return out_d
 
 
@@ -556,6 +752,8 @@ def main() -> int:
     ap.add_argument("--bh-n", type=int, default=2097152)
     ap.add_argument("--bh-evals", type=int, default=5)
     ap.add_argument("--no-barnes-hut", action="store_true")
+    ap.add_argument("--no-plummer-vv", action="store_true",
+                    help="skip the N = 4096 velocity Verlet line (CPU path and device)")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--alone-steps", type=int, default=3)
@@ -683,12 +881,19 @@ def main() -> int:
             rng = np.random.default_rng(12345)
             tgt = np.sort(rng.choice(n, size=min(n, args.error_targets), replace=False))
             result["force_error"] = cpu.fp64_errors(state, eps, (acc["ax"], acc["ay"], acc["az"]), tgt)
-            result["cpu_baseline"] = cpu.direct_baseline(state, eps)
+            log("[bench] CPU: direct AVX2 baseline at N = 262,144 (sampled targets) and 65,536")
+            result["cpu_baseline"] = cpu.direct_baseline(state, eps, targets=8192)
+            if "direct_n65536" in result:
+                result["direct_n65536"]["cpu_baseline"] = cpu.direct_baseline(W.plummer(65536, 1), eps,
+                                                                              targets=65536)
             log("[bench] direct accuracy at N = 65,536 and 262,144, seeds 1-3")
             result["force_error_by_case"] = direct_accuracy(E, W, ctx, cpu)
         else:
             result["force_error"] = None
             result["cpu_baseline"] = None
+    if not args.no_plummer_vv:
+        log("[bench] Plummer N=4096, velocity Verlet, 100 steps")
+        result["plummer4096_vv"] = bench_plummer_vv(E, W, ctx, cpu)
     if not args.no_barnes_hut:
         result["barnes_hut"] = bench_barnes_hut(E, W, ctx, args.bh_n, args.bh_evals,
                                                 args.error_targets, cpu)

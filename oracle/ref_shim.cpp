@@ -259,23 +259,31 @@ int ref_barnes_hut(std::uint64_t n, const double* x, const double* y, const doub
     return 0;
 }
 
-// Runs `steps` steps of VV (integrator 0) or Yoshida-4 (1) with the direct
-// solver (solver 0 scalar, 2 AVX2) or Barnes-Hut (1), single-threaded.
-int ref_steps(std::uint64_t n, double* x, double* y, double* z, double* vx, double* vy,
-              double* vz, double* ax, double* ay, double* az, const double* m, int integrator,
-              int solver_kind, double eps, double theta, std::uint64_t cap, double dt,
-              std::uint64_t steps) {
+// Runs `steps` steps of VV (integrator 0), Yoshida-4 (1) or RK4 (2) with the
+// direct solver (solver 0 scalar, 2 AVX2) or Barnes-Hut (1). `threads` > 1
+// puts the solver on a work-stealing executor of that many workers, the way
+// sim/assembly.cpp:94-107 and :233-243 assemble a threaded CPU run; 1 runs it
+// on the calling thread.
+int ref_steps_threaded(std::uint64_t n, double* x, double* y, double* z, double* vx,
+                       double* vy, double* vz, double* ax, double* ay, double* az,
+                       const double* m, int integrator, int solver_kind, double eps,
+                       double theta, std::uint64_t cap, double dt, std::uint64_t steps,
+                       unsigned threads) {
     core::ParticleData d = load(n, x, y, z, vx, vy, vz, ax, ay, az, m);
+    std::unique_ptr<backend::WorkStealingExecutor> ex;
+    if (threads > 1) ex = std::make_unique<backend::WorkStealingExecutor>(threads);
     std::unique_ptr<solvers::ForceSolver> solver;
     if (solver_kind == 1) {
         solvers::TreeParameters p;
         p.opening_angle = theta;
         p.leaf_capacity = cap;
-        auto bh = std::make_unique<solvers::BarnesHutSolver>(p, core::Softening{eps});
+        auto bh = ex ? std::make_unique<solvers::BarnesHutSolver>(p, core::Softening{eps}, *ex)
+                     : std::make_unique<solvers::BarnesHutSolver>(p, core::Softening{eps});
         bh->select_kernel(solvers::KernelKind::kAvx2);
         solver = std::move(bh);
     } else {
-        auto ds = std::make_unique<solvers::DirectSolver>(core::Softening{eps});
+        auto ds = ex ? std::make_unique<solvers::DirectSolver>(core::Softening{eps}, *ex)
+                     : std::make_unique<solvers::DirectSolver>(core::Softening{eps});
         ds->select_kernel(solver_kind == 2 ? solvers::KernelKind::kAvx2
                                            : solvers::KernelKind::kScalar);
         solver = std::move(ds);
@@ -298,6 +306,15 @@ int ref_steps(std::uint64_t n, double* x, double* y, double* z, double* vx, doub
         ax[i] = a.x[i]; ay[i] = a.y[i]; az[i] = a.z[i];
     }
     return 0;
+}
+
+// Single-threaded form of ref_steps_threaded.
+int ref_steps(std::uint64_t n, double* x, double* y, double* z, double* vx, double* vy,
+              double* vz, double* ax, double* ay, double* az, const double* m, int integrator,
+              int solver_kind, double eps, double theta, std::uint64_t cap, double dt,
+              std::uint64_t steps) {
+    return ref_steps_threaded(n, x, y, z, vx, vy, vz, ax, ay, az, m, integrator, solver_kind,
+                              eps, theta, cap, dt, steps, 1u);
 }
 
 int ref_diagnostics(std::uint64_t n, const double* x, const double* y, const double* z,

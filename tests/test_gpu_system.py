@@ -185,6 +185,50 @@ def test_steps_match_cpu_reference_path(ctx, integrator):
     s.close()
 
 
+@pytest.mark.parametrize("integrator,coarse,order", [("velocity-verlet", 256, 2),
+                                                     ("yoshida4", 32, 4)])
+def test_symplectic_measured_order(ctx, integrator, coarse, order):
+    # tests/integrators/convergence_test.cpp:83-120 on the device: a circular
+    # equal-mass orbit (a = 1) closed after one period in `coarse` and
+    # 2 coarse steps; log2 of the closure errors' ratio is the order, within
+    # 0.2 of the method's (the reference's FP64 tolerance) and of the oracle's
+    # FP64 measurement. Velocity Verlet at the reference's 256 / 512 steps;
+    # Yoshida-4 at 32 / 64, where its closure errors (4.4e-3, 2.7e-4) stay
+    # three orders above what FP32 forces add over a period (a few 1e-7; at
+    # 128 steps the closure is 1.7e-5 and that floor moves the order by 0.08).
+    x0 = np.array([-0.5, 0.5])
+    v0 = np.array([-np.sqrt(0.5), np.sqrt(0.5)])
+    period = 2 * np.pi * np.sqrt(0.5)
+    z = np.zeros(2)
+
+    def closure(steps):
+        s = E.DeviceSystem(ctx, 2)
+        s.upload(x0, z, z, z, v0, z, np.array([1.0, 1.0]))
+        cfg = E.StepConfig(integrator=integrator, solver="direct", softening=0.0)
+        s.refresh(cfg)
+        s.step(cfg, period / steps, steps)
+        st = s.download()
+        s.close()
+        return max(np.hypot(st["x"][k] - x0[k], st["y"][k]) for k in range(2))
+
+    def closure_oracle(steps):
+        st = oracle.Particles(x0.copy(), z.copy(), z.copy(), z.copy(), v0.copy(), z.copy(),
+                              np.array([1.0, 1.0]))
+        f = oracle.make_field({"velocity-verlet": "vv", "yoshida4": "yoshida4"}[integrator],
+                              "direct", eps2=0.0, kernel=0)
+        acc = oracle.refresh(st, f)
+        oracle.steps(st, acc, f, period / steps, steps)
+        return max(np.hypot(st.x[k] - x0[k], st.y[k]) for k in range(2))
+
+    dev = [closure(coarse), closure(2 * coarse)]
+    ref = [closure_oracle(coarse), closure_oracle(2 * coarse)]
+    for d, r in zip(dev, ref):
+        assert abs(d - r) <= 2e-3 * r + 1e-6, (d, r)
+    measured, measured_ref = np.log2(dev[0] / dev[1]), np.log2(ref[0] / ref[1])
+    print(f"{integrator}: closure {dev} (oracle {ref}); order {measured:.4f} (oracle {measured_ref:.4f})")
+    assert abs(measured - order) <= 0.2 and abs(measured - measured_ref) <= 0.05, (measured, measured_ref)
+
+
 def test_rk4_measured_order_and_tree_field(ctx):
     # Circular two-body orbit over one period (tests/integrators/convergence_test.cpp):
     # RK4's closure error falls by ~2^4 when the step halves. At 24 and 48 steps

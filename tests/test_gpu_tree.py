@@ -215,6 +215,8 @@ def test_momentum_residual(ctx):
 
 
 def test_disc_collision_262144_sampled(ctx):
+    # The reference's own bounds (cuda_tree_solver_test.cpp:392-412) without
+    # quadrupoles, on the oracle's two-disc model.
     p = oracle.disc_collision(262144, 1)
     eps, theta, cap = 0.01, 0.5, 16
     (ax, ay, az), s = tree_eval(ctx, p, eps, theta, cap)
@@ -225,6 +227,31 @@ def test_disc_collision_262144_sampled(ctx):
     w, r = oracle.error_summary(oracle.relative_errors(ax[i], ay[i], az[i], *ref))
     print(f"disc N=262144 theta=0.5 leaf 16: rms={r:.3e} worst={w:.3e} {s.tree_summary()}")
     assert r < 1e-2 and w < 1e-1
+
+
+def disc_quadrupole_rms(ctx, n, targets, seed):
+    from orrery_amd import workloads as W
+    p = W.disc_collision(n, 1)
+    eps, theta, cap = 0.01, 0.5, 16
+    s = E.HipTreeSolver(E.TreeParameters(theta, cap, quadrupole=True), E.Softening(eps), context=ctx)
+    acc = [np.full(n, np.nan) for _ in range(3)]
+    s.evaluate((p.x, p.y, p.z), p.m, acc)
+    rng = np.random.default_rng(seed)
+    tgt = np.sort(rng.choice(n, targets, replace=False)).astype(np.uint64)
+    ref = oracle.reference_acceleration(p.x, p.y, p.z, p.m, eps * eps, targets=tgt)
+    i = tgt.astype(np.int64)
+    w, r = oracle.error_summary(oracle.relative_errors(acc[0][i], acc[1][i], acc[2][i], *ref))
+    print(f"two-disc N={n} theta=0.5 leaf 16 quadrupole, {targets} targets: rms={r:.3e} worst={w:.3e}")
+    return r, w
+
+
+@pytest.mark.parametrize("n,targets", [(262144, 4096), (2097152, 1024)])
+def test_config3_rms_within_1e3(ctx, n, targets):
+    # The Barnes-Hut configuration (two-disc collision, theta = 0.5, leaf 16,
+    # quadrupoles, the benchmark's generator) against the FP64 direct sum on
+    # sampled targets: RMS relative error <= 1e-3.
+    r, w = disc_quadrupole_rms(ctx, n, targets, 5)
+    assert r <= 1e-3 and w < 5e-2, (r, w)
 
 
 def test_device_resident_tree_stepping(ctx):
