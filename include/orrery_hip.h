@@ -109,6 +109,21 @@ typedef struct {
 } orr_direct_args;
 ORR_API int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* args);
 
+/* No counterpart in the reference: orr_direct_eval for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * hosts that forbid hipMalloc). Every buffer lives in one static device array
+ * of 16 MiB per GPU, held from the first call on that GPU until the process
+ * ends; nothing is allocated, no graph is built, phases are not timed. Same
+ * arguments and result as orr_direct_eval, with these limits: count at most
+ * 32,768 (ORR_ERR_LENGTH beyond); always the one-sided kernel; where the
+ * one-sided kernel's split partials would not fit, fewer splits (the same
+ * terms in fewer Kahan-summed groups, so not the same bits as orr_direct_eval;
+ * the same bits whenever they fit and orr_direct_eval takes the one-sided
+ * kernel). Calls that overlap on one GPU share the array and race. */
+ORR_API int orr_direct_eval_static(orr_ctx* ctx, const orr_direct_args* args);
+/* The limits above: the largest count and the scratch bytes per GPU. */
+ORR_API int orr_direct_static_limits(uint64_t* max_count, uint64_t* scratch_bytes);
+
 /* Launch configuration of the direct kernel (no reference counterpart: the
  * CUDA solver fixes its block size, cuda_direct_solver.cpp:46-63).
  * targets_per_lane in {1, 2, 4}; refine = 1 applies a Newton step to the

@@ -145,6 +145,8 @@ SIGNATURES = {
     "orr_download": [_vp, _vp, _vp, ctypes.c_uint64],
     "orr_memset": [_vp, _vp, ctypes.c_int32, ctypes.c_uint64],
     "orr_direct_eval": [_vp, ctypes.POINTER(OrrDirectArgs)],
+    "orr_direct_eval_static": [_vp, ctypes.POINTER(OrrDirectArgs)],
+    "orr_direct_static_limits": [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
     "orr_ctx_set_direct_config": [_vp, ctypes.POINTER(OrrDirectConfig)],
     "orr_ctx_get_direct_config": [_vp, ctypes.POINTER(OrrDirectConfig)],
     "orr_tree_eval": [_vp, ctypes.POINTER(OrrTreeArgs), ctypes.POINTER(OrrTreeStats)],

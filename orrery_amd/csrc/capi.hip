@@ -251,6 +251,28 @@ int orr_direct_eval(orr_ctx* ctx, const orr_direct_args* a) {
     });
 }
 
+int orr_direct_eval_static(orr_ctx* ctx, const orr_direct_args* a) {
+    return guarded([&] {
+        require(ctx != nullptr && a != nullptr, "null argument");
+        require(a->softening_squared >= 0, "softening squared must be >= 0");
+        if (a->count == 0) return;
+        require(a->x && a->y && a->z && a->m && a->ax && a->ay && a->az, "null particle array");
+        use_device(ctx);
+        orr::direct_evaluate_static(ctx, *a);
+        // A plain stream synchronise: ctx->sync() would also fold pending
+        // timing events into the context's host tables.
+        ORR_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int orr_direct_static_limits(uint64_t* max_count, uint64_t* scratch_bytes) {
+    return guarded([&] {
+        require(max_count != nullptr && scratch_bytes != nullptr, "null argument");
+        *max_count = orr::direct_static_max_count();
+        *scratch_bytes = orr::direct_static_scratch_bytes();
+    });
+}
+
 int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* c) {
     return guarded([&] {
         require(ctx != nullptr && c != nullptr, "null argument");

@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "close_pair.hpp"
 #include "engine.hpp"
@@ -1168,45 +1169,75 @@ void launch_pairs(orr_ctx* ctx, const PairPlan& plan, const float4* packed, cons
 
 }  // namespace
 
-void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order) {
-    const std::uint64_t n = a.count;
-    if (n == 0) return;
+namespace {
+
+// The shape of one single-system evaluation: tiles, one-sided launch grid,
+// near mask and pairs plan, as direct_enqueue chooses them for the context's
+// configuration.
+struct DirectShape {
+    std::uint32_t tiles = 0, npad = 0, blocks_i = 0, splits = 0, per_split = 0, words = 0;
+    int R = 1;
+    bool masked = false;
+    PairPlan plan;
+};
+
+DirectShape direct_shape(const orr_ctx* ctx, std::uint64_t n) {
     if (n > (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "direct: more than 2^31 particles");
-    const std::uint32_t tiles = static_cast<std::uint32_t>((n + kTile - 1) / kTile);
-    const std::uint32_t npad = tiles * kTile;
-    const int R = ctx->direct_targets_per_lane;
-    const std::uint32_t blocks_i = (npad + kBlock * R - 1) / (kBlock * R);
+    DirectShape d;
+    d.tiles = static_cast<std::uint32_t>((n + kTile - 1) / kTile);
+    d.npad = d.tiles * kTile;
+    d.R = ctx->direct_targets_per_lane;
+    d.blocks_i = (d.npad + kBlock * d.R - 1) / (kBlock * d.R);
     // Enough blocks to fill 256 CUs several times over; each split keeps >= 1 tile.
     const std::uint32_t want = ctx->direct_block_target;
-    std::uint32_t splits = std::max<std::uint32_t>(1, (want + blocks_i - 1) / blocks_i);
-    splits = std::min(splits, tiles);
-    const std::uint32_t per_split = (tiles + splits - 1) / splits;
-    splits = (tiles + per_split - 1) / per_split;
+    std::uint32_t splits = std::max<std::uint32_t>(1, (want + d.blocks_i - 1) / d.blocks_i);
+    splits = std::min(splits, d.tiles);
+    d.per_split = (d.tiles + splits - 1) / splits;
+    d.splits = (d.tiles + d.per_split - 1) / d.per_split;
+    d.masked = d.tiles <= kMaskTiles;
+    d.words = (d.tiles + 31) / 32;
+    d.plan = pair_plan(d.tiles, d.masked, ctx->direct_pair_mode);
+    return d;
+}
 
-    const bool masked = tiles <= kMaskTiles;
-    const PairPlan plan = pair_plan(tiles, masked, ctx->direct_pair_mode);
-    const std::size_t cpart_floats = 6 * static_cast<std::size_t>(npad);
-    ctx->direct_packed.reserve(2 * sizeof(float4) * npad);
-    ctx->direct_centres.reserve((sizeof(double4) + sizeof(float4) + sizeof(float2)) * tiles + 16);
-    ctx->direct_partials.reserve(
-        plan.use ? sizeof(double) * plan.ipart_doubles + sizeof(float) * (plan.jpart_floats + cpart_floats)
-                 : sizeof(float) * 6 * static_cast<std::size_t>(npad) * splits);
-    float4* packed = ctx->direct_packed.as<float4>();
+// Device buffers of one evaluation (DirectShape's sizes).
+struct DirectBuffers {
+    float4* packed = nullptr;     // [npad] {o, m}, then [npad] residuals
+    double4* centres = nullptr;   // [tiles], then extents [tiles], mass ranges [tiles], uniform
+    float* partials = nullptr;    // one-sided: [splits][6][npad]; pairs: slots
+    std::uint32_t* mask = nullptr;  // [tiles][words] when masked
+};
+
+std::size_t packed_buffer_bytes(const DirectShape& d) { return 2 * sizeof(float4) * d.npad; }
+std::size_t centre_buffer_bytes(const DirectShape& d) {
+    return (sizeof(double4) + sizeof(float4) + sizeof(float2)) * d.tiles + 16;
+}
+std::size_t partial_buffer_bytes(const DirectShape& d) {
+    const std::size_t cpart_floats = 6 * static_cast<std::size_t>(d.npad);
+    return d.plan.use ? sizeof(double) * d.plan.ipart_doubles +
+                            sizeof(float) * (d.plan.jpart_floats + cpart_floats)
+                      : sizeof(float) * 6 * static_cast<std::size_t>(d.npad) * d.splits;
+}
+std::size_t mask_buffer_bytes(const DirectShape& d) {
+    return d.masked ? sizeof(std::uint32_t) * static_cast<std::size_t>(d.words) * d.tiles : 0;
+}
+
+// Every launch of one evaluation on given buffers; phases timed when `timed`.
+void direct_launch(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order,
+                   const DirectShape& d, const DirectBuffers& buf, bool timed) {
+    const std::uint64_t n = a.count;
+    const std::uint32_t tiles = d.tiles, npad = d.npad, words = d.words;
+    const PairPlan& plan = d.plan;
+    float4* packed = buf.packed;
     float4* packed_lo = packed + npad;
-    double4* centres = ctx->direct_centres.as<double4>();
+    double4* centres = buf.centres;
     float4* extents = reinterpret_cast<float4*>(centres + tiles);
     float2* mass_range = reinterpret_cast<float2*>(extents + tiles);
     float* uniform = reinterpret_cast<float*>(mass_range + tiles);
-    float* partials = ctx->direct_partials.as<float>();
-
-    const std::uint32_t words = (tiles + 31) / 32;
-    std::uint32_t* mask = nullptr;
-    if (masked) {
-        ctx->direct_near.reserve(sizeof(std::uint32_t) * static_cast<std::size_t>(words) * tiles);
-        mask = ctx->direct_near.as<std::uint32_t>();
-    }
+    float* partials = buf.partials;
+    std::uint32_t* mask = d.masked ? buf.mask : nullptr;
     {
-        PhaseTimer t(ctx, kDirectPrepare);
+        PhaseTimer t(ctx, kDirectPrepare, timed);
         hipLaunchKernelGGL(direct_prepare, dim3(tiles), dim3(kTile), 0, ctx->stream, a.x, a.y, a.z,
                            a.m, order, n, packed, packed_lo, centres, extents, mass_range);
         ORR_LAUNCHED("direct_prepare");
@@ -1215,7 +1246,7 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
                                ctx->direct_equal_masses == 1 ? 1 : 0, uniform);
             ORR_LAUNCHED("mass_uniform");
         }
-        if (masked) {
+        if (d.masked) {
             hipLaunchKernelGGL(near_mask, dim3((tiles + kMaskRows - 1) / kMaskRows), dim3(256), 0,
                                ctx->stream, centres, extents, tiles, words, mask);
             ORR_LAUNCHED("near_mask");
@@ -1230,14 +1261,14 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
         float* jpart = reinterpret_cast<float*>(ipart + plan.ipart_doubles);
         float* cpart = jpart + plan.jpart_floats;
         {
-            PhaseTimer t(ctx, kDirectKernel);
+            PhaseTimer t(ctx, kDirectKernel, timed);
             auto launch = safe ? (refine ? launch_pairs<true, true> : launch_pairs<true, false>)
                                : (refine ? launch_pairs<false, true> : launch_pairs<false, false>);
             launch(ctx, plan, packed, packed_lo, centres, extents, mask, tiles, words, eps2, uniform,
                    jpart, ipart, cpart);
         }
         {
-            PhaseTimer t(ctx, kDirectReduce);
+            PhaseTimer t(ctx, kDirectReduce, timed);
             const std::uint32_t blocks = static_cast<std::uint32_t>((n + 255) / 256);
             hipLaunchKernelGGL(pairs_reduce, dim3(blocks), dim3(256), 0, ctx->stream, jpart, ipart,
                                cpart, uniform, tiles, plan.blocks, plan.S, n, order,
@@ -1247,21 +1278,38 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
         return;
     }
     {
-        PhaseTimer t(ctx, kDirectKernel);
-        TilesLauncher launch = R == 1 ? pick<1>(safe, refine, masked)
-                             : R == 4 ? pick<4>(safe, refine, masked)
-                                      : pick<2>(safe, refine, masked);
-        launch(ctx, blocks_i, splits, packed, packed_lo, centres, extents, mask, npad, tiles,
-               per_split, eps2, partials);
+        PhaseTimer t(ctx, kDirectKernel, timed);
+        TilesLauncher launch = d.R == 1 ? pick<1>(safe, refine, d.masked)
+                             : d.R == 4 ? pick<4>(safe, refine, d.masked)
+                                        : pick<2>(safe, refine, d.masked);
+        launch(ctx, d.blocks_i, d.splits, packed, packed_lo, centres, extents, mask, npad, tiles,
+               d.per_split, eps2, partials);
         ORR_LAUNCHED("direct_tiles");
     }
     {
-        PhaseTimer t(ctx, kDirectReduce);
+        PhaseTimer t(ctx, kDirectReduce, timed);
         const std::uint32_t blocks = static_cast<std::uint32_t>((n + 255) / 256);
-        hipLaunchKernelGGL(direct_reduce, dim3(blocks), dim3(256), 0, ctx->stream, partials, splits,
+        hipLaunchKernelGGL(direct_reduce, dim3(blocks), dim3(256), 0, ctx->stream, partials, d.splits,
                            npad, n, order, a.gravitational_constant, a.ax, a.ay, a.az);
         ORR_LAUNCHED("direct_reduce");
     }
+}
+
+}  // namespace
+
+void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order) {
+    if (a.count == 0) return;
+    const DirectShape d = direct_shape(ctx, a.count);
+    ctx->direct_packed.reserve(packed_buffer_bytes(d));
+    ctx->direct_centres.reserve(centre_buffer_bytes(d));
+    ctx->direct_partials.reserve(partial_buffer_bytes(d));
+    if (d.masked) ctx->direct_near.reserve(mask_buffer_bytes(d));
+    DirectBuffers buf;
+    buf.packed = ctx->direct_packed.as<float4>();
+    buf.centres = ctx->direct_centres.as<double4>();
+    buf.partials = ctx->direct_partials.as<float>();
+    buf.mask = d.masked ? ctx->direct_near.as<std::uint32_t>() : nullptr;
+    direct_launch(ctx, a, order, d, buf, true);
 }
 
 namespace {
@@ -1332,6 +1380,119 @@ void direct_batch_enqueue(orr_ctx* ctx, const double* x, const double* y, const 
                            b.tile, b.seg_of, b.G, b.count, order, ax, ay, az);
         ORR_LAUNCHED("batch_reduce");
     }
+}
+
+// ---- the direct sum on the library's static scratch ----------------------
+//
+// For callers that cannot allocate device memory: every buffer of the
+// evaluation -- packed tiles, centres, near mask, the Morton order's codes,
+// index and histograms, the one-sided kernel's split partials -- is carved out
+// of `scratch` below, a fixed array each device holds from the first call
+// until the process ends. Limits: at most kStaticMaxCount particles; the
+// one-sided kernel always (the pairs path's slots do not fit); the split count
+// of direct_shape, lowered where its partials would not fit (the same bits as
+// direct_evaluate with the one-sided kernel whenever it fits, else the same
+// terms grouped into fewer Kahan-summed splits). Nothing in the scratch has to
+// start at zero: every launch writes what the next reads. Calls on one device
+// share the array, so calls that overlap on one device (two contexts, two
+// host threads) race.
+
+constexpr std::size_t kStaticScratchBytes = std::size_t{16} << 20;
+constexpr std::uint64_t kStaticMaxCount = 32768;
+
+}  // namespace orr
+
+static __device__ __align__(256) unsigned char scratch[orr::kStaticScratchBytes];
+
+namespace {
+
+using orr::kStaticMaxCount;
+using orr::kStaticScratchBytes;
+constexpr int kMaxDevices = 64;
+
+// The array's device address on each device, looked up once per device with
+// that device current (host code must never use the symbol itself as an
+// address).
+std::mutex g_scratch_lock;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) orr::fail(ORR_ERR_INVALID_ARGUMENT, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    if (g_scratch_ptr[device] == nullptr) {
+        void* ptr = nullptr;
+        ORR_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (ptr == nullptr) orr::fail(ORR_ERR_RUNTIME, "static scratch: no device address");
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+// Bump allocation inside the scratch, 256-byte aligned.
+struct Carver {
+    unsigned char* base;
+    std::size_t used = 0;
+    template<typename T> T* take(std::size_t bytes) {
+        T* p = reinterpret_cast<T*>(base + used);
+        used += (bytes + 255) & ~std::size_t{255};
+        return p;
+    }
+};
+
+std::size_t aligned(std::size_t bytes) { return (bytes + 255) & ~std::size_t{255}; }
+
+}  // namespace
+
+namespace orr {
+
+std::size_t direct_static_scratch_bytes() { return kStaticScratchBytes; }
+std::uint64_t direct_static_max_count() { return kStaticMaxCount; }
+
+void direct_evaluate_static(orr_ctx* ctx, const orr_direct_args& a) {
+    if (a.count == 0) return;
+    if (a.count > kStaticMaxCount) {
+        fail(ORR_ERR_LENGTH, "direct (static scratch): more than 32768 particles; use orr_direct_eval");
+    }
+    const std::uint64_t n = a.count;
+    DirectShape d = direct_shape(ctx, n);
+    d.plan = PairPlan{};  // the one-sided kernel
+    const bool ordered = a.spatial_order != 0 && n > 64;
+    const OrderSizes oz = order_sizes(n, 1, 0, false);
+    const std::size_t fixed = aligned(packed_buffer_bytes(d)) + aligned(centre_buffer_bytes(d)) +
+                              aligned(mask_buffer_bytes(d)) +
+                              (ordered ? aligned(oz.bbox) + 2 * aligned(oz.codes) +
+                                             2 * aligned(oz.index) + aligned(oz.hist)
+                                       : 0);
+    const std::size_t split_bytes = aligned(sizeof(float) * 6 * static_cast<std::size_t>(d.npad));
+    if (fixed + split_bytes > kStaticScratchBytes) {
+        fail(ORR_ERR_LENGTH, "direct (static scratch): the evaluation does not fit the scratch");
+    }
+    const std::uint32_t fit = static_cast<std::uint32_t>((kStaticScratchBytes - fixed) / split_bytes);
+    if (d.splits > fit) {
+        d.per_split = (d.tiles + fit - 1) / fit;
+        d.splits = (d.tiles + d.per_split - 1) / d.per_split;
+    }
+    Carver c{static_cast<unsigned char*>(scratch_on(ctx->device))};
+    DirectBuffers buf;
+    buf.packed = c.take<float4>(packed_buffer_bytes(d));
+    buf.centres = c.take<double4>(centre_buffer_bytes(d));
+    buf.mask = d.masked ? c.take<std::uint32_t>(mask_buffer_bytes(d)) : nullptr;
+    const std::uint32_t* order = nullptr;
+    if (ordered) {
+        OrderScratch ob;
+        ob.bbox = c.take<double>(oz.bbox);
+        ob.codes[0] = c.take<std::uint64_t>(oz.codes);
+        ob.codes[1] = c.take<std::uint64_t>(oz.codes);
+        ob.index[0] = c.take<std::uint32_t>(oz.index);
+        ob.index[1] = c.take<std::uint32_t>(oz.index);
+        ob.hist = c.take<std::uint32_t>(oz.hist);
+        // The order direct_evaluate uses: the four upper radix passes.
+        const int cur = morton_enqueue_into(ctx, a.x, a.y, a.z, n, SegmentView{}, nullptr, 4, ob, false);
+        order = ob.index[cur];
+    }
+    buf.partials = c.take<float>(split_bytes * d.splits);
+    if (c.used > kStaticScratchBytes) fail(ORR_ERR_RUNTIME, "direct (static scratch): layout overflow");
+    direct_launch(ctx, a, order, d, buf, false);
 }
 
 }  // namespace orr

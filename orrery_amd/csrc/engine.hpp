@@ -181,8 +181,14 @@ void free_context(orr_ctx* ctx) noexcept;
 // RAII phase marker: records events around the launches it brackets.
 class PhaseTimer {
 public:
-    PhaseTimer(orr_ctx* ctx, int phase) : ctx_(ctx), phase_(phase) { ctx_->begin(phase_, start_); }
-    ~PhaseTimer() { ctx_->end(phase_, start_); }
+    // on == false records nothing whatever the context's timing setting (for
+    // paths that must not create events or grow host tables).
+    PhaseTimer(orr_ctx* ctx, int phase, bool on = true) : ctx_(on ? ctx : nullptr), phase_(phase) {
+        if (ctx_ != nullptr) ctx_->begin(phase_, start_);
+    }
+    ~PhaseTimer() {
+        if (ctx_ != nullptr) ctx_->end(phase_, start_);
+    }
     PhaseTimer(const PhaseTimer&) = delete;
     PhaseTimer& operator=(const PhaseTimer&) = delete;
 
@@ -244,6 +250,22 @@ struct OrderSizes {
 };
 OrderSizes order_sizes(std::uint64_t n, std::uint32_t segments, std::uint32_t chunks, bool aos);
 
+// The ordering stage's device buffers (sizes from order_sizes): codes and
+// index twice (the radix passes alternate), the histogram table with its scan
+// scratch, the cubes and bounding-box partials, the optional records.
+struct OrderScratch {
+    double* bbox = nullptr;
+    double4* aos = nullptr;
+    std::uint64_t* codes[2] = {nullptr, nullptr};
+    std::uint32_t* index[2] = {nullptr, nullptr};
+    std::uint32_t* hist = nullptr;
+};
+// morton_enqueue_segments on given buffers, phases timed when `timed`.
+// Returns which of the two (codes, index) pairs holds the sorted order.
+int morton_enqueue_into(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                        std::uint64_t n, const SegmentView& segments, const double* m,
+                        int first_digit, const OrderScratch& buffers, bool timed);
+
 // tree.hip: bytes tree_evaluate_segments reserves for n particles in `segments`
 // systems at this leaf capacity (its node arrays are sized to the worst-case
 // node count, n + 21 floor(n / (cap + 1)) + 22 per system).
@@ -280,6 +302,11 @@ template<typename T> void key_add(std::vector<unsigned char>& key, const T& v) {
 // direct.hip: Morton order (when asked for; replayed as a graph where
 // run_captured allows) and the direct sum.
 void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a);
+// direct.hip: the same sum with every buffer in the library's static scratch
+// (no allocation, no graph, no phase timing; limits in direct.hip).
+void direct_evaluate_static(orr_ctx* ctx, const orr_direct_args& a);
+std::size_t direct_static_scratch_bytes();
+std::uint64_t direct_static_max_count();
 
 // context.hip: device bytes held by a context's scratch buffers.
 std::size_t ctx_device_bytes(const orr_ctx* ctx);

@@ -382,10 +382,7 @@ void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, con
                              int first_digit) {
     if (n == 0) return;
     if (n > (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "morton: more than 2^31 particles");
-    const bool segmented = segs.count > 1;
     const OrderSizes sz = order_sizes(n, segs.count, segs.chunk_count, m != nullptr);
-    const std::uint32_t bbox_blocks = sz.bbox_blocks;
-    const std::uint32_t sort_blocks = sz.sort_blocks;
     ctx->order_bbox.reserve(sz.bbox);
     if (m != nullptr) ctx->order_aos.reserve(sz.aos);
     ctx->order_codes.reserve(sz.codes);
@@ -393,10 +390,34 @@ void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, con
     ctx->order_index.reserve(sz.index);
     ctx->order_index_alt.reserve(sz.index);
     ctx->order_hist.reserve(sz.hist);
-    double* cube = ctx->order_bbox.as<double>();  // cubes[4 * segment]
+    OrderScratch b;
+    b.bbox = ctx->order_bbox.as<double>();
+    b.aos = m != nullptr ? ctx->order_aos.as<double4>() : nullptr;
+    b.codes[0] = ctx->order_codes.as<std::uint64_t>();
+    b.codes[1] = ctx->order_codes_alt.as<std::uint64_t>();
+    b.index[0] = ctx->order_index.as<std::uint32_t>();
+    b.index[1] = ctx->order_index_alt.as<std::uint32_t>();
+    b.hist = ctx->order_hist.as<std::uint32_t>();
+    if (morton_enqueue_into(ctx, x, y, z, n, segs, m, first_digit, b, true) != 0) {
+        // An odd pass count ends in the alternate buffers: swap the roles.
+        std::swap(ctx->order_codes, ctx->order_codes_alt);
+        std::swap(ctx->order_index, ctx->order_index_alt);
+    }
+}
+
+int morton_enqueue_into(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                        std::uint64_t n, const SegmentView& segs, const double* m,
+                        int first_digit, const OrderScratch& b, bool timed) {
+    if (n == 0) return 0;
+    if (n > (std::uint64_t{1} << 31)) fail(ORR_ERR_LENGTH, "morton: more than 2^31 particles");
+    const bool segmented = segs.count > 1;
+    const OrderSizes sz = order_sizes(n, segs.count, segs.chunk_count, m != nullptr);
+    const std::uint32_t bbox_blocks = sz.bbox_blocks;
+    const std::uint32_t sort_blocks = sz.sort_blocks;
+    double* cube = b.bbox;  // cubes[4 * segment]
     double* partial = cube + 4 * static_cast<std::size_t>(segs.count);
     {
-        PhaseTimer t(ctx, kBbox);
+        PhaseTimer t(ctx, kBbox, timed);
         if (segmented) {
             hipLaunchKernelGGL(bbox_chunks, dim3(segs.chunk_count), dim3(kBlock), 0, ctx->stream, x, y,
                                z, segs.chunks, partial);
@@ -413,25 +434,25 @@ void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, con
             ORR_LAUNCHED("bbox_final");
         }
     }
-    std::uint64_t* keys[2] = {ctx->order_codes.as<std::uint64_t>(), ctx->order_codes_alt.as<std::uint64_t>()};
-    std::uint32_t* vals[2] = {ctx->order_index.as<std::uint32_t>(), ctx->order_index_alt.as<std::uint32_t>()};
+    std::uint64_t* const* keys = b.codes;
+    std::uint32_t* const* vals = b.index;
     const std::uint32_t* seg_of = segmented ? segs.seg_of : nullptr;
     {
-        PhaseTimer t(ctx, kEncode);
+        PhaseTimer t(ctx, kEncode, timed);
         hipLaunchKernelGGL(encode, dim3(static_cast<std::uint32_t>((n + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, ctx->stream, x, y, z, m, n, cube, seg_of, keys[0], vals[0],
-                           m != nullptr ? ctx->order_aos.as<double4>() : nullptr);
+                           m != nullptr ? b.aos : nullptr);
         ORR_LAUNCHED("encode");
     }
+    int cur = 0;
     {
-        PhaseTimer t(ctx, kSort);
-        std::uint32_t* hist = ctx->order_hist.as<std::uint32_t>();
+        PhaseTimer t(ctx, kSort, timed);
+        std::uint32_t* hist = b.hist;
         // Eight passes over the 63-bit codes (from first_digit: the order of the
         // top digits only), then (segments > 1) one or two passes over the
         // segment of each element: segment-major, then (code, index).
         int passes = 8;
         if (segmented) passes += segs.count > 256 ? 2 : 1;
-        int cur = 0;
         for (int pass = first_digit; pass < passes; ++pass) {
             const bool by_segment = pass >= 8;
             const int shift = by_segment ? 8 * (pass - 8) : 8 * pass;
@@ -446,12 +467,8 @@ void morton_enqueue_segments(orr_ctx* ctx, const double* x, const double* y, con
             ORR_LAUNCHED("radix_scatter");
             cur = 1 - cur;
         }
-        if (cur != 0) {
-            // An odd pass count ends in the alternate buffers: swap the roles.
-            std::swap(ctx->order_codes, ctx->order_codes_alt);
-            std::swap(ctx->order_index, ctx->order_index_alt);
-        }
     }
+    return cur;
 }
 
 void morton_cube(orr_ctx* ctx, double cube[4]) {
