@@ -28,9 +28,11 @@ Further legs in the same JSON line:
   systems stepped one by one;
 * ``served``     -- Poisson arrivals of run requests (N log-uniform in [1e3, 2e6],
   70/30 Plummer / two-disc collision, 100-1000 Yoshida-4 or velocity Verlet
-  steps) through the continuous-batching server: p50/p99 time to completion,
-  throughput, per-request energy drift against the CPU reference, and beside
-  them the same trace served one request at a time on the single-system path;
+  steps) at 0.8 of the single-system path's service rate, through the
+  continuous-batching server: p50/p99 time to completion, throughput,
+  per-request energy drift against the CPU reference, and beside them the same
+  requests and arrivals served one at a time on the single-system path (every
+  request run whole; first-in-first-out latencies);
 * ``cpu_baseline`` and the ``force_error`` figures -- the CPU leg: the
   reference's threaded AVX2 FP64 kernels (or the oracle's restatement) timed on
   this host's cores under the reference's protocol (2 warm-ups, settling within
@@ -614,25 +616,82 @@ def bench_batched(E, W, ctx, count: int, steps: int, alone_steps: int, cpu) -> d
     }
 
 
-def bench_serving(E, W, ctx, count: int, rate: float, seed: int, cpu) -> dict:
-    """Poisson arrivals of run requests served by the continuous-batching loop."""
+def make_request_state(spec):
+    """One request's initial state (host only)."""
+    kind, n, seed = spec
+    from orrery_amd import workloads as W
+    p = W.plummer(n, seed) if kind == "plummer" else W.disc_collision(n, seed)
+    return p.arrays()
+
+
+def bench_serving(E, W, ctx, count: int, rho: float, seed: int, cpu, use_graph: bool = True) -> dict:
+    """The mixed request stream: continuous batching against one request at a
+    time on the single-system path, on the same requests and arrival times.
+
+    1. Each request alone, first in first out, on the device-resident
+       single-system path (upload, start diagnostics, refresh, every step, end
+       diagnostics, download), timed whole: its service time.
+    2. Arrivals: a seeded unit-rate Poisson sequence scaled to the rate
+       lambda = rho / mean(service), so the single-system server is busy a
+       fraction rho of the time. The FIFO latencies follow from the arrival
+       and service times exactly (one server, requests in arrival order).
+    3. The same arrivals served in real time by the continuous-batching loop.
+    """
     eps, dt, below = 0.01, 1e-3, 4096
     rng = np.random.default_rng(seed)
     sizes = [int(v) for v in np.exp(rng.uniform(np.log(1e3), np.log(2e6), count))]
     plummer = rng.random(count) < 0.7
     steps = [int(v) for v in rng.integers(100, 1001, count)]
     integ = ["yoshida4" if u < 0.5 else "velocity-verlet" for u in rng.random(count)]
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    unit_arrivals = np.cumsum(rng.exponential(1.0, count))
     log(f"[bench] serving: generating {count} requests ({sum(sizes)} particles)")
-    work = [W.plummer(n, 1000 + k) if plummer[k] else W.disc_collision(n, 1000 + k)
-            for k, n in enumerate(sizes)]
+    specs = [("plummer" if plummer[k] else "disc", n, 1000 + k) for k, n in enumerate(sizes)]
+    work = [W.Workload(*make_request_state(spec)) for spec in specs]
+
+    # 1. One at a time on the single-system path.
+    log(f"[bench] serving: {count} requests one at a time on the single-system path")
+    service, diag_s = [], []
+    for k in range(count):
+        cfg = E.StepConfig(integrator=integ[k], solver="direct" if sizes[k] < below else "barnes-hut",
+                           softening=eps, opening_angle=0.5, leaf_capacity=16)
+        t0 = time.perf_counter()
+        s = E.DeviceSystem(ctx, sizes[k])
+        s.upload(*work[k].arrays())
+        t1 = time.perf_counter()
+        s.diagnostics(eps)
+        d0 = time.perf_counter() - t1
+        s.refresh(cfg)
+        s.step(cfg, dt, steps[k])
+        t2 = time.perf_counter()
+        s.diagnostics(eps)
+        d1 = time.perf_counter() - t2
+        s.download()
+        s.close()
+        service.append(time.perf_counter() - t0)
+        diag_s.append(d0 + d1)
+        log(f"[bench] serving: alone {k + 1}/{count} N={sizes[k]} {integ[k]} {steps[k]} steps "
+            f"{service[-1]:.2f} s")
+    service = np.array(service)
+    rate = rho / float(service.mean())
+    arrivals = unit_arrivals / rate
+    free_at, fifo = 0.0, []
+    for k in range(count):
+        free_at = max(float(arrivals[k]), free_at) + float(service[k])
+        fifo.append(free_at - float(arrivals[k]))
+    fifo = np.array(fifo)
+    big = int(np.argmax(sizes))
+
+    # 3. Continuous batching, real time, same arrivals.
     server = E.Server(ctx, E.ServerConfig(round_ms=50.0, direct_below=below, alone_from=131072,
-                                          opening_angle=0.5, leaf_capacity=16, tiles_per_item=16))
-    log(f"[bench] serving: {count} requests at {rate}/s")
+                                          opening_angle=0.5, leaf_capacity=16, tiles_per_item=16,
+                                          use_graph=use_graph))
+    log(f"[bench] serving: {count} requests through the continuous-batching loop, "
+        f"{rate:.4f} requests/s (rho {rho})")
     ids = {}
     start = server.now()
     nxt = 0
     pending = 0
+    last_log = time.perf_counter()
     while nxt < count or pending > 0:
         now = server.now() - start
         while nxt < count and arrivals[nxt] <= now:
@@ -643,6 +702,9 @@ def bench_serving(E, W, ctx, count: int, rate: float, seed: int, cpu) -> dict:
             time.sleep(max(0.0, arrivals[nxt] - (server.now() - start)))
             continue
         pending = server.round()
+        if time.perf_counter() - last_log > 30:
+            last_log = time.perf_counter()
+            log(f"[bench] serving: {nxt} submitted, {pending} pending at {now:.0f} s")
     makespan = server.now() - start
     st = [server.status(ids[k]) for k in range(count)]
     stats = server.stats()
@@ -651,39 +713,11 @@ def bench_serving(E, W, ctx, count: int, rate: float, seed: int, cpu) -> dict:
     queued = np.array([s["admitted"] - s["submitted"] for s in st])
     drift = [(s["end"]["total"] - s["start"]["total"]) / abs(s["start"]["total"]) for s in st]
     particle_steps = sum(n * k for n, k in zip(sizes, steps))
+    large = np.array(sizes) >= 131072
 
-    # The plain single-system form on the same arrival trace: each request alone
-    # on the device-resident path (orr_system: upload, start diagnostics, refresh,
-    # steps, end diagnostics, download), served first in, first out. Each service
-    # time is measured on the device: set-up and first step, then up to 100 more
-    # steps whose mean stands for the rest.
-    log("[bench] serving: the same requests one at a time on the single-system path")
-    service = []
-    for k in range(count):
-        cfg = E.StepConfig(integrator=integ[k], solver="direct" if sizes[k] < below else "barnes-hut",
-                           softening=eps, opening_angle=0.5, leaf_capacity=16)
-        t0 = time.perf_counter()
-        s = E.DeviceSystem(ctx, sizes[k])
-        s.upload(*work[k].arrays())
-        s.diagnostics(eps)
-        s.refresh(cfg)
-        s.step(cfg, dt, 1)
-        first = time.perf_counter() - t0
-        rest = min(steps[k] - 1, 100)
-        t1 = time.perf_counter()
-        if rest > 0:
-            s.step(cfg, dt, rest)
-        per_step = (time.perf_counter() - t1) / max(1, rest)
-        t2 = time.perf_counter()
-        s.diagnostics(eps)
-        s.download()
-        s.close()
-        service.append(first + per_step * (steps[k] - 1) + (time.perf_counter() - t2))
-    free_at, fifo = 0.0, []
-    for k in range(count):
-        free_at = max(float(arrivals[k]), free_at) + service[k]
-        fifo.append(free_at - float(arrivals[k]))
-    fifo = np.array(fifo)
+    def pct(v):
+        return {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99)),
+                "max": float(v.max()), "min": float(v.min()), "mean": float(v.mean())}
 
     # Energy drift against the CPU reference (FP64 direct, same integrator and
     # initial state) for the requests the CPU can redo in a bounded time.
@@ -711,26 +745,36 @@ def bench_serving(E, W, ctx, count: int, rate: float, seed: int, cpu) -> dict:
         cpu_drift = {"requests": rows, "cores": cpu.threads, "seconds": spent,
                      "reference": "oracle FP64 direct sum and integrator, same initial state"}
     return {
-        "workload": f"{count} requests, Poisson arrivals at {rate}/s, N log-uniform in [1e3, 2e6], "
-                    f"70/30 Plummer / two-disc collision, 100-1000 steps, Yoshida-4 or velocity "
-                    f"Verlet, eps {eps}, dt {dt}; direct below {below} particles, Barnes-Hut "
-                    f"(theta 0.5, leaf 16) above",
-        "requests": count, "arrival_rate_per_s": rate, "seed": seed,
+        "workload": f"{count} requests, Poisson arrivals at rho {rho} of the single-system path's "
+                    f"measured service (lambda {rate:.4f}/s), N log-uniform in [1e3, 2e6], 70/30 "
+                    f"Plummer / two-disc collision, 100-1000 steps, Yoshida-4 or velocity Verlet, "
+                    f"eps {eps}, dt {dt}; direct below {below} particles, Barnes-Hut (theta 0.5, "
+                    f"leaf 16) above; requests of 131,072 particles or more run alone, one at a time",
+        "requests": count, "arrival_rate_per_s": rate, "utilisation_target": rho, "seed": seed,
+        "large_requests": int(large.sum()),
         "particles_total": int(sum(sizes)), "particle_steps": int(particle_steps),
         "makespan_s": makespan, "requests_per_s": count / makespan,
         "particle_steps_per_s": particle_steps / makespan,
-        "latency_s": {"p50": float(np.percentile(latency, 50)), "p99": float(np.percentile(latency, 99)),
-                      "max": float(latency.max()), "min": float(latency.min())},
+        "latency_s": pct(latency),
+        "latency_small_s": pct(latency[~large]) if (~large).any() else None,
+        "latency_large_s": pct(latency[large]) if large.any() else None,
         "queue_wait_s": {"p50": float(np.percentile(queued, 50)), "max": float(queued.max())},
         "single_system_fifo": {
-            "latency_s": {"p50": float(np.percentile(fifo, 50)), "p99": float(np.percentile(fifo, 99)),
-                          "max": float(fifo.max()), "min": float(fifo.min())},
+            "latency_s": pct(fifo),
+            "latency_small_s": pct(fifo[~large]) if (~large).any() else None,
+            "latency_large_s": pct(fifo[large]) if large.any() else None,
             "makespan_s": float(max(float(arrivals[k]) + fifo[k] for k in range(count))),
-            "busy_s": float(sum(service)),
-            "method": "each request alone on the device-resident single-system path, first in first "
-                      "out on the same arrival times; service time = measured set-up + first step + "
-                      "the mean of up to 100 further steps x the remaining steps + end diagnostics "
-                      "and download"},
+            "busy_s": float(service.sum()),
+            "method": "each request alone on the device-resident single-system path, every step "
+                      "run and the whole request timed (upload, start diagnostics, refresh, steps, "
+                      "end diagnostics, download); first-in-first-out latencies from those "
+                      "service times and the same arrival times"},
+        "diagnostics": {"largest_request_n": sizes[big],
+                        "largest_request_diagnostics_s": diag_s[big],
+                        "largest_request_service_s": float(service[big]),
+                        "share_of_all_service": float(sum(diag_s) / service.sum()),
+                        "note": "start + end FP64 diagnostics (O(N^2) pairwise potential) per request"},
+        "graphs": use_graph,
         "rounds": stats["rounds"], "batch_builds": stats["batch_builds"],
         "busy_s": stats["busy_s"], "peak_resident_bytes": stats["peak_resident_bytes"],
         "memory_budget": stats["memory_budget"],
@@ -758,9 +802,12 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--alone-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=16)
-    ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rho", type=float, default=0.8,
+                    help="arrival rate as a fraction of the single-system path's service rate")
     ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--serve-no-graph", action="store_true",
+                    help="served batches step eagerly instead of replaying a captured hipGraph")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-direct", action="store_true", help="skip the headline direct leg")
     ap.add_argument("--n65536-steps", type=int, default=20,
@@ -901,8 +948,8 @@ def main() -> int:
         result["batched"] = bench_batched(E, W, ctx, args.batch_systems, args.batch_steps,
                                           args.alone_steps, cpu)
     if not args.no_serving:
-        result["served"] = bench_serving(E, W, ctx, args.serve_requests, args.serve_rate,
-                                         args.serve_seed, cpu)
+        result["served"] = bench_serving(E, W, ctx, args.serve_requests, args.serve_rho,
+                                         args.serve_seed, cpu, use_graph=not args.serve_no_graph)
     log("[bench] ceilings")
     result["ceilings"] = measure_ceilings(ctx)
     ctx.close()

@@ -408,14 +408,16 @@ ORR_API int orr_batch_counts(orr_batch* batch, uint32_t system, uint64_t out[2])
  * Requests (an initial state, an integrator and a step count) are admitted
  * against an HBM budget as earlier ones finish; admitted requests stay on the
  * device until done and are stepped in batches (small systems of one
- * integrator together, large ones alone), all batches concurrently, in
- * scheduling rounds. A request's result does not depend on what else runs. */
+ * integrator together, large ones -- alone_from particles or more -- alone,
+ * one large request at a time in arrival order), every batch advancing on its
+ * own streams in quanta of steps, concurrently with the others. A request's
+ * result does not depend on what else runs. */
 typedef struct orr_server orr_server;
 
 typedef struct {
     uint64_t memory_budget;          /* bytes of HBM for admitted requests; 0: the device's
                                         free memory at creation, at most 288 GB */
-    double round_ms;                 /* target duration of one scheduling round (0: 50 ms) */
+    double round_ms;                 /* target duration of one batch's quantum of steps (0: 50 ms) */
     uint32_t direct_below;           /* direct solver below this many particles (0: 4096) */
     uint32_t alone_from;             /* requests this large get a batch of their own (0: 131072) */
     double opening_angle;            /* Barnes-Hut requests (0: 0.5) */
@@ -457,7 +459,7 @@ typedef struct {
     uint64_t rounds, batch_builds, particle_steps, completed;
     uint64_t resident_bytes, peak_resident_bytes, memory_budget;
     uint32_t active, queued;
-    double busy_s;                   /* wall time inside rounds */
+    double busy_s;                   /* wall time with some batch's quantum in flight */
 } orr_server_stats;
 
 ORR_API int orr_server_create(orr_ctx* ctx, const orr_server_config* config, orr_server** out);
@@ -465,8 +467,11 @@ ORR_API int orr_server_destroy(orr_server* server);
 /* Seconds since the server was created (the clock of the status times). */
 ORR_API int orr_server_now(orr_server* server, double* seconds);
 ORR_API int orr_server_submit(orr_server* server, const orr_request* request, uint64_t* id);
-/* One scheduling round: admit, (re)batch, step every batch concurrently,
- * retire. `pending` gets the number of requests queued or running after it. */
+/* One pass of the loop, which never waits for the device: retire the quanta
+ * that finished (and the requests they finished), admit, (re)batch, and start
+ * the next quantum on every batch that has none in flight; sleeps 0.1 ms when
+ * there was nothing to do. Call it repeatedly (submitting between calls).
+ * `pending` gets the number of requests queued or running after it. */
 ORR_API int orr_server_round(orr_server* server, uint32_t* pending);
 ORR_API int orr_server_status(orr_server* server, uint64_t id, orr_request_status* out);
 /* Final state of a finished request (host arrays, null: skipped); then its

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "engine.hpp"
@@ -109,7 +110,11 @@ struct orr_batch {
     hipEvent_t fork = nullptr, join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::uint64_t graph_epoch = 0;
+    // The device buffers a captured step was recorded with (its launches hold
+    // their addresses by value): the graph replays while these are unchanged.
+    std::vector<std::pair<const void*, std::size_t>> graph_buffers;
+    // dctx and tctx lent by the caller (orr::batch_create_on): kept, not freed.
+    bool borrowed = false;
 
     double* array(int k) const { return state.as<double>() + static_cast<std::size_t>(k) * n; }
     std::uint32_t systems_internal() const { return D + T; }
@@ -128,6 +133,20 @@ void drop_graph(orr_batch* b) noexcept {
 void sync_all(orr_batch* b) {
     b->tctx->sync();
     b->dctx->sync();
+}
+
+// Address and size of every device buffer a step touches (the batch's own and
+// its two contexts' scratch), in address order: the ordering stage exchanges
+// its two key and index buffers after an odd number of radix passes, which a
+// captured step replays by itself, so only a new or resized buffer counts.
+std::vector<std::pair<const void*, std::size_t>> step_buffers(const orr_batch* b) {
+    std::vector<std::pair<const void*, std::size_t>> v;
+    for (const Buffer* x : {&b->state, &b->tables}) v.emplace_back(x->ptr, x->bytes);
+    for (orr_ctx* c : {b->dctx, b->tctx}) {
+        for (const Buffer* x : c->buffers()) v.emplace_back(x->ptr, x->bytes);
+    }
+    std::sort(v.begin(), v.end());
+    return v;
 }
 
 // Host tables for the current tiles_per_item; uploads everything and drops
@@ -333,12 +352,22 @@ void step_once(orr_batch* b) {
 
 void capture(orr_batch* b) {
     drop_graph(b);
-    const std::uint64_t epoch = g_reallocations.load();
+    const auto before = step_buffers(b);
     hipStream_t st = b->dctx->stream;
     ORR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     try {
         step_once(b);
     } catch (...) {
+        // A failure between the fork and the join leaves the Barnes-Hut
+        // stream inside the capture: join it back first, or ending the
+        // capture on the direct stream fails as unjoined and leaves that
+        // stream capturing for good.
+        hipStreamCaptureStatus status = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(b->tctx->stream, &status) == hipSuccess &&
+            status != hipStreamCaptureStatusNone) {
+            (void)hipEventRecord(b->join, b->tctx->stream);
+            (void)hipStreamWaitEvent(st, b->join, 0);
+        }
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(st, &g);
         if (g != nullptr) (void)hipGraphDestroy(g);
@@ -346,11 +375,11 @@ void capture(orr_batch* b) {
     }
     ORR_HIP(hipStreamEndCapture(st, &b->graph));
     ORR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    if (g_reallocations.load() != epoch) {
+    if (step_buffers(b) != before) {
         drop_graph(b);
         fail(ORR_ERR_RUNTIME, "batch: device buffers grew while a step was captured");
     }
-    b->graph_epoch = epoch;
+    b->graph_buffers = before;
 }
 
 // Enqueues `steps` steps on the batch's streams. Only the first step after a
@@ -358,11 +387,11 @@ void capture(orr_batch* b) {
 void step_enqueue(orr_batch* b, std::uint64_t steps) {
     if (b->n == 0) return;
     for (std::uint64_t k = 0; k < steps; ++k) {
-        if (b->cfg.use_graph == 0) {
+        if (b->cfg.use_graph == 0 || !b->dctx->graphs) {
             step_once(b);
             continue;
         }
-        if (b->exec != nullptr && b->graph_epoch == g_reallocations.load()) {
+        if (b->exec != nullptr && step_buffers(b) == b->graph_buffers) {
             ORR_HIP(hipGraphLaunch(b->exec, b->dctx->stream));
             continue;
         }
@@ -442,8 +471,10 @@ void destroy(orr_batch* b) noexcept {
     if (b->join != nullptr) (void)hipEventDestroy(b->join);
     b->state.release();
     b->tables.release();
-    free_context(b->dctx);
-    free_context(b->tctx);
+    if (!b->borrowed) {
+        free_context(b->dctx);
+        free_context(b->tctx);
+    }
     delete b;
 }
 
@@ -505,11 +536,14 @@ void batch_download_enqueue(orr_batch* b, std::uint32_t system, double* host) {
 using orr::guarded;
 using orr::require;
 
-extern "C" {
+namespace orr {
 
-int orr_batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t count,
-                     const orr_batch_config* config, orr_batch** out) {
-    return guarded([&] {
+// orr_batch_create's body; with dctx and tctx given, the batch runs on those
+// contexts (streams and scratch) and leaves them to the caller on destroy.
+void batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t count,
+                  const orr_batch_config* config, orr_ctx* lend_d, orr_ctx* lend_t,
+                  orr_batch** out) {
+    {
         require(ctx != nullptr && out != nullptr && config != nullptr, "null argument");
         require(count == 0 || systems != nullptr, "systems is null");
         *out = nullptr;
@@ -561,8 +595,14 @@ int orr_batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t cou
         b->tune_leaf = config->leaf_capacity == 0;
         b->tiles_per_item = b->tune_items ? 16u : config->tiles_per_item;
         b->leaf_capacity = b->tune_leaf ? 16u : config->leaf_capacity;
-        b->dctx = orr::make_context(ctx->device);
-        b->tctx = orr::make_context(ctx->device);
+        if (lend_d != nullptr && lend_t != nullptr) {
+            b->dctx = lend_d;
+            b->tctx = lend_t;
+            b->borrowed = true;
+        } else {
+            b->dctx = orr::make_context(ctx->device);
+            b->tctx = orr::make_context(ctx->device);
+        }
         // The owner's kernel settings carry over to the batch's private contexts.
         b->tctx->walk_targets_per_lane = ctx->walk_targets_per_lane;
         b->dctx->direct_refine = ctx->direct_refine;
@@ -575,7 +615,24 @@ int orr_batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t cou
         ORR_HIP(hipMemsetAsync(b->state.ptr, 0, b->state.bytes, b->dctx->stream));
         orr::plan(b.get());
         *out = b.release();
-    });
+    }
+}
+
+orr_batch* batch_create_on(orr_ctx* owner, const orr_batch_system* systems, std::uint32_t count,
+                           const orr_batch_config* config, orr_ctx* dctx, orr_ctx* tctx) {
+    require(dctx != nullptr && tctx != nullptr, "null context");
+    orr_batch* b = nullptr;
+    batch_create(owner, systems, count, config, dctx, tctx, &b);
+    return b;
+}
+
+}  // namespace orr
+
+extern "C" {
+
+int orr_batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t count,
+                     const orr_batch_config* config, orr_batch** out) {
+    return guarded([&] { orr::batch_create(ctx, systems, count, config, nullptr, nullptr, out); });
 }
 
 int orr_batch_destroy(orr_batch* batch) {

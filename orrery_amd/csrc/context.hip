@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -56,12 +57,13 @@ std::size_t ctx_device_bytes(const orr_ctx* ctx) {
 
 void drop_captured(orr_ctx::Captured& g) noexcept {
     if (g.exec != nullptr) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph != nullptr) (void)hipGraphDestroy(g.graph);
     g = orr_ctx::Captured{};
 }
 
 void run_captured(orr_ctx* ctx, orr_ctx::Captured& g, std::vector<unsigned char> key,
                   const std::function<void()>& enqueue, bool untimed) {
-    if (ctx->timing && !untimed) {
+    if (!ctx->graphs || (ctx->timing && !untimed)) {
         drop_captured(g);
         enqueue();
         return;
@@ -93,8 +95,8 @@ void run_captured(orr_ctx* ctx, orr_ctx::Captured& g, std::vector<unsigned char>
             throw;
         }
         ORR_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        g.graph = graph;
         const hipError_t made = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
         if (made != hipSuccess || g.epoch != g_reallocations.load()) {
             // Nothing may allocate while capturing; if something did, run eagerly.
             drop_captured(g);
@@ -115,6 +117,8 @@ orr_ctx* make_context(int device) {
     ctx->compute_units = p.multiProcessorCount;
     // Default priority on purpose: streams of mixed priority are not used.
     ORR_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    const char* graphs = std::getenv("ORR_GRAPHS");
+    ctx->graphs = !(graphs != nullptr && std::strcmp(graphs, "0") == 0);
     return ctx.release();
 }
 

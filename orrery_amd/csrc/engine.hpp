@@ -140,6 +140,10 @@ struct orr_ctx {
 
     // Phase timing with hipEvents recorded on `stream`.
     bool timing = false;
+    // Captured graphs allowed (false when the environment sets ORR_GRAPHS=0:
+    // every sequence then runs eagerly, for tools that cannot follow graph
+    // launches).
+    bool graphs = true;
     struct Pending {
         int phase;
         hipEvent_t start, stop;
@@ -155,6 +159,7 @@ struct orr_ctx {
         std::vector<unsigned char> key;  // the arguments and settings, field by field
         std::uint64_t epoch = 0;         // g_reallocations when they last ran
         bool seen = false;               // this key has run once eagerly
+        hipGraph_t graph = nullptr;      // kept for as long as its executable
         hipGraphExec_t exec = nullptr;
     };
     Captured tree_graph, direct_graph;
@@ -348,8 +353,12 @@ std::uint64_t system_count(const orr_system* s);
 void system_download(orr_system* s, int first, int arrays, double* host);
 void system_upload(orr_system* s, int first, int arrays, const double* host);
 
-// batch.hip, for the serving loop: enqueue steps of a batch on its own streams
-// without waiting (batches run concurrently), then wait for them.
+// batch.hip, for the serving loop: a batch on the caller's two contexts (its
+// streams and scratch, which outlive the batch: a rebuilt batch reuses them).
+orr_batch* batch_create_on(orr_ctx* owner, const orr_batch_system* systems, std::uint32_t count,
+                           const orr_batch_config* config, orr_ctx* dctx, orr_ctx* tctx);
+// Enqueue steps of a batch on its own streams without waiting (batches run
+// concurrently), then wait for them.
 void batch_step_enqueue(orr_batch* b, std::uint64_t steps);
 void batch_wait(orr_batch* b);
 // The stream every step of the batch starts and ends on.

@@ -82,38 +82,89 @@ struct Compensated {
 constexpr int kTerms = 8;  // ke, px, py, pz, lx, ly, lz, pe
 
 // Columns of the potential's rows are summed in at most kColumnChunks chunks
-// of at least kColumnMin columns, so a system of a few thousand particles
-// still spreads over many waves.
+// of at least kColumnMin columns (whole tiles of kPotTile), so a system of a
+// few thousand particles still spreads over many waves.
+constexpr int kPotTile = 256;  // columns staged in LDS per step = threads per block
 constexpr std::uint64_t kColumnMin = 2048;
 constexpr std::uint64_t kColumnChunks = 16;
 
 __host__ __device__ inline std::uint64_t column_width(std::uint64_t n) {
     const std::uint64_t even = (n + kColumnChunks - 1) / kColumnChunks;
-    return even > kColumnMin ? even : kColumnMin;
+    const std::uint64_t w = even > kColumnMin ? even : kColumnMin;
+    return (w + kPotTile - 1) / kPotTile * kPotTile;
 }
 
-// Row i of the potential over column chunk blockIdx.y: plain FP64 sum over
-// j > i of m_j / sqrt(r^2 + e^2) in index order (diagnostics.cpp:129-158 sums
+// 1 / sqrt(v) in FP64: the hardware reciprocal square root refined by two
+// Newton steps, y <- y (3/2 - (v/2) y^2) with the residual by FMA (each step
+// squares the relative error, so the result is within an ulp or two).
+// Branch-free: v = 0 gives +inf and NaN stays NaN, as 1 / sqrt(v) would.
+__device__ inline double inverse_sqrt(double v) {
+    const double h = 0.5 * v;
+    double y = __builtin_amdgcn_rsq(v);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-h, y * y, 0.5);
+        y = fma(y, e, y);
+    }
+    return v > 0.0 ? y : (v == 0.0 ? __builtin_huge_val() : __builtin_nan(""));
+}
+
+// m / sqrt(|d|^2 + e2) for the separation d = (dx, dy, dz).
+__device__ inline double pair_potential(double m, double dx, double dy, double dz, double e2) {
+    return m * inverse_sqrt(fma(dz, dz, fma(dy, dy, fma(dx, dx, e2))));
+}
+
+// Row i of the potential over column chunk blockIdx.y: a plain FP64 sum over
+// j > i of m_j / sqrt(r^2 + e^2) in index order (r^2 + e^2 by FMA) (diagnostics.cpp:129-158 sums
 // the whole row plainly; here the chunks' sums are folded in chunk order).
-__global__ __launch_bounds__(256) void potential_rows(std::uint64_t n, std::uint64_t width,
-                                                      const double* __restrict__ x,
-                                                      const double* __restrict__ y,
-                                                      const double* __restrict__ z,
-                                                      const double* __restrict__ m, double e2,
-                                                      double* __restrict__ rows) {
-    const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// A block holds kPotTile rows; the chunk's columns pass through LDS a tile at
+// a time and every lane reads each staged column (a broadcast). Blocks wholly
+// below the diagonal write zeros and leave.
+__global__ __launch_bounds__(kPotTile) void potential_rows(std::uint64_t n, std::uint64_t width,
+                                                           const double* __restrict__ x,
+                                                           const double* __restrict__ y,
+                                                           const double* __restrict__ z,
+                                                           const double* __restrict__ m, double e2,
+                                                           double* __restrict__ rows) {
+    __shared__ double sx[kPotTile], sy[kPotTile], sz[kPotTile], sm[kPotTile];
+    const std::uint64_t row0 = static_cast<std::uint64_t>(blockIdx.x) * kPotTile;
+    const std::uint64_t i = row0 + threadIdx.x;
     const std::uint64_t c0 = static_cast<std::uint64_t>(blockIdx.y) * width;
-    if (i >= n) return;
+    const std::uint64_t c1 = c0 + width < n ? c0 + width : n;
+    // First column any row of the block needs: row0 + 1.
+    const std::uint64_t j_begin = c0 > row0 + 1 ? c0 : row0 + 1;
     double inter = 0.0;
-    const std::uint64_t j1 = c0 + width < n ? c0 + width : n;
-    if (j1 > i + 1) {
-        const double xi = x[i], yi = y[i], zi = z[i];
-        for (std::uint64_t j = (i + 1 > c0 ? i + 1 : c0); j < j1; ++j) {
-            const double dx = x[j] - xi, dy = y[j] - yi, dz = z[j] - zi;
-            inter += m[j] * (1.0 / sqrt(((dx * dx) + (dy * dy) + (dz * dz)) + e2));
+    if (j_begin < c1) {
+        const bool valid_row = i < n;
+        const double xi = valid_row ? x[i] : 0.0, yi = valid_row ? y[i] : 0.0;
+        const double zi = valid_row ? z[i] : 0.0;
+        // Tiles aligned to kPotTile from the first one holding j_begin.
+        for (std::uint64_t t0 = j_begin / kPotTile * kPotTile; t0 < c1; t0 += kPotTile) {
+            __syncthreads();
+            const std::uint64_t j = t0 + threadIdx.x;
+            const bool in = j < c1;
+            sx[threadIdx.x] = in ? x[j] : 0.0;
+            sy[threadIdx.x] = in ? y[j] : 0.0;
+            sz[threadIdx.x] = in ? z[j] : 0.0;
+            sm[threadIdx.x] = in ? m[j] : 0.0;
+            __syncthreads();
+            const std::uint64_t t1 = t0 + kPotTile < c1 ? t0 + kPotTile : c1;
+            const int count = static_cast<int>(t1 - t0);
+            if (t0 > row0 + kPotTile - 1 && count == kPotTile) {
+                // Every column of the tile lies above every row of the block.
+#pragma unroll 4
+                for (int k = 0; k < kPotTile; ++k) {
+                    inter += pair_potential(sm[k], sx[k] - xi, sy[k] - yi, sz[k] - zi, e2);
+                }
+            } else {
+                for (int k = 0; k < count; ++k) {
+                    if (t0 + k <= i) continue;
+                    inter += pair_potential(sm[k], sx[k] - xi, sy[k] - yi, sz[k] - zi, e2);
+                }
+            }
         }
     }
-    rows[static_cast<std::uint64_t>(blockIdx.y) * n + i] = inter;
+    if (i < n) rows[static_cast<std::uint64_t>(blockIdx.y) * n + i] = inter;
 }
 
 // Per-particle terms, term-major [kTerms][n].
@@ -250,8 +301,8 @@ void diagnostics_enqueue(orr_ctx* ctx, std::uint64_t n, const double* x, const d
     double* result = parts + 2 * kTerms * chunks;
     PhaseTimer t(ctx, kEnergy);
     const std::uint32_t row_blocks = static_cast<std::uint32_t>((n + 255) / 256);
-    hipLaunchKernelGGL(potential_rows, dim3(row_blocks, columns), dim3(256), 0, ctx->stream, n,
-                       width, x, y, z, m, e2, rows);
+    hipLaunchKernelGGL(potential_rows, dim3(static_cast<std::uint32_t>((n + kPotTile - 1) / kPotTile), columns),
+                       dim3(kPotTile), 0, ctx->stream, n, width, x, y, z, m, e2, rows);
     ORR_LAUNCHED("potential_rows");
     hipLaunchKernelGGL(diagnostic_terms, dim3(row_blocks), dim3(256), 0, ctx->stream, n, x, y, z,
                        vx, vy, vz, m, rows, columns, G, terms);

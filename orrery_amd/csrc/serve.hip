@@ -1,24 +1,33 @@
 // Serving: a request loop with continuous batching (host code).
 //
 // Requests carry an initial state, an integrator, a timestep and a step count.
-// The server admits queued requests in arrival order as the HBM budget allows
-// (first fit: a request that does not fit lets later, smaller ones pass),
-// keeps each admitted request's state on the device until it is done, and
-// advances all of them in scheduling rounds:
+// Admitted requests keep their state on the device until done. Small requests
+// of one integrator (and one softening and G) share a batch; a request of
+// `alone_from` particles or more is large and gets a batch of its own. Every
+// group (one batch, on a pair of contexts -- streams and scratch -- that it
+// keeps across rebuilds) advances on its own, concurrently with the others:
+// there is no round barrier. One call of round():
 //
-//   1. admit what fits;
-//   2. rebuild the batches whose membership changed: small requests of one
-//      integrator (and one softening and G) share a batch, requests of
-//      `alone_from` particles or more get their own; a continuing request's
-//      state moves device to device, a new one is uploaded and its starting
-//      diagnostics and the refresh are enqueued on the batch's stream;
-//   3. enqueue a quantum of steps on every batch -- each batch on its own
-//      streams, so the batches run concurrently -- sized so that the round
-//      lasts about `round_ms` by the batch's last measured step time, and never
-//      past the nearest finish of its members; wait for all;
-//   4. retire the requests whose last quantum ran: their end diagnostics and
-//      final state were enqueued behind that quantum on their batch's stream,
-//      so they overlap the other batches' work; memory returns to the budget.
+//   1. retires the quanta that finished (polled by event, never waited for):
+//      their members' step counts advance; requests whose last quantum ran
+//      are done -- their end diagnostics and final state were enqueued behind
+//      that quantum on their batch's stream -- and their memory returns to
+//      the budget;
+//   2. admits queued requests in arrival order as the HBM budget allows (first
+//      fit: a small request that does not fit lets later, smaller ones pass).
+//      Large requests run one at a time, in arrival order: a later large
+//      request waits for the one ahead of it to finish. Small requests never
+//      wait for a large one, and a large one is never stretched by another
+//      large one sharing the device, so the longest requests finish no later
+//      than first-in-first-out service would finish them, while the small ones
+//      overtake;
+//   3. on every group with no quantum in flight: rebuilds its batch if its
+//      membership changed (a continuing request's state moves device to
+//      device; a new one is uploaded, its starting diagnostics and the refresh
+//      are enqueued on the batch's stream), then enqueues its next quantum of
+//      steps, sized to about `round_ms` by the group's last measured step time
+//      and never past the nearest finish of its members;
+//   4. sleeps 0.1 ms when none of this found anything to do.
 //
 // A request's arithmetic does not depend on its batch (see batch.hip), nor on
 // how its steps are split into quanta, so its result is the same alone and in
@@ -32,6 +41,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -78,21 +88,32 @@ struct GroupKey {
     }
 };
 
+// The two contexts a group's batches run on (direct and Barnes-Hut branch).
+struct Lanes {
+    orr_ctx* d = nullptr;
+    orr_ctx* t = nullptr;
+};
+
 struct Group {
     std::vector<std::uint64_t> members;  // request ids; batch system k = members[k] after a build
     std::map<std::uint64_t, std::uint32_t> slot;  // id -> system index in `batch`
     orr_batch* batch = nullptr;
+    Lanes lanes;
     bool dirty = false;
-    double ms_per_step = 0;  // last measured, sharing the device with the other batches
+    bool in_flight = false;  // a quantum is enqueued and not yet retired
+    double ms_per_step = 0;  // last measured, sharing the device with the other groups
     std::uint64_t particles = 0;
     std::uint64_t quantum = 0;
-    hipEvent_t start = nullptr, stop = nullptr;  // around the quantum, on the batch's stream
+    // Around the quantum's steps (timing), and after everything the quantum
+    // enqueued, finishing members' diagnostics and downloads included.
+    hipEvent_t start = nullptr, stop = nullptr, done = nullptr;
 };
 
 void release_events(Group& g) noexcept {
-    if (g.start != nullptr) (void)hipEventDestroy(g.start);
-    if (g.stop != nullptr) (void)hipEventDestroy(g.stop);
-    g.start = g.stop = nullptr;
+    for (hipEvent_t* e : {&g.start, &g.stop, &g.done}) {
+        if (*e != nullptr) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
 }
 
 void call(int status) {
@@ -112,7 +133,11 @@ struct orr_server {
     std::uint64_t resident = 0, peak = 0, budget = 0;
     std::uint64_t rounds = 0, builds = 0, particle_steps = 0, completed = 0;
     std::uint32_t fail_rebuilds = 0;  // testing aid: rebuilds still to fail
-    double busy = 0;
+    std::uint64_t large_active = 0;  // the admitted large request, 0: none
+    std::vector<Lanes> spare;        // contexts of groups that emptied, for new groups
+    double busy = 0;                 // wall time with some quantum in flight
+    double last_round = 0;
+    bool was_busy = false;
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -145,11 +170,43 @@ std::uint64_t request_bytes(const orr_server* s, std::uint64_t count) {
            orr::grown_bytes(orr::diagnostics_scratch_bytes(count)) + kBytesPerRequest;
 }
 
+Lanes take_lanes(orr_server* s) {
+    if (!s->spare.empty()) {
+        const Lanes l = s->spare.back();
+        s->spare.pop_back();
+        return l;
+    }
+    Lanes l;
+    l.d = orr::make_context(s->ctx->device);
+    try {
+        l.t = orr::make_context(s->ctx->device);
+    } catch (...) {
+        orr::free_context(l.d);
+        throw;
+    }
+    // The server context's kernel settings carry over.
+    l.t->walk_targets_per_lane = s->ctx->walk_targets_per_lane;
+    l.d->direct_refine = s->ctx->direct_refine;
+    return l;
+}
+
+// A group that emptied: its streams go back to the pool, its scratch is freed
+// (the budget charges scratch to the requests that used it).
+void give_back_lanes(orr_server* s, Group& g) {
+    if (g.lanes.d == nullptr) return;
+    for (orr_ctx* c : {g.lanes.d, g.lanes.t}) {
+        ORR_HIP(hipStreamSynchronize(c->stream));
+        for (orr::Buffer* b : c->buffers()) b->release();
+    }
+    s->spare.push_back(g.lanes);
+    g.lanes = Lanes{};
+}
+
 // Builds the group's batch from its members. Nothing the group or its
 // requests hold changes until the new batch is complete: a failure on the way
 // (an allocation, an upload, a device-to-device copy) leaves the old batch, the
 // slots and every new request's host copy as they were, so a later round can
-// try again.
+// try again. The old and the new batch share the group's streams.
 void rebuild(orr_server* s, const GroupKey& key, Group& g) {
     orr_batch* old = g.batch;
     if (g.members.empty()) {
@@ -160,6 +217,7 @@ void rebuild(orr_server* s, const GroupKey& key, Group& g) {
         if (old != nullptr) call(orr_batch_destroy(old));
         return;
     }
+    if (g.lanes.d == nullptr) g.lanes = take_lanes(s);
     std::vector<orr_batch_system> sys;
     std::uint64_t particles = 0;
     for (std::uint64_t id : g.members) {
@@ -169,8 +227,8 @@ void rebuild(orr_server* s, const GroupKey& key, Group& g) {
         particles += q.count;
     }
     const orr_batch_config bc = batch_config(s, key.integrator);
-    orr_batch* nb = nullptr;
-    call(orr_batch_create(s->ctx, sys.data(), static_cast<std::uint32_t>(sys.size()), &bc, &nb));
+    orr_batch* nb = orr::batch_create_on(s->ctx, sys.data(), static_cast<std::uint32_t>(sys.size()),
+                                         &bc, g.lanes.d, g.lanes.t);
     std::map<std::uint64_t, std::uint32_t> slot;
     try {
         for (std::uint32_t k = 0; k < g.members.size(); ++k) {
@@ -209,8 +267,7 @@ void rebuild(orr_server* s, const GroupKey& key, Group& g) {
     }
     g.particles = particles;
     ++s->builds;
-    // The copies are done: the old batch goes before the new one allocates its
-    // scratch, so the two never hold scratch at once.
+    // The copies are done (orr_batch_copy_system waits for them).
     if (old != nullptr) call(orr_batch_destroy(old));
     // Accelerations for the current positions (a continuing request's come
     // out as the bits it already had), enqueued before the steps.
@@ -218,10 +275,21 @@ void rebuild(orr_server* s, const GroupKey& key, Group& g) {
     g.dirty = false;
 }
 
+bool is_large(const orr_server* s, const Request& r) { return r.spec.count >= s->cfg.alone_from; }
+
 void admit(orr_server* s) {
+    // Large requests one at a time in arrival order: none passes an earlier
+    // large request that is running or still waiting for memory.
+    bool large_blocked = s->large_active != 0;
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         Request& r = s->requests.at(*it);
+        const bool large = is_large(s, r);
+        if (large && large_blocked) {
+            ++it;
+            continue;
+        }
         if (s->resident + r.bytes > s->budget) {
+            large_blocked = large_blocked || large;
             ++it;
             continue;
         }
@@ -235,6 +303,10 @@ void admit(orr_server* s) {
         r.admitted = s->now();
         s->resident += r.bytes;
         s->peak = std::max(s->peak, s->resident);
+        if (large) {
+            s->large_active = *it;
+            large_blocked = true;
+        }
         Group& g = s->groups[key_of(s, *it, r)];
         g.members.push_back(*it);
         g.dirty = true;
@@ -242,12 +314,23 @@ void admit(orr_server* s) {
     }
 }
 
-// After a round: requests whose last quantum ran (their end diagnostics and
-// final state were enqueued behind it) are done.
+// A finished quantum: step counts advance, and the requests it finished (their
+// end diagnostics and final state were enqueued behind it) are done.
 void retire(orr_server* s, Group& g) {
+    float ms = 0.f;
+    ORR_HIP(hipEventElapsedTime(&ms, g.start, g.stop));
+    g.ms_per_step = static_cast<double>(ms) / static_cast<double>(g.quantum);
+    g.in_flight = false;
     std::vector<std::uint64_t> keep;
     for (std::uint64_t id : g.members) {
         Request& r = s->requests.at(id);
+        // Admitted while the quantum ran: not in the batch yet, no steps.
+        if (g.slot.count(id) == 0) {
+            keep.push_back(id);
+            continue;
+        }
+        r.steps_done += g.quantum;
+        s->particle_steps += r.spec.count * g.quantum;
         if (!r.finishing) {
             keep.push_back(id);
             continue;
@@ -257,85 +340,111 @@ void retire(orr_server* s, Group& g) {
         r.completed = s->now();
         s->resident -= r.bytes;
         ++s->completed;
+        if (s->large_active == id) s->large_active = 0;
         g.dirty = true;
     }
     g.members = std::move(keep);
 }
 
+// The group's next quantum, enqueued on its batch's streams.
+void launch_quantum(orr_server* s, Group& g) {
+    std::uint64_t remaining = ~std::uint64_t{0};
+    for (std::uint64_t id : g.members) {
+        const Request& r = s->requests.at(id);
+        remaining = std::min(remaining, r.spec.steps - r.steps_done);
+    }
+    std::uint64_t q = 1;  // first quantum of a group: one step, to measure it
+    if (g.ms_per_step > 0) {
+        q = static_cast<std::uint64_t>(std::max(1.0, std::floor(s->cfg.round_ms / g.ms_per_step)));
+    }
+    g.quantum = std::min(q, remaining);
+    if (g.start == nullptr) ORR_HIP(hipEventCreate(&g.start));
+    if (g.stop == nullptr) ORR_HIP(hipEventCreate(&g.stop));
+    if (g.done == nullptr) ORR_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    hipStream_t st = orr::batch_stream(g.batch);
+    ORR_HIP(hipEventRecord(g.start, st));
+    orr::batch_step_enqueue(g.batch, g.quantum);
+    ORR_HIP(hipEventRecord(g.stop, st));
+    // Members this quantum finishes: end diagnostics and final state behind it.
+    for (std::uint64_t id : g.members) {
+        Request& r = s->requests.at(id);
+        if (r.spec.steps - r.steps_done != g.quantum) continue;
+        const std::uint32_t k = g.slot.at(id);
+        orr::batch_diagnostics_enqueue(g.batch, k, r.diag + 8);
+        orr::batch_download_enqueue(g.batch, k, r.result);
+        r.finishing = true;
+    }
+    ORR_HIP(hipEventRecord(g.done, st));
+    g.in_flight = true;
+}
+
 void round(orr_server* s) {
-    const double start = s->now();
-    // Groups that lost members last round still hold their memory in their
-    // old batches: rebuild them before admitting into the budget they freed.
+    const double t = s->now();
+    if (s->was_busy) s->busy += t - s->last_round;
+    bool progressed = false;
+    // 1. Quanta that finished.
+    for (auto& [key, g] : s->groups) {
+        if (!g.in_flight) continue;
+        const hipError_t e = hipEventQuery(g.done);
+        if (e == hipErrorNotReady) continue;
+        ORR_HIP(e);
+        retire(s, g);
+        progressed = true;
+    }
+    // Groups that lost members still hold their memory in their old batches:
+    // rebuild the idle ones before admitting into the budget they freed.
     if (!s->queue.empty()) {
         for (auto& [key, g] : s->groups) {
-            if (g.dirty) rebuild(s, key, g);
+            if (g.dirty && !g.in_flight) rebuild(s, key, g);
         }
     }
+    // 2. Admission.
+    const std::size_t queued = s->queue.size();
     admit(s);
+    progressed = progressed || s->queue.size() != queued;
+    // 3. Idle groups: rebuild where membership changed, then the next quantum.
     for (auto& [key, g] : s->groups) {
-        if (g.dirty) rebuild(s, key, g);
-    }
-    std::vector<Group*> running;
-    const double target = s->cfg.round_ms;
-    for (auto& [key, g] : s->groups) {
+        if (g.in_flight) continue;
+        if (g.dirty) {
+            rebuild(s, key, g);
+            progressed = true;
+        }
         if (g.members.empty()) continue;
-        std::uint64_t remaining = ~std::uint64_t{0};
-        for (std::uint64_t id : g.members) {
-            const Request& r = s->requests.at(id);
-            remaining = std::min(remaining, r.spec.steps - r.steps_done);
-        }
-        std::uint64_t q = 1;  // first round of a group: one step, to measure it
-        if (g.ms_per_step > 0) {
-            q = static_cast<std::uint64_t>(std::max(1.0, std::floor(target / g.ms_per_step)));
-        }
-        g.quantum = std::min(q, remaining);
-        if (g.start == nullptr) ORR_HIP(hipEventCreate(&g.start));
-        if (g.stop == nullptr) ORR_HIP(hipEventCreate(&g.stop));
-        ORR_HIP(hipEventRecord(g.start, orr::batch_stream(g.batch)));
-        orr::batch_step_enqueue(g.batch, g.quantum);
-        ORR_HIP(hipEventRecord(g.stop, orr::batch_stream(g.batch)));
-        // Members this quantum finishes: end diagnostics and final state behind it.
-        for (std::uint64_t id : g.members) {
-            Request& r = s->requests.at(id);
-            if (r.spec.steps - r.steps_done != g.quantum) continue;
-            const std::uint32_t k = g.slot.at(id);
-            orr::batch_diagnostics_enqueue(g.batch, k, r.diag + 8);
-            orr::batch_download_enqueue(g.batch, k, r.result);
-            r.finishing = true;
-        }
-        running.push_back(&g);
-    }
-    for (Group* g : running) orr::batch_wait(g->batch);
-    for (Group* g : running) {
-        float ms = 0.f;
-        ORR_HIP(hipEventElapsedTime(&ms, g->start, g->stop));
-        g->ms_per_step = static_cast<double>(ms) / static_cast<double>(g->quantum);
-        for (std::uint64_t id : g->members) {
-            Request& r = s->requests.at(id);
-            r.steps_done += g->quantum;
-            s->particle_steps += r.spec.count * g->quantum;
-        }
-        retire(s, *g);
+        launch_quantum(s, g);
+        progressed = true;
     }
     for (auto it = s->groups.begin(); it != s->groups.end();) {
         Group& g = it->second;
-        if (g.members.empty()) {
+        if (g.members.empty() && !g.in_flight) {
             if (g.batch != nullptr) call(orr_batch_destroy(g.batch));
+            g.batch = nullptr;
+            give_back_lanes(s, g);
             release_events(g);
             it = s->groups.erase(it);
         } else {
             ++it;
         }
     }
+    bool busy = false;
+    for (const auto& [key, g] : s->groups) busy = busy || g.in_flight;
+    // 4. Nothing to do until a quantum finishes: yield the host thread briefly.
+    if (!progressed && busy) std::this_thread::sleep_for(std::chrono::microseconds(100));
     ++s->rounds;
-    s->busy += s->now() - start;
+    s->last_round = s->now();
+    if (s->was_busy || busy) s->busy += s->last_round - t;
+    s->was_busy = busy;
 }
 
 void destroy(orr_server* s) noexcept {
     if (s == nullptr) return;
     for (auto& [key, g] : s->groups) {
         if (g.batch != nullptr) orr_batch_destroy(g.batch);
+        for (orr_ctx* c : {g.lanes.d, g.lanes.t}) orr::free_context(c);
         release_events(g);
+    }
+    for (const Lanes& l : s->spare) {
+        orr::free_context(l.d);
+        orr::free_context(l.t);
     }
     for (auto& [id, r] : s->requests) release(r);
     delete s;
@@ -441,9 +550,12 @@ int orr_server_status(orr_server* s, uint64_t id, orr_request_status* out) {
         out->admitted_s = r.admitted;
         out->completed_s = r.completed;
         out->charged_bytes = r.bytes;
-        // Start diagnostics exist once a round has run since admission, end
+        // Start diagnostics exist once the request's first quantum has been
+        // retired (they run ahead of it on its batch's stream), end
         // diagnostics once the request is done.
-        if (r.diag != nullptr && r.state != kQueued) std::memcpy(out->diagnostics_start, r.diag, 8 * sizeof(double));
+        if (r.diag != nullptr && (r.steps_done > 0 || r.state == kDone)) {
+            std::memcpy(out->diagnostics_start, r.diag, 8 * sizeof(double));
+        }
         if (r.diag != nullptr && r.state == kDone) std::memcpy(out->diagnostics_end, r.diag + 8, 8 * sizeof(double));
     });
 }
