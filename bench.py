@@ -17,7 +17,8 @@ direct_complement, the pairs inside each block of 8 tiles) from HIP events
 recorded on the engine's own stream.
 
 Further legs in the same JSON line:
-* ``direct_n65536`` -- the same direct stepping at N = 65,536;
+* ``direct_n65536``, ``direct_n524288`` -- the same direct stepping at
+  N = 65,536 and 524,288 (the latter with its error on sampled targets);
 * ``plummer4096_vv`` -- the plain CPU path's run (Plummer N = 4096, eps 0.01,
   velocity Verlet dt = 1e-3, 100 steps): time per step and energy drift on the
   host's reference kernel, beside the device-resident single-system path;
@@ -347,15 +348,22 @@ def direct_accuracy(E, W, ctx, cpu, seeds=(1, 2, 3)) -> list:
     return rows
 
 
-def direct_rate(E, W, ctx, n: int, steps: int, seed: int = 1) -> dict:
-    """Config 2's other size: the same Yoshida-4 stepping of a device-resident
-    Plummer sphere, timed over `steps` steps after one warm-up step."""
+def direct_rate(E, W, ctx, n: int, steps: int, seed: int = 1, cpu=None, targets: int = 1024) -> dict:
+    """The same Yoshida-4 stepping of a device-resident Plummer sphere at
+    another N, timed over `steps` steps after one warm-up step; with the CPU
+    leg, the first evaluation's error on sampled targets against FP64."""
     eps, dt = 0.005, 5e-4
     state = W.plummer(n, seed)
     s = E.DeviceSystem(ctx, n)
     s.upload(*state.arrays())
     cfg = E.StepConfig(integrator="yoshida4", solver="direct", softening=eps)
     s.refresh(cfg)
+    error = None
+    if cpu is not None:
+        a = s.download()
+        rng = np.random.default_rng(seed + 100)
+        tgt = np.sort(rng.choice(n, size=min(n, targets), replace=False))
+        error = cpu.fp64_errors(state, eps, (a["ax"], a["ay"], a["az"]), tgt)
     s.step(cfg, dt, 1)
     ctx.set_timing(True)
     ctx.reset_timings()
@@ -368,10 +376,12 @@ def direct_rate(E, W, ctx, n: int, steps: int, seed: int = 1) -> dict:
     evals = 3 * steps
     k_ms, k_launches = timings.get("direct_kernel", (0.0, 0))
     kernel_ms = k_ms / max(1, k_launches)
+    fold_ms, fold_launches = timings.get("direct_reduce", (0.0, 0))
     return {"n": n, "steps": steps, "interactions_per_s": evals * n * (n - 1) / elapsed,
             "ms_per_evaluation": elapsed * 1e3 / evals, "kernel_ms_per_evaluation": kernel_ms,
+            "fold_ms_per_evaluation": fold_ms / max(1, fold_launches),
             "kernel_frac_of_fp32_peak": FLOP_PER_INTERACTION * n * (n - 1) / (kernel_ms * 1e-3) / 1e12
-            / FP32_PEAK_TFLOPS}
+            / FP32_PEAK_TFLOPS, "force_error": error}
 
 
 def bench_plummer_vv(E, W, ctx, cpu) -> dict:
@@ -812,6 +822,8 @@ def main() -> int:
     ap.add_argument("--no-direct", action="store_true", help="skip the headline direct leg")
     ap.add_argument("--n65536-steps", type=int, default=20,
                     help="Yoshida-4 steps of the N = 65,536 direct leg (0 skips it)")
+    ap.add_argument("--n524288-steps", type=int, default=2,
+                    help="Yoshida-4 steps of the N = 524,288 direct leg (0 skips it)")
     ap.add_argument("--walk-targets", type=int, default=0,
                     help="targets per lane of the Barnes-Hut walk (1 or 2; 0: the engine's default)")
     ap.add_argument("--probes-only", action="store_true",
@@ -923,6 +935,9 @@ def main() -> int:
         if args.n65536_steps > 0:
             log("[bench] direct at N = 65,536")
             result["direct_n65536"] = direct_rate(E, W, ctx, 65536, args.n65536_steps)
+        if args.n524288_steps > 0:
+            log("[bench] direct at N = 524,288")
+            result["direct_n524288"] = direct_rate(E, W, ctx, 524288, args.n524288_steps, cpu=cpu)
         if cpu is not None:
             log("[bench] CPU leg: FP64 reference forces and the AVX2 baseline")
             rng = np.random.default_rng(12345)

@@ -127,13 +127,14 @@ ORR_API int orr_direct_static_limits(uint64_t* max_count, uint64_t* scratch_byte
 /* Launch configuration of the direct kernel (no reference counterpart: the
  * CUDA solver fixes its block size, cuda_direct_solver.cpp:46-63).
  * targets_per_lane in {1, 2, 4}; refine = 1 applies a Newton step to the
- * hardware reciprocal square root (off by default: v_rsq_f32 already keeps the
- * force within 1e-6 relative); block_target is the number of thread blocks the
+ * hardware reciprocal square root of the cubed argument (off by default: the
+ * pair terms' FP32 rounding, the close pairs' double-float terms and the
+ * compensated sums keep the force within 1e-6 relative without it); block_target is the number of thread blocks the
  * source range is split to reach (default 32768). pair_mode 1 (default)
  * evaluates each far pair of 64-particle tiles once and applies it to both
  * tiles where that is faster and fits: from 704 tiles (44,993 particles) up to
- * 428,032 particles (3 doubles per particle and later block of 8 tiles within
- * 4 GiB); pair_mode 3 does so wherever it fits (from 4 blocks of 8 tiles,
+ * 856,064 particles (3 doubles per particle and later block of 8 tiles within
+ * 16 GiB); pair_mode 3 does so wherever it fits (from 4 blocks of 8 tiles,
  * 1,537 particles); pair_mode 2 always runs the one-sided kernel, to which
  * targets_per_lane and block_target apply. equal_masses 1 (default) lets the
  * pairs path take its equal-mass form when every particle has one mass (the

@@ -122,6 +122,14 @@ orr_ctx* make_context(int device) {
     return ctx.release();
 }
 
+void ensure_aux(orr_ctx* ctx) {
+    if (ctx->aux != nullptr) return;
+    // Default priority, as the main stream.
+    ORR_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    ORR_HIP(hipEventCreateWithFlags(&ctx->aux_fork, hipEventDisableTiming));
+    ORR_HIP(hipEventCreateWithFlags(&ctx->aux_join, hipEventDisableTiming));
+}
+
 void free_context(orr_ctx* ctx) noexcept {
     if (ctx == nullptr) return;
     (void)hipSetDevice(ctx->device);
@@ -134,6 +142,12 @@ void free_context(orr_ctx* ctx) noexcept {
         (void)hipEventDestroy(p.stop);
     }
     for (hipEvent_t e : ctx->free_events) (void)hipEventDestroy(e);
+    if (ctx->aux != nullptr) {
+        (void)hipStreamSynchronize(ctx->aux);
+        (void)hipStreamDestroy(ctx->aux);
+    }
+    if (ctx->aux_fork != nullptr) (void)hipEventDestroy(ctx->aux_fork);
+    if (ctx->aux_join != nullptr) (void)hipEventDestroy(ctx->aux_join);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -30,7 +30,7 @@
 //   * By default (direct_pairs, below) each far pair of tiles is evaluated
 //     once and applied to both tiles, with the same packing, near test and
 //     compensation; the one-sided kernel above then covers what the pairs path
-//     does not take (configured off, too few tiles, or slots beyond 4 GiB).
+//     does not take (configured off, too few tiles, or slots beyond 16 GiB).
 //
 // The reference counts 20 flop per interaction (docs/performance.md:103-109).
 #include <hip/hip_runtime.h>
@@ -1102,7 +1102,7 @@ constexpr std::uint32_t kMaskTiles = 32768;
 // The pairs path: blocks of kStay tiles, chunks of S tiles per item, slot
 // counts. Taken when the mask exists, there are at least four blocks (pair
 // mode 1: at least kPairMinTiles tiles) and the streamed-side slots (3 doubles
-// per particle and later block) fit in 4 GiB; otherwise, or when configured
+// per particle and later block) fit in 16 GiB; otherwise, or when configured
 // off, the one-sided kernel runs.
 struct PairPlan {
     bool use = false;
@@ -1111,7 +1111,11 @@ struct PairPlan {
     std::size_t jpart_floats = 0, ipart_doubles = 0;
 };
 
-constexpr std::size_t kPairSlotBytes = std::size_t{1} << 32;
+// The streamed-side slots grow as N^2 (one per tile and later block), and so
+// does the kernel's work: their write and fold cost a fixed few per cent of an
+// evaluation at any N. 16 GiB of the 288 GB HBM takes the pairs path to
+// 856,064 particles.
+constexpr std::size_t kPairSlotBytes = std::size_t{1} << 34;
 // Items wanted per evaluation: some sixteen rounds of the 4096 resident waves.
 // Short items balance the items that carry near passes (measured at
 // N = 262,144: 16384 items 14.8 ms, 32768 14.0, 65536 13.7, 131072 13.5 ms,
@@ -1146,6 +1150,10 @@ PairPlan pair_plan(std::uint32_t tiles, bool masked, int mode) {
     return p;
 }
 
+// The complement (one wave per tile: too few waves to fill the device below a
+// few hundred thousand particles, and latency-bound there) runs on the
+// context's second stream, beside the pairs kernel; the main stream waits for
+// it before the fold.
 template<bool kSafe, bool kRefine>
 void launch_pairs(orr_ctx* ctx, const PairPlan& plan, const float4* packed, const float4* packed_lo,
                   const double4* centres, const float4* extents, const std::uint32_t* mask,
@@ -1153,6 +1161,14 @@ void launch_pairs(orr_ctx* ctx, const PairPlan& plan, const float4* packed, cons
                   float* jpart, double* ipart, float* cpart) {
     constexpr std::uint32_t kWaves = kBlock / kTile;
     const std::uint32_t items = static_cast<std::uint32_t>(plan.items);
+    ensure_aux(ctx);
+    ORR_HIP(hipEventRecord(ctx->aux_fork, ctx->stream));
+    ORR_HIP(hipStreamWaitEvent(ctx->aux, ctx->aux_fork, 0));
+    hipLaunchKernelGGL((direct_complement<kSafe, kRefine>), dim3((tiles + kWaves - 1) / kWaves),
+                       dim3(kBlock), 0, ctx->aux, packed, packed_lo, centres, extents, mask, tiles,
+                       words, eps2, cpart);
+    ORR_LAUNCHED("direct_complement");
+    ORR_HIP(hipEventRecord(ctx->aux_join, ctx->aux));
     hipLaunchKernelGGL((direct_pairs<kSafe, kRefine, true>), dim3((items + kWaves - 1) / kWaves),
                        dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, mask, tiles, words, plan.blocks,
                        plan.S, items, eps2, uniform, jpart, ipart);
@@ -1161,10 +1177,7 @@ void launch_pairs(orr_ctx* ctx, const PairPlan& plan, const float4* packed, cons
                        dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, mask, tiles, words, plan.blocks,
                        plan.S, items, eps2, uniform, jpart, ipart);
     ORR_LAUNCHED("direct_pairs");
-    hipLaunchKernelGGL((direct_complement<kSafe, kRefine>), dim3((tiles + kWaves - 1) / kWaves),
-                       dim3(kBlock), 0, ctx->stream, packed, packed_lo, centres, extents, mask, tiles,
-                       words, eps2, cpart);
-    ORR_LAUNCHED("direct_complement");
+    ORR_HIP(hipStreamWaitEvent(ctx->stream, ctx->aux_join, 0));
 }
 
 }  // namespace

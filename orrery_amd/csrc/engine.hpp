@@ -103,6 +103,11 @@ enum Phase : int {
 struct orr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // A second stream (default priority, made on first use) for work that can
+    // overlap the main stream's within one evaluation, forked and joined by
+    // the two events.
+    hipStream_t aux = nullptr;
+    hipEvent_t aux_fork = nullptr, aux_join = nullptr;
     int compute_units = 0;
 
     // Scratch, grown on demand.
@@ -177,6 +182,10 @@ namespace orr {
 inline constexpr double kCubeRootOfTwo = 1.2599210498948731647672106;
 inline constexpr double kOuterWeight = 1.0 / (2 - kCubeRootOfTwo);
 inline constexpr double kInnerWeight = -kCubeRootOfTwo / (2 - kCubeRootOfTwo);
+
+// context.hip: the context's second stream and its fork/join events, made on
+// first use.
+void ensure_aux(orr_ctx* ctx);
 
 // context.hip: a context with its own stream (default priority); freeing
 // synchronises it and releases every buffer.

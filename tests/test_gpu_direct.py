@@ -316,10 +316,10 @@ def test_pair_mode_with_newton_step(ctx):
 
 
 def test_pair_mode_beyond_slot_budget_is_one_sided(ctx):
-    # Above 428,032 particles the streamed-side slots would pass 4 GiB and pair
-    # mode 1 runs the one-sided kernel: bit for bit pair mode 2's result, and
-    # within the direct bounds of the FP64 sum on sampled targets.
-    n, eps = 428032 + 64, 0.005
+    # Above 856,064 particles the streamed-side slots would pass 16 GiB and
+    # pair mode 1 runs the one-sided kernel: bit for bit pair mode 2's result,
+    # and within the direct bounds of the FP64 sum on sampled targets.
+    n, eps = 856064 + 64, 0.005
     p = oracle.plummer(n, 14)
     out = {}
     try:
@@ -339,10 +339,10 @@ def test_pair_mode_beyond_slot_budget_is_one_sided(ctx):
 
 
 def test_pair_mode_at_largest_size(ctx):
-    # 428,032 particles: the largest system whose streamed-side slots fit the
-    # pairs path's 4 GiB (6,688 tiles, 836 blocks; slot offsets past 2^32
-    # bytes). Every particle outside the last block sums slots up to the last.
-    n, eps = 428032, 0.005
+    # 856,064 particles: the largest system whose streamed-side slots fit the
+    # pairs path's 16 GiB (13,376 tiles, 1,672 blocks; slot offsets past 2^32
+    # doubles). Every particle outside the last block sums slots up to the last.
+    n, eps = 856064, 0.005
     p = oracle.plummer(n, 15)
     (ax, ay, az), _ = evaluate(ctx, (p.x, p.y, p.z), p.m, eps=eps)
     rng = np.random.default_rng(15)
