@@ -1,116 +1,73 @@
-// Close pairs in double-float FP32 arithmetic.
+// Close pairs in FP64.
 //
 // Where two particles are much closer than their tiles are wide, their one
 // term can be many times the net force on either particle (a bound pair near
 // the centre of a cluster). An FP32 term carries a relative error of a few
-// 1e-8 from rounding the separation alone, and that error, scaled by the term
-// over the net force, is what the worst particles' error is made of. These
-// helpers evaluate such a term with every quantity held as an unevaluated sum
-// hi + lo of two floats (Dekker's and Knuth's error-free transformations:
-// TwoSum, TwoProd by FMA), to about 1e-12 relative:
+// 1e-8 from its own roundings, and that error, scaled by the term over the net
+// force, is what the worst particles' error is made of. These pairs are
+// evaluated in FP64 instead, from the positions the kernels hold as (hi, lo)
+// float pairs relative to one tile centre:
 //
-//   D  = b - a                        from the two positions' (hi, lo) parts,
-//   R  = |D|^2 + eps^2                summed with TwoSum, squares by TwoProd,
-//   y0 = rsq(R_hi)                    the hardware reciprocal square root,
-//   e  = 1 - R y0^2                   the residual, computed exactly enough,
-//   G  = y0^3 (1 + 3e/2)              one Newton step on y0^3 (O(e^2) left),
-//   T  = m G D                        products by TwoProd.
+//   D  = (b_hi + b_lo) - (a_hi + a_lo)   each sum exact in FP64, one rounding,
+//   R  = |D|^2 + eps^2                   by FMA,
+//   y  = 1 / sqrt(R)                     the hardware FP64 reciprocal square
+//                                        root and two Newton steps,
+//   G  = y^3 D                           the term per unit mass, about 1e-15
+//                                        relative.
 //
-// Host and device: the host build uses 1/sqrt(x) for the seed (a CPU test
-// checks the transformation against long-double arithmetic).
+// FP64 instructions issue at the rate of unpacked FP32 ones on this GPU, so
+// the term costs some 30 of them: less than an exact double-float FP32
+// evaluation of the same term, and more accurate. The caller scales G by the
+// source's (or target's) mass and adds it to its FP32 sums as (hi, lo).
+//
+// Host and device: the host build seeds the reciprocal square root with
+// 1/sqrt(x) (a CPU test checks the term against long-double arithmetic).
 #pragma once
 
 #include <cmath>
 
 #ifndef __HIPCC__
 #define ORR_HD
-#define ORR_FMAF(a, b, c) std::fma(a, b, c)
 #else
 #define ORR_HD __host__ __device__
-#define ORR_FMAF(a, b, c) __builtin_fmaf(a, b, c)
 #endif
 
 namespace orr {
 
-struct DF {
-    float hi, lo;
-};
-
-ORR_HD inline void two_sum(float a, float b, float& s, float& e) {
-#pragma clang fp contract(off)
-    s = a + b;
-    const float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
-ORR_HD inline void two_prod(float a, float b, float& p, float& e) {
-#pragma clang fp contract(off)
-    p = a * b;
-    e = ORR_FMAF(a, b, -p);
-}
-
-ORR_HD inline float seed_rsq(float x) {
+ORR_HD inline double seed_rsq64(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_rsqf(x);
+    return __builtin_amdgcn_rsq(x);
 #else
-    return 1.0f / std::sqrt(x);
+    return 1.0 / std::sqrt(x);
 #endif
 }
 
-// Separation D = b - a of two positions given as (hi, lo) float pairs in one
-// frame, and G = (|D|^2 + eps2)^(-3/2), both as double-floats.
+// G = (|D|^2 + eps2)^(-3/2) D for the separation D = b - a of two positions
+// given as (hi, lo) float pairs in one frame. R > 0 is the caller's condition
+// (a close pair is never a coincident, unsoftened one).
 ORR_HD inline void close_pair(const float ah[3], const float al[3], const float bh[3],
-                              const float bl[3], float eps2, DF D[3], DF& G) {
+                              const float bl[3], float eps2, double G[3]) {
 #pragma clang fp contract(off)
-    float rh = eps2, rl = 0.f;
+    double d[3];
     for (int c = 0; c < 3; ++c) {
-        // (bh + bl) - (ah + al) as a double-float: both differences exact with
-        // their error terms, then renormalised twice. For the closest pairs the
-        // lo parts' difference is as large as the hi parts', so it is not
-        // rounded on its own.
-        float s, e, t, f;
-        two_sum(bh[c], -ah[c], s, e);
-        two_sum(bl[c], -al[c], t, f);
-        two_sum(s, e + t, s, e);
-        two_sum(s, e + f, D[c].hi, D[c].lo);
-        float p, pe;
-        two_prod(D[c].hi, D[c].hi, p, pe);
-        float q, qe;
-        two_sum(rh, p, q, qe);
-        rh = q;
-        rl = rl + (qe + (pe + 2.f * D[c].hi * D[c].lo));
+        d[c] = (static_cast<double>(bh[c]) + static_cast<double>(bl[c])) -
+               (static_cast<double>(ah[c]) + static_cast<double>(al[c]));
     }
-    {
-        const float s = rh + rl;
-        rl = rl - (s - rh);
-        rh = s;
+    const double r2 = __builtin_fma(d[2], d[2], __builtin_fma(d[1], d[1], __builtin_fma(d[0], d[0], static_cast<double>(eps2))));
+    const double h = 0.5 * r2;
+    double y = seed_rsq64(r2);
+    for (int k = 0; k < 2; ++k) {
+        const double e = __builtin_fma(-h, y * y, 0.5);
+        y = __builtin_fma(y, e, y);
     }
-    const float y0 = seed_rsq(rh);
-    float y2, y2e;
-    two_prod(y0, y0, y2, y2e);
-    float p, pe;
-    two_prod(rh, y2, p, pe);
-    // p is within a few ulps of 1, so 1 - p is exact.
-    const float e = (1.f - p) - (pe + (rh * y2e + rl * y2));
-    float c3, c3e;
-    two_prod(y2, y0, c3, c3e);
-    c3e = ORR_FMAF(y2e, y0, c3e);
-    G.hi = c3;
-    G.lo = c3e + c3 * (1.5f * e);
+    const double g = (y * y) * y;
+    for (int c = 0; c < 3; ++c) G[c] = g * d[c];
 }
 
-// T = m G D as double-floats (m exact in FP32).
-ORR_HD inline void close_term(float m, const DF& G, const DF D[3], DF T[3]) {
-#pragma clang fp contract(off)
-    float fh, fe;
-    two_prod(m, G.hi, fh, fe);
-    const float fl = fe + m * G.lo;
-    for (int c = 0; c < 3; ++c) {
-        float th, te;
-        two_prod(fh, D[c].hi, th, te);
-        T[c].hi = th;
-        T[c].lo = te + (fh * D[c].lo + fl * D[c].hi);
-    }
+// v split into floats hi + lo (hi the rounding of v, lo that of the rest).
+ORR_HD inline void split(double v, float& hi, float& lo) {
+    hi = static_cast<float>(v);
+    lo = static_cast<float>(v - static_cast<double>(hi));
 }
 
 }  // namespace orr

@@ -204,7 +204,7 @@ __device__ __forceinline__ bool near_tiles(const double4& C, const float4& E, do
 }
 
 // Close pairs: closer than kClose times the wider tile's largest half-extent.
-// Their terms are evaluated in double-float arithmetic (close_pair.hpp); about
+// Their terms are evaluated in FP64 (close_pair.hpp); about
 // one step in twenty of a near pass has such a pair on some lane.
 constexpr float kClose = 0.25f;
 
@@ -227,9 +227,8 @@ __device__ __forceinline__ void kahan_df(float h, float l, float& s, float& c) {
 //     centres, rounded once); the terms are summed plainly in FP32 in runs of
 //     kRun and each run is Kahan-added per component into (s, k).
 //   * Near tiles: double-float separations, runs of kNearRun terms summed in
-//     FP32 and Kahan-added into the tile's own pair, close pairs in
-//     double-float arithmetic added to it directly; the tile's pair then
-//     joins (s, k).
+//     FP32 and Kahan-added into the tile's own pair, close pairs evaluated in
+//     FP64 and added to it as (hi, lo); the tile's pair then joins (s, k).
 //
 // kUnroll sets how many far-path sources are in flight (their SGPR loads
 // included).
@@ -321,7 +320,7 @@ __device__ __forceinline__ void source_tile(
                 r2 = __builtin_fmaf(dy, dy, r2);
                 r2 = __builtin_fmaf(dz, dz, r2);
                 // A close pair's FP32 term is left out here and added in
-                // double-float below.
+                // FP64 below.
                 const bool near_pair = r2 < lim[r] && r2 > 0.f;
                 close[r] |= near_pair ? std::uint64_t{1} << j : std::uint64_t{0};
                 const float f = near_pair ? 0.f : q.w * inv_cubed<kSafe, kRefine>(r2);
@@ -347,12 +346,16 @@ __device__ __forceinline__ void source_tile(
             const float4 q = src[j];
             const float4 ql = src_lo[j];
             const float bh[3] = {q.x, q.y, q.z}, bl[3] = {ql.x, ql.y, ql.z};
-            DF D[3], G, T[3];
-            close_pair(th[r], tl[r], bh, bl, eps2, D, G);
-            close_term(q.w, G, D, T);
-            kahan_df(T[0].hi, T[0].lo, px[r], pkx[r]);
-            kahan_df(T[1].hi, T[1].lo, py[r], pky[r]);
-            kahan_df(T[2].hi, T[2].lo, pz[r], pkz[r]);
+            double G[3];
+            close_pair(th[r], tl[r], bh, bl, eps2, G);
+            const double mq = q.w;
+            float h, l;
+            split(mq * G[0], h, l);
+            kahan_df(h, l, px[r], pkx[r]);
+            split(mq * G[1], h, l);
+            kahan_df(h, l, py[r], pky[r]);
+            split(mq * G[2], h, l);
+            kahan_df(h, l, pz[r], pkz[r]);
         }
     }
 #pragma unroll
@@ -848,7 +851,7 @@ __device__ __forceinline__ void pairs_item(const float4* __restrict__ packed,
             // tile's centre in FP64 and split into (hi, lo)); both sides sum
             // runs of kFlush terms, the stationary runs Kahan-added, the
             // streamed runs added to the FP64 slots. Close pairs are marked
-            // and evaluated afterwards in double-float arithmetic.
+            // and evaluated afterwards in FP64.
             const float4 il = packed_lo[static_cast<std::size_t>(I) * kTile + lane];
             float xl = il.x, yl = il.y, zl = il.z;
 #pragma unroll
@@ -921,16 +924,20 @@ __device__ __forceinline__ void pairs_item(const float4* __restrict__ packed,
                     const float4 b = packed[k];
                     const float4 bl4 = packed_lo[k];
                     const float bh[3] = {b.x, b.y, b.z}, bl[3] = {bl4.x, bl4.y, bl4.z};
-                    DF D[3], G, T[3];
-                    close_pair(th, tl, bh, bl, eps2, D, G);
-                    close_term(kEqual ? 1.f : b.w, G, D, T);
-                    stay_add(r, 0, T[0].hi, T[0].lo, sx[r]);
-                    stay_add(r, 1, T[1].hi, T[1].lo, sy[r]);
-                    stay_add(r, 2, T[2].hi, T[2].lo, sz[r]);
-                    close_term(mr, G, D, T);
-                    atomicAdd(&home[p], -(static_cast<double>(T[0].hi) + T[0].lo));
-                    atomicAdd(&home[kTile + p], -(static_cast<double>(T[1].hi) + T[1].lo));
-                    atomicAdd(&home[2 * kTile + p], -(static_cast<double>(T[2].hi) + T[2].lo));
+                    double G[3];
+                    close_pair(th, tl, bh, bl, eps2, G);
+                    const double mb = kEqual ? 1.0 : static_cast<double>(b.w);
+                    const double ms = static_cast<double>(mr);
+                    float h, l;
+                    split(mb * G[0], h, l);
+                    stay_add(r, 0, h, l, sx[r]);
+                    split(mb * G[1], h, l);
+                    stay_add(r, 1, h, l, sy[r]);
+                    split(mb * G[2], h, l);
+                    stay_add(r, 2, h, l, sz[r]);
+                    atomicAdd(&home[p], -(ms * G[0]));
+                    atomicAdd(&home[kTile + p], -(ms * G[1]));
+                    atomicAdd(&home[2 * kTile + p], -(ms * G[2]));
                 }
             }
         }

@@ -1,5 +1,5 @@
-// The double-float close-pair term (orrery_amd/csrc/close_pair.hpp) against
-// long-double arithmetic: separations from 1e-7 to 1 of the positions'
+// The FP64 close-pair term of the direct kernels (orrery_amd/csrc/close_pair.hpp)
+// against long-double arithmetic: separations from 1e-7 to 1 of the positions'
 // magnitude, softenings from 0 to larger than the separation.
 #include <cmath>
 #include <cstdio>
@@ -37,15 +37,17 @@ int main() {
             r2 += d[c] * d[c];
         }
         if (!(r2 > 0)) continue;
-        const float m = 0.75f;
+        const double m = 0.75;
         const long double g = 1.0L / (r2 * std::sqrt(r2));
-        orr::DF D[3], G, T[3];
-        orr::close_pair(ah, al, bh, bl, eps2, D, G);
-        orr::close_term(m, G, D, T);
+        double G[3];
+        orr::close_pair(ah, al, bh, bl, eps2, G);
         long double err = 0, mag = 0, err32 = 0;
         for (int c = 0; c < 3; ++c) {
             const long double exact = m * g * d[c];
-            const long double got = static_cast<long double>(T[c].hi) + T[c].lo;
+            // As the kernels add it: m G split into two floats.
+            float hi, lo;
+            orr::split(m * G[c], hi, lo);
+            const long double got = static_cast<long double>(hi) + lo;
             err += (got - exact) * (got - exact);
             mag += exact * exact;
         }
@@ -57,7 +59,7 @@ int main() {
             rr = std::fma(dd[1], dd[1], rr);
             rr = std::fma(dd[2], dd[2], rr);
             const float inv = 1.0f / std::sqrt(rr);
-            const float f = (m * inv) * (inv * inv);
+            const float f = (static_cast<float>(m) * inv) * (inv * inv);
             for (int c = 0; c < 3; ++c) {
                 const long double e = static_cast<long double>(f * dd[c]) - m * g * d[c];
                 err32 += e * e;

@@ -1,10 +1,12 @@
-"""The double-float close-pair term of the direct kernels, on the CPU.
+"""The FP64 close-pair term of the direct kernels, on the CPU.
 
-orrery_amd/csrc/close_pair.hpp is plain FP32 arithmetic (error-free TwoSum /
-TwoProd transformations around one reciprocal square root seed and a Newton
-step), so the host compiler runs the same steps as the device; the harness
-tests/cpp/test_close_pair.cpp checks them against long-double arithmetic on
-200,000 pairs whose separations reach down to the positions' own rounding.
+orrery_amd/csrc/close_pair.hpp is plain FP64 arithmetic (the separation from
+the positions' (hi, lo) float parts, a reciprocal square root seed and two
+Newton steps), so the host compiler runs the same steps as the device from a
+correctly rounded seed; the harness tests/cpp/test_close_pair.cpp checks the
+term, split into two floats as the kernels add it, against long-double
+arithmetic on 200,000 pairs whose separations reach down to the positions' own
+rounding.
 """
 import os
 import subprocess
