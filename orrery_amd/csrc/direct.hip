@@ -854,10 +854,16 @@ __device__ __forceinline__ void pairs_item(const float4* __restrict__ packed,
             // and evaluated afterwards in FP64.
             const float4 il = packed_lo[static_cast<std::size_t>(I) * kTile + lane];
             float xl = il.x, yl = il.y, zl = il.z;
+            // The stationary tiles' centres and offsets are read again here
+            // (cache hits), from an index hidden afresh: otherwise the
+            // compiler keeps the far pass's FP64 centre differences alive
+            // through the far loop, and they spill on every streamed tile.
+            std::uint32_t again = t0;
+            asm volatile("" : "+s"(again));
 #pragma unroll
             for (int r = 0; r < kStay; ++r) {
                 if (((near >> r) & 1u) == 0) continue;
-                const std::uint32_t tile = min(first + r, tiles - 1);
+                const std::uint32_t tile = min(again + r, tiles - 1);
                 const double4 C = centres[tile];
                 const float4 o = packed[static_cast<std::size_t>(tile) * kTile + lane];
                 const float4 l = packed_lo[static_cast<std::size_t>(tile) * kTile + lane];
