@@ -14,7 +14,8 @@ compensated accumulation) and the FP64 kick/drift updates around them.
 evaluation, inputs resident in HBM. ``roofline`` is for the dominant launches
 (direct_pairs, which evaluates each far pair of tiles once for both tiles, and
 direct_complement, the pairs inside each block of 8 tiles) from HIP events
-recorded on the engine's own stream.
+recorded on the engine's own stream (direct_complement runs on a second
+stream beside direct_pairs; the events bracket both).
 
 Further legs in the same JSON line:
 * ``direct_n65536``, ``direct_n524288`` -- the same direct stepping at
@@ -922,8 +923,9 @@ def main() -> int:
                                            "TFLOPs": FLOP_PER_INTERACTION * mix / 1e12,
                                            "frac": achieved * 1e12 / FLOP_PER_INTERACTION / mix},
                          "note": "20 flop per interaction x N(N-1) per evaluation (the reference's "
-                                 "convention) / mean time of the evaluation's direct_pairs + "
-                                 "direct_complement launches; direct_pairs evaluates each far pair "
+                                 "convention) / mean time per evaluation from the fork of "
+                                 "direct_complement (second stream) to the join after direct_pairs "
+                                 "(HIP events on the engine's stream); direct_pairs evaluates each far pair "
                                  "of tiles once for both tiles (16 VALU + 1 v_rsq_f32 per pair), so "
                                  "this counts interactions delivered, about twice the pair "
                                  "evaluations executed; issue_ceiling = the engine's mix probe, the "
