@@ -13,24 +13,26 @@
 //   * A wave reads a source tile through the scalar unit: the address is
 //     wave-uniform, so each {ox, oy, oz, m} arrives in SGPRs and is broadcast
 //     to all 64 lanes as an instruction operand. No LDS, no barriers.
-//   * Each lane owns R targets. Per target and source tile the FP64
-//     centre-to-centre shift is rounded once to FP32; far tiles use the
-//     separation (o_j - (o_i - shift)), 13 VALU operations per interaction.
+//   * Each lane owns R targets. Per target and source tile the target is
+//     re-expressed against the source tile's centre as one FP64 difference
+//     rounded once; far tiles use the separation o_j - t_hi (one rounding of
+//     the separation), 12 VALU operations and one v_rsq_f32 of (r^2+eps^2)^3
+//     per interaction.
 //   * Near tiles -- boxes whose gap is at most a tenth of their summed
-//     half-widths, the only place where a separation can be much smaller than
-//     the offsets (about 2% of tile pairs at N = 262,144) -- use a
-//     double-float separation (o_j - t) + (o_lo_j - t_lo), exact to FP32
-//     rounding of the separation itself, at 19 operations per interaction.
-//     Every pairwise term therefore has O(FP32 epsilon) relative error.
-//   * The 64 terms of one source tile are summed plainly in FP32 and the tile
-//     partial is Kahan-added into the running sum (compensation at tile
-//     granularity).
+//     half-widths (the mask: one bit per tile pair, written once per
+//     evaluation) -- use the double-float separation (o_j - t) + (o_lo_j -
+//     t_lo). Pairs among them closer than a quarter of the wider tile's
+//     half-width are marked and evaluated afterwards in FP64 (close_pair.hpp):
+//     their single term can be many times the net force on either particle.
+//   * Far tiles' terms are summed in FP32 in runs of 16, near tiles' in runs
+//     of 8, each run Kahan-added into the target's running sum.
 //   * The source range is split over gridDim.y; a fixed-order FP64 reduction
 //     kernel folds the splits, multiplies by G and scatters to particle order.
 //   * By default (direct_pairs, below) each far pair of tiles is evaluated
-//     once and applied to both tiles, with the same packing, near test and
-//     compensation; the one-sided kernel above then covers what the pairs path
-//     does not take (configured off, too few tiles, or slots beyond 16 GiB).
+//     once and applied to both tiles, with the same packing, near test, close
+//     pairs and compensation; the one-sided kernel above then covers what the
+//     pairs path does not take (configured off, too few tiles, or slots beyond
+//     16 GiB).
 //
 // The reference counts 20 flop per interaction (docs/performance.md:103-109).
 #include <hip/hip_runtime.h>
@@ -643,11 +645,15 @@ __global__ __launch_bounds__(256) void batch_reduce(const float* __restrict__ pa
 //     one-sided kernel's double-float separation. The pairs inside a block are
 //     left to the complement kernel, one-sided.
 //   * Stationary side: the 64 terms of one streamed tile summed in FP32, then
-//     Kahan-added, per item written to its own slot. Streamed side: per step
-//     the kStay terms summed, then added to the travelling accumulator (64
-//     steps), written per (particle, block) to its own FP64 slot (the near
-//     passes' sums are Kahan-added into it). A fixed-order FP64 fold
-//     (pairs_reduce) sums a particle's slots.
+//     Kahan-added (compensations in LDS), per item written to its own slot.
+//     Streamed side: per step the kStay terms summed, then added to the
+//     travelling accumulator, which is flushed every kFlush steps into the
+//     particle's FP64 sum in LDS; after the block's 64 steps (and the near
+//     passes, which flush the same way) the sums are written per (particle,
+//     block) to their own FP64 slots. A fixed-order FP64 fold (pairs_reduce)
+//     sums a particle's slots.
+//   * Close pairs of the near passes are evaluated in FP64 (close_pair.hpp)
+//     and applied to both sides.
 //
 // Items: block Q >= 1 streams tiles [0, Q kStay) in chunks of S tiles, S a
 // divisor of kStay (kStay Q / S chunks) or a multiple of it (s = S / kStay:

@@ -177,3 +177,32 @@ def test_failed_rebuild_keeps_requests_and_is_retried(ctx):
             assert np.array_equal(got[key], ref[key]), (w.n, integ, key)
     assert s.stats()["resident_bytes"] == 0
     s.close()
+
+
+def test_large_requests_run_one_at_a_time_in_arrival_order(ctx):
+    # Requests of alone_from particles or more are admitted one at a time, in
+    # arrival order: the second large request is admitted only after the first
+    # has finished, while a small request submitted after both is admitted at
+    # once and runs beside the first. Each result still equals the request
+    # served alone.
+    reqs = [
+        (W.plummer(9000, 41), "velocity-verlet", 30),
+        (W.disc_collision(10000, 42), "yoshida4", 12),
+        (W.plummer(1500, 43), "velocity-verlet", 5),
+    ]
+    s = E.Server(ctx, config())
+    ids = [s.submit(w, integ, EPS, 1e-3, steps) for w, integ, steps in reqs]
+    s.round()
+    st = [s.status(i) for i in ids]
+    assert [x["state"] for x in st] == ["running", "queued", "running"]
+    s.drain()
+    st = [s.status(i) for i in ids]
+    assert all(x["state"] == "done" for x in st)
+    assert st[1]["admitted"] >= st[0]["completed"]
+    assert st[2]["admitted"] < st[0]["completed"]
+    got = [s.result(i) for i in ids]
+    s.close()
+    for (w, integ, steps), g in zip(reqs, got):
+        alone, _ = run_alone(ctx, w, integ, steps)
+        for key in ("x", "y", "z", "vx", "vy", "vz"):
+            assert np.array_equal(g[key], alone[key]), (w.n, integ, key)
