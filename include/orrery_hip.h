@@ -468,11 +468,13 @@ ORR_API int orr_server_destroy(orr_server* server);
 /* Seconds since the server was created (the clock of the status times). */
 ORR_API int orr_server_now(orr_server* server, double* seconds);
 ORR_API int orr_server_submit(orr_server* server, const orr_request* request, uint64_t* id);
-/* One pass of the loop, which never waits for the device: retire the quanta
- * that finished (and the requests they finished), admit, (re)batch, and start
- * the next quantum on every batch that has none in flight; sleeps 0.1 ms when
- * there was nothing to do. Call it repeatedly (submitting between calls).
- * `pending` gets the number of requests queued or running after it. */
+/* One pass of the loop: retire the quanta that finished (and the requests
+ * they finished), admit, (re)batch, and start the next quantum on every batch
+ * that has none in flight; sleeps 0.1 ms when there was nothing to do. It
+ * never waits for a quantum in flight; rebuilding a batch (its copies) and
+ * capturing a new batch's step wait for that batch's own streams only. Call it
+ * repeatedly (submitting between calls). `pending` gets the number of
+ * requests queued or running after it. */
 ORR_API int orr_server_round(orr_server* server, uint32_t* pending);
 ORR_API int orr_server_status(orr_server* server, uint64_t id, orr_request_status* out);
 /* Final state of a finished request (host arrays, null: skipped); then its
