@@ -82,6 +82,24 @@ def build_host(verbose: bool = False) -> None:
         print(f"built {HOST_LIBRARY} and {HOST_TEST}")
 
 
+REFERENCE = os.environ.get("ORRERY_REFERENCE", "/root/reference")
+REFERENCE_CLI = os.path.join(HERE, "host", "_cli", "orrery")
+
+
+def build_reference_cli(verbose: bool = False) -> None:
+    """The reference's own simulator with reference.patch applied and the
+    engine linked in (host/reference_cli.mk); only where the reference checkout
+    exists, since it is compiled from the reference's sources."""
+    if not os.path.isdir(os.path.join(REFERENCE, "src", "sim")):
+        if verbose:
+            print("reference checkout absent: reference simulator not built")
+        return
+    _run(["make", "-s", "-f", os.path.join(HERE, "host", "reference_cli.mk"), "-j8",
+          f"REFERENCE={REFERENCE}"], "reference simulator")
+    if verbose:
+        print(f"built {REFERENCE_CLI}")
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=8) as pool:
@@ -95,6 +113,7 @@ def build(verbose: bool = False) -> str:
     if verbose:
         print(f"built {LIBRARY}")
     build_host(verbose)
+    build_reference_cli(verbose)
     return LIBRARY
 
 
