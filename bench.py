@@ -425,16 +425,18 @@ def bench_plummer_vv(E, W, ctx, cpu) -> dict:
 
 
 def bench_barnes_hut(E, W, ctx, n: int, evals: int, error_targets: int, cpu) -> dict:
-    """Barnes-Hut on the two-disc collision: N particles, theta 0.5, leaf 16."""
+    """Barnes-Hut on the two-disc collision: N particles, theta 0.5, leaf 16; the
+    same settings at 262,144 particles as the cross-check size."""
     eps, theta, cap = 0.01, 0.5, 16
     log(f"[bench] Barnes-Hut: generating two-disc collision N={n}")
     p = W.disc_collision(n, 1)
-    d = [E.DeviceArray.from_host(ctx, a) for a in (p.x, p.y, p.z, p.m)]
-    out = [E.DeviceArray(ctx, n) for _ in range(3)]
-    rng = np.random.default_rng(2)
-    tgt = np.sort(rng.choice(n, size=min(n, error_targets), replace=False))
 
-    def run(quadrupole: bool):
+    def run(p, quadrupole: bool):
+        n = p.n
+        d = [E.DeviceArray.from_host(ctx, a) for a in (p.x, p.y, p.z, p.m)]
+        out = [E.DeviceArray(ctx, n) for _ in range(3)]
+        rng = np.random.default_rng(2)
+        tgt = np.sort(rng.choice(n, size=min(n, error_targets), replace=False))
         s = E.HipTreeSolver(E.TreeParameters(theta, cap, quadrupole=quadrupole), E.Softening(eps), context=ctx)
         # Warm-up: the first evaluation sizes the buffers, the second is
         # captured as a hipGraph; the timed ones replay it.
@@ -455,10 +457,12 @@ def bench_barnes_hut(E, W, ctx, n: int, evals: int, error_targets: int, cpu) -> 
         ctx.set_timing(False)
         acc = [a.to_host() for a in out]
         err = cpu.fp64_errors(p, eps, acc, tgt) if cpu is not None else None
+        for a in d + out:
+            a.release()
         return s, wall, phases, err
 
-    mono_s, mono_wall, mono_phases, mono_err = run(False)
-    s, wall, phases, err = run(True)
+    mono_s, mono_wall, mono_phases, mono_err = run(p, False)
+    s, wall, phases, err = run(p, True)
     c = s.interaction_count()
     tree = s.tree_summary()
     M = tree["node_count"]
@@ -492,14 +496,18 @@ def bench_barnes_hut(E, W, ctx, n: int, evals: int, error_targets: int, cpu) -> 
                                      "evaluation's wall time less the eager walk time (within the "
                                      "walk's run-to-run spread of about 0.05 ms)"},
     }
-    for a in d + out:
-        a.release()
+    small = W.disc_collision(262144, 1) if n != 262144 else p
+    if n != 262144:
+        # The cross-check size with the headline's settings (quadrupole).
+        log("[bench] Barnes-Hut: N=262,144 cross-check")
+        _, x_wall, x_phases, x_err = run(small, True)
+        out_d["n262144"] = {"ms_per_evaluation": x_wall * 1e3, "phases_ms": x_phases,
+                            "force_error": x_err}
     if cpu is not None:
         # The GPU headline's settings (quadrupole, theta 0.5, leaf 16) on the
         # same arrays; at 2M one evaluation takes seconds on the host, so that
         # size runs one warm-up and three timed trials (stated in "protocol").
         log("[bench] CPU: Barnes-Hut N=262,144 and the GPU leg's arrays")
-        small = W.disc_collision(262144, 1) if n != 262144 else p
         out_d["cpu_baseline"] = {
             "n262144": cpu.barnes_hut_baseline(small, eps, theta, cap, True),
             f"n{n}": cpu.barnes_hut_baseline(p, eps, theta, cap, True, trials=3, warmup=1,
