@@ -54,6 +54,7 @@ def _compile(src: str) -> str:
 ROOT = os.path.dirname(HERE)
 HOST_LIBRARY = os.path.join(HERE, "liborrery_host.so")
 HOST_TEST = os.path.join(ROOT, "tests", "cpp", "test_host_solvers")
+HOST_TEST_ASAN = HOST_TEST + "_asan"
 CXX = os.environ.get("CXX", "g++")
 HOST_INCLUDES = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host", "include"),
                  "-I", os.path.join(HERE, "host", "compat")]
@@ -78,8 +79,17 @@ def build_host(verbose: bool = False) -> None:
           os.path.join(ROOT, "tests", "cpp", "test_host_solvers.cpp"), "-L", HERE, "-lorrery_host",
           "-lorrery_hip", "-Wl,-rpath,$ORIGIN/../../orrery_amd", "-o", tmp], "host test driver")
     os.replace(tmp, HOST_TEST)
+    # The same driver and hosts under AddressSanitizer and UndefinedBehaviorSanitizer
+    # (host code only; the engine library it links is the ordinary build).
+    tmp = HOST_TEST_ASAN + ".tmp"
+    _run([CXX, "-std=c++20", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+          "-fno-sanitize-recover=undefined", "-Wall", *HOST_INCLUDES,
+          os.path.join(HERE, "host", "hip_solvers.cpp"), os.path.join(HERE, "host", "hip_assembly.cpp"),
+          os.path.join(ROOT, "tests", "cpp", "test_host_solvers.cpp"), "-L", HERE, "-lorrery_hip",
+          "-Wl,-rpath,$ORIGIN/../../orrery_amd", "-o", tmp], "sanitized host test driver")
+    os.replace(tmp, HOST_TEST_ASAN)
     if verbose:
-        print(f"built {HOST_LIBRARY} and {HOST_TEST}")
+        print(f"built {HOST_LIBRARY}, {HOST_TEST} and {HOST_TEST_ASAN}")
 
 
 REFERENCE = os.environ.get("ORRERY_REFERENCE", "/root/reference")

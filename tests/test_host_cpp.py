@@ -4,7 +4,8 @@
   drop-in claim), and without a device try_create returns null
   (cuda_direct_solver_test.cpp:132-136).
 * GPU: tests/cpp/test_host_solvers runs the reference's closed-form and
-  agreement cases through HipDirectSolver / HipTreeSolver.
+  agreement cases through HipDirectSolver / HipTreeSolver, once as built and
+  once built with AddressSanitizer and UndefinedBehaviorSanitizer.
 """
 import os
 import subprocess
@@ -41,3 +42,20 @@ def test_host_solvers_on_device():
     res = subprocess.run([BINARY], capture_output=True, text=True, timeout=120)
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
+
+
+@pytest.mark.gpu
+def test_host_solvers_on_device_under_asan_and_ubsan():
+    # The driver and hosts built with AddressSanitizer and UndefinedBehaviorSanitizer
+    # (host code only). Leaks inside the ROCm runtime's own libraries are
+    # suppressed by library name (tests/cpp/lsan_rocm.supp); any other leak,
+    # bad access or undefined behaviour fails the run.
+    binary = BINARY + "_asan"
+    assert os.path.exists(binary), "build first: python -m orrery_amd.build"
+    supp = os.path.join(ROOT, "tests", "cpp", "lsan_rocm.supp")
+    env = {**os.environ, "LSAN_OPTIONS": f"suppressions={supp}"}
+    res = subprocess.run([binary], capture_output=True, text=True, timeout=300, env=env)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "all host-solver checks passed" in res.stdout
+    assert "ERROR: AddressSanitizer" not in res.stderr and "runtime error" not in res.stderr, res.stderr
