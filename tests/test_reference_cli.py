@@ -37,7 +37,7 @@ BINARY = os.path.join(ROOT, "orrery_amd", "host", "_cli", "orrery")
 PATCH = os.path.join(ROOT, "orrery_amd", "host", "reference.patch")
 REFERENCE = "/root/reference"
 PATCHED_FILES = ["include/orrery/sim/configuration.hpp", "src/sim/configuration.cpp",
-                 "src/sim/assembly.cpp", "src/solvers/CMakeLists.txt"]
+                 "src/sim/assembly.cpp", "src/solvers/CMakeLists.txt", "python/src/sim_bindings.cpp"]
 
 needs_binary = pytest.mark.skipif(not os.path.exists(BINARY),
                                   reason="reference simulator not built (needs the reference checkout)")
