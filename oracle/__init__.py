@@ -135,12 +135,21 @@ def _empty(n):
     return Particles(*(np.zeros(n) for _ in range(7)))
 
 
-def plummer(n, seed, total_mass=1.0, scale_radius=1.0, cutoff=None, lib=None):
-    """Plummer sphere; cutoff defaults to truncation at r < 10 a."""
+def plummer(n, seed, total_mass=1.0, scale_radius=1.0, cutoff=None, lib=None,
+            velocity_direction_first=False):
+    """Plummer sphere; cutoff defaults to truncation at r < 10 a.
+    velocity_direction_first: the draw order of a GCC build of plummer.cpp:122
+    (the oracle's restatement only)."""
     if cutoff is None:
         cutoff = (100.0 / 101.0) ** 1.5
     p = _empty(n)
     lib = lib or load()
+    if velocity_direction_first:
+        rc = lib.orc_plummer_ordered(_u64(n), ctypes.c_double(total_mass), ctypes.c_double(scale_radius),
+                                     ctypes.c_double(cutoff), _u64(seed), ctypes.c_int(1),
+                                     *(_ptr(a) for a in p.arrays()))
+        assert rc == 0
+        return p
     fn = lib.orc_plummer if hasattr(lib, "orc_plummer") else lib.ref_plummer
     rc = fn(_u64(n), ctypes.c_double(total_mass), ctypes.c_double(scale_radius),
             ctypes.c_double(cutoff), _u64(seed), *(_ptr(a) for a in p.arrays()))

@@ -787,6 +787,12 @@ int orc_random_uniform(uint64_t seed, uint64_t count, double* out) {
 
 int orc_plummer(uint64_t count, double M, double a, double cutoff, uint64_t seed, double* x,
                 double* y, double* z, double* vx, double* vy, double* vz, double* m) {
+    return orc_plummer_ordered(count, M, a, cutoff, seed, 0, x, y, z, vx, vy, vz, m);
+}
+
+int orc_plummer_ordered(uint64_t count, double M, double a, double cutoff, uint64_t seed,
+                        int velocity_direction_first, double* x, double* y, double* z, double* vx,
+                        double* vy, double* vz, double* m) {
     if (!(M > 0) || !(a > 0) || !(cutoff > 0) || !(cutoff < 1)) return 1;
     if (count == 0) return 0;
     Random rnd(seed);
@@ -804,8 +810,17 @@ int orc_plummer(uint64_t count, double M, double a, double cutoff, uint64_t seed
         }
         const double vesc = std::sqrt(2 * 1.0 * M / std::sqrt((r * r) + (a * a)));
         const double speed = q * vesc;
-        const V3<double> pd = rnd.unit_vector();  // position direction first
-        const V3<double> vd = rnd.unit_vector();
+        // plummer.cpp:122 draws both directions inside one argument list:
+        // left to right (Clang) the position's first, right to left (GCC) the
+        // velocity's.
+        V3<double> pd, vd;
+        if (velocity_direction_first != 0) {
+            vd = rnd.unit_vector();
+            pd = rnd.unit_vector();
+        } else {
+            pd = rnd.unit_vector();
+            vd = rnd.unit_vector();
+        }
         p.add(scale(r, pd), scale(speed, vd), pm);
     }
     move_to_centre_of_mass_frame(p);

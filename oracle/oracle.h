@@ -32,6 +32,13 @@ int orc_random_uniform(uint64_t seed, uint64_t count, double* out);
 int orc_plummer(uint64_t count, double total_mass, double scale_radius,
                 double mass_fraction_cutoff, uint64_t seed, double* x, double* y,
                 double* z, double* vx, double* vy, double* vz, double* m);
+/* The same with the two directions of plummer.cpp:122 drawn in either order:
+ * velocity_direction_first = 1 is a GCC build's right-to-left evaluation, the
+ * order the reference's docs/performance/tree_accuracy.csv was sampled in. */
+int orc_plummer_ordered(uint64_t count, double total_mass, double scale_radius,
+                        double mass_fraction_cutoff, uint64_t seed, int velocity_direction_first,
+                        double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                        double* m);
 
 /* initial_conditions/uniform_sphere.cpp:32-61 (zero velocities, COM frame). */
 int orc_uniform_sphere(uint64_t count, double total_mass, double radius, uint64_t seed,
