@@ -1,4 +1,8 @@
-"""The reference's Barnes-Hut accuracy curve (tests/golden/tree_accuracy.csv,
+"""The reference's published Barnes-Hut tables: the accuracy curve
+(tests/golden/tree_accuracy.csv) and the scaling table
+(tests/golden/tree_scaling.csv), both measured on the same configurations.
+
+The accuracy curve (tests/golden/tree_accuracy.csv,
 a copy of docs/performance/tree_accuracy.csv as written by
 benchmarks/tree_scaling.cpp:227-243, 150-186).
 
@@ -78,3 +82,47 @@ def test_device_tree_follows_the_curve(ctx):
         # own error is smallest and the FP32 terms' share largest.
         assert abs(rms - float(row["rms_error"])) <= 3e-3 * float(row["rms_error"]), (rms, row)
         assert abs(worst - float(row["worst_error"])) <= 1e-3 * float(row["worst_error"]), (worst, row)
+
+
+# The reference's tree scaling table (tests/golden/tree_scaling.csv, a copy of
+# docs/performance/tree_scaling.csv from benchmarks/tree_scaling.cpp:188-226):
+# the same sampler and softening at N = 1024 ... 262,144, theta 0.5, leaf 32;
+# nodes, leaves and depth of the tree and the counts of one evaluation.
+SCALING = list(csv.DictReader(open(os.path.join(HERE, "golden", "tree_scaling.csv"))))
+
+
+def scaled_sample(n):
+    return oracle.plummer(n, SEED, scale_radius=3 * np.pi / 16, cutoff=0.999, velocity_direction_first=True)
+
+
+def test_oracle_reproduces_the_scaling_table():
+    for row in SCALING:
+        n = int(row["particles"])
+        p = scaled_sample(n)
+        codes, index, origin, size = oracle.morton_order(p.x, p.y, p.z)
+        i = index.astype(np.int64)
+        t = oracle.tree_build(p.x[i], p.y[i], p.z[i], p.m[i], codes, origin, size, 0.5, LEAF)
+        assert (len(t.mass), t.leaves, t.depth) == (int(row["nodes"]), int(row["leaves"]), int(row["depth"])), row
+        *_, counts = oracle.barnes_hut(p.x, p.y, p.z, p.m, EPS * EPS, 0.5, LEAF, kernel=1, threads=8)
+        assert counts == (int(row["particle_particle"]), int(row["particle_cell"])), row
+
+
+@pytest.mark.gpu
+def test_device_tree_matches_the_scaling_table(ctx):
+    import orrery_amd as E
+    for row in SCALING:
+        n = int(row["particles"])
+        p = scaled_sample(n)
+        s = E.HipTreeSolver(E.TreeParameters(0.5, LEAF), E.Softening(EPS), context=ctx)
+        acc = [np.zeros(n) for _ in range(3)]
+        s.evaluate((p.x, p.y, p.z), p.m, acc)
+        tree = s.tree_summary()
+        c = s.interaction_count()
+        fpp, fpc = int(row["particle_particle"]), int(row["particle_cell"])
+        print(f"N={n}: nodes {tree['node_count']} leaves {tree['leaf_count']} depth {tree['depth']}, "
+              f"counts {c.particle_particle} {c.particle_cell} ({fpp} {fpc})")
+        assert (tree["node_count"], tree["leaf_count"], tree["depth"]) == \
+            (int(row["nodes"]), int(row["leaves"]), int(row["depth"])), row
+        mirror = oracle.barnes_hut_fp32_counts(p.x, p.y, p.z, p.m, 0.5, LEAF, threads=8)
+        assert (c.particle_particle, c.particle_cell) == tuple(mirror), row
+        assert abs(c.particle_particle - fpp) <= 1e-6 * fpp and abs(c.particle_cell - fpc) <= 1e-6 * fpc, row
