@@ -185,19 +185,22 @@ typedef struct {
     uint64_t leaf_count;
     uint32_t depth;
     uint32_t reserved;
-    uint64_t wave_steps;     /* walk iterations summed over waves (64 lanes each) */
+    uint64_t wave_steps;     /* walk lane-target slots / 64, summed over waves */
 } orr_tree_stats;
 ORR_API int orr_tree_eval(orr_ctx* ctx, const orr_tree_args* args, orr_tree_stats* stats);
 
 /* Walk configuration (no reference counterpart; cuda_tree_solver.cpp fixes its
  * traversal): targets_per_lane 1 or 2 -- how many targets each lane of a
  * wave walks together -- or -1 to choose by size (2 for one system of 2^20 or
- * more particles, else 1; the default). Results are bit-identical either way;
- * only the speed differs. Zero keeps the current value; the getter reports
- * -1 while the choice is by size. */
+ * more particles, else 1; the default); lanes 64 or 32 -- how many lanes of a
+ * wave hold targets -- or -1 to choose by size (32 for a launch of at most
+ * 2048 full waves, two per SIMD, else 64; the default). Results are
+ * bit-identical either way; only the speed differs. Zero keeps the current
+ * value; the getter reports -1 while a choice is by size. */
 typedef struct {
     int32_t targets_per_lane;
-    int32_t reserved[3];
+    int32_t lanes;
+    int32_t reserved[2];
 } orr_tree_config;
 ORR_API int orr_ctx_set_tree_config(orr_ctx* ctx, const orr_tree_config* config);
 ORR_API int orr_ctx_get_tree_config(orr_ctx* ctx, orr_tree_config* config);

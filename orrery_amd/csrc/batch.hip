@@ -605,6 +605,7 @@ void batch_create(orr_ctx* ctx, const orr_batch_system* systems, uint32_t count,
         }
         // The owner's kernel settings carry over to the batch's private contexts.
         b->tctx->walk_targets_per_lane = ctx->walk_targets_per_lane;
+        b->tctx->walk_lanes = ctx->walk_lanes;
         b->dctx->direct_refine = ctx->direct_refine;
         ORR_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
         ORR_HIP(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));

@@ -295,8 +295,12 @@ int orr_ctx_set_tree_config(orr_ctx* ctx, const orr_tree_config* config) {
         require(ctx != nullptr && config != nullptr, "null argument");
         const int r = config->targets_per_lane;
         require(r >= -1 && r <= 2, "targets_per_lane must be 1, 2, 0 (keep) or -1 (by size)");
+        const int l = config->lanes;
+        require(l == -1 || l == 0 || l == 32 || l == 64, "lanes must be 32, 64, 0 (keep) or -1 (by size)");
         if (r == -1) ctx->walk_targets_per_lane = 0;
         if (r > 0) ctx->walk_targets_per_lane = r;
+        if (l == -1) ctx->walk_lanes = 0;
+        if (l > 0) ctx->walk_lanes = l;
     });
 }
 
@@ -306,6 +310,7 @@ int orr_ctx_get_tree_config(orr_ctx* ctx, orr_tree_config* config) {
         *config = orr_tree_config{};
         // -1 while the choice is by size, as orr_ctx_set_tree_config spells it.
         config->targets_per_lane = ctx->walk_targets_per_lane != 0 ? ctx->walk_targets_per_lane : -1;
+        config->lanes = ctx->walk_lanes != 0 ? ctx->walk_lanes : -1;
     });
 }
 

@@ -134,6 +134,7 @@ struct orr_ctx {
     int direct_equal_masses = 1;
     // Barnes-Hut walk configuration (orr_ctx_set_tree_config); 0 chooses by size.
     int walk_targets_per_lane = 0;
+    int walk_lanes = 0;  // active lanes per walk wave: 32, 64, or 0 = by size
 
     // Host mirror of the last tree summary and the shape of its device layout.
     orr_tree_stats last_tree{};

@@ -186,6 +186,7 @@ Lanes take_lanes(orr_server* s) {
     }
     // The server context's kernel settings carry over.
     l.t->walk_targets_per_lane = s->ctx->walk_targets_per_lane;
+    l.t->walk_lanes = s->ctx->walk_lanes;
     l.d->direct_refine = s->ctx->direct_refine;
     return l;
 }

@@ -42,6 +42,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #include "engine.hpp"
@@ -558,6 +559,13 @@ __device__ __forceinline__ void kahan(float p, float& s, float& c) {
 // system (one contiguous eighth each would leave the XCD with the densest part
 // finishing last). A bijection of [0, blocks): blocks past the last whole
 // round of runs keep their own index.
+// Up to this many full (64-lane) walk waves -- two per SIMD of the 256 CUs --
+// half-lane waves are faster (quadrupole walk, two-disc collision: 1.088 ->
+// 0.981 ms at N = 65,536, 1.328 -> 1.265 ms at 131,072); above it they are
+// slower (1.813 -> 2.194 ms at 262,144): the walk is bound by issue, not by
+// latency, once each SIMD has a few waves.
+constexpr std::uint64_t kHalfLaneWaves = 2 * 4 * 256;
+
 constexpr std::uint32_t kXcds = 8;
 constexpr std::uint32_t kXcdRun = 16;
 __device__ __forceinline__ std::uint32_t xcd_block(std::uint32_t b, std::uint32_t blocks) {
@@ -671,12 +679,14 @@ __device__ __forceinline__ void leaf_pairs(WalkTarget& t, const double4& cb, dou
     t.px = t.py = t.pz = 0.f;
 }
 
-// Wave-coherent walk. Each lane owns kR targets (wave w owns the 64 kR sorted
-// particles from 64 kR w, target r of lane l being 64 r + l); the wave visits
+// Wave-coherent walk. Each of the first kLanes lanes owns kR targets (wave w
+// owns the kLanes kR sorted particles from kLanes kR w, target r of lane l
+// being kLanes r + l; with kLanes = 32 the other half of the wave holds none,
+// which doubles the waves a small system gets); the wave visits
 // the smallest node any of its targets needs, loads it once, and every target
 // that needs it takes its step. A target therefore sees exactly the node
 // sequence and arithmetic of a walk of its own, whatever the other targets are.
-template<int kR, bool kSafe, bool kQuad>
+template<int kR, bool kSafe, bool kQuad, int kLanes>
 __global__ __launch_bounds__(kBlock) void tree_walk(
     const WalkNode* __restrict__ nodes, const QuadF* __restrict__ quads,
     const double4* __restrict__ node_geo,
@@ -700,8 +710,8 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
     __shared__ double s_own[kR][3][kBlock];
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
-        const std::uint32_t i = wave * (64u * kR) + 64u * r + lane;
-        valid[r] = i < n;
+        const std::uint32_t i = wave * (kLanes * kR) + kLanes * r + lane;
+        valid[r] = lane < kLanes && i < n;
         const std::uint32_t ic = valid[r] ? i : n - 1;
         t[r].xi = root_rel[ic];
         t[r].oi = leaf_rel[ic];
@@ -793,7 +803,7 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
         kahan(t[r].py, t[r].sy, t[r].ky);
         kahan(t[r].pz, t[r].sz, t[r].kz);
         if (valid[r]) {
-            const std::uint32_t dst = index[wave * (64u * kR) + 64u * r + lane];
+            const std::uint32_t dst = index[wave * (kLanes * kR) + kLanes * r + lane];
             ax[dst] = G * (static_cast<double>(t[r].sx) - static_cast<double>(t[r].kx));
             ay[dst] = G * (static_cast<double>(t[r].sy) - static_cast<double>(t[r].ky));
             az[dst] = G * (static_cast<double>(t[r].sz) - static_cast<double>(t[r].kz));
@@ -815,7 +825,8 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
             mine_slot[0] = wpp;
             mine_slot[1] = wpc;
             mine_slot[2] = wvisits;
-            mine_slot[3] = static_cast<unsigned long long>(steps) * kR;  // lane-target slots: 64 per count
+            // Lane-target slots, 64 per count.
+            mine_slot[3] = static_cast<unsigned long long>(steps) * kR * kLanes / 64;
         }
     } else {
 #pragma unroll
@@ -829,7 +840,7 @@ __global__ __launch_bounds__(kBlock) void tree_walk(
         }
         if (lane == 0) {
             atomicAdd(&stats[4 * static_cast<std::size_t>(seg0) + 3],
-                      static_cast<unsigned long long>(steps) * kR);
+                      static_cast<unsigned long long>(steps) * kR * kLanes / 64);
             mine_slot[0] = mine_slot[1] = mine_slot[2] = mine_slot[3] = 0;
         }
     }
@@ -915,7 +926,7 @@ std::size_t layout_bytes(std::uint64_t n, std::size_t nodes, bool quadrupole, st
     const std::size_t per_node = nodes * (4 * 3 + 1 + 4 * kMaxChildren + 32 + 32 + sizeof(WalkNode) + per_quad);
     const std::size_t counters = 16 + 8 * 4 * static_cast<std::size_t>(segments) +
                                  8 * ((n + kBlock - 1) / kBlock) +
-                                 32 * (kBlock / 64) * ((n + kBlock - 1) / kBlock) + 512;
+                                 32 * (kBlock / 64) * ((2 * n + kBlock - 1) / kBlock) + 512;
     return per_particle + per_node + counters + 22 * 256 + 64;
 }
 
@@ -949,7 +960,8 @@ Layout layout(orr_ctx* ctx, std::uint64_t n, std::size_t nodes, bool quadrupole,
     L.c32 = carve<std::uint32_t>(p, 4);
     L.c64 = carve<unsigned long long>(p, 4 * static_cast<std::size_t>(segments));
     L.block_counts = carve<uint2>(p, (n + kBlock - 1) / kBlock);
-    L.wave_stats = carve<unsigned long long>(p, 4 * (kBlock / 64) * ((n + kBlock - 1) / kBlock));
+    // Half-lane walks (kLanes = 32) launch twice the waves of full ones.
+    L.wave_stats = carve<unsigned long long>(p, 4 * (kBlock / 64) * ((2 * n + kBlock - 1) / kBlock));
     return L;
 }
 
@@ -1052,16 +1064,28 @@ void tree_enqueue(orr_ctx* ctx, const orr_tree_args& a, const SegmentView& segs,
         // N = 2^21); zero in the context means choose by that rule.
         const int R = ctx->walk_targets_per_lane != 0 ? ctx->walk_targets_per_lane
                       : (segs.count == 1 && n >= (std::uint64_t{1} << 20) ? 2 : 1);
-        using Walk = decltype(&tree_walk<1, true, true>);
-        Walk walk;
-        if (R == 2) {
-            walk = safe ? (quad ? tree_walk<2, true, true> : tree_walk<2, true, false>)
-                        : (quad ? tree_walk<2, false, true> : tree_walk<2, false, false>);
-        } else {
-            walk = safe ? (quad ? tree_walk<1, true, true> : tree_walk<1, true, false>)
-                        : (quad ? tree_walk<1, false, true> : tree_walk<1, false, false>);
-        }
-        const std::uint32_t per_block = kBlock * static_cast<std::uint32_t>(R);
+        // Half the lanes of each wave for a launch of at most kHalfLaneWaves
+        // full waves (twice the waves, each over a tighter group of targets);
+        // zero in the context means choose by that rule. A target's walk and
+        // sums do not depend on which targets share its wave, so the results
+        // are the same bits either way.
+        const std::uint64_t full_waves = (n + 64u * R - 1) / (64u * R);
+        const int lanes = ctx->walk_lanes != 0 ? ctx->walk_lanes
+                          : (full_waves <= kHalfLaneWaves ? 32 : 64);
+        using Walk = decltype(&tree_walk<1, true, true, 64>);
+        auto pick = [&](auto r, auto l) -> Walk {
+            constexpr int kRr = decltype(r)::value, kLl = decltype(l)::value;
+            return safe ? (quad ? tree_walk<kRr, true, true, kLl> : tree_walk<kRr, true, false, kLl>)
+                        : (quad ? tree_walk<kRr, false, true, kLl> : tree_walk<kRr, false, false, kLl>);
+        };
+        using One = std::integral_constant<int, 1>;
+        using Two = std::integral_constant<int, 2>;
+        using Half = std::integral_constant<int, 32>;
+        using Full = std::integral_constant<int, 64>;
+        const Walk walk = R == 2 ? (lanes == 32 ? pick(Two{}, Half{}) : pick(Two{}, Full{}))
+                                 : (lanes == 32 ? pick(One{}, Half{}) : pick(One{}, Full{}));
+        const std::uint32_t per_wave = static_cast<std::uint32_t>(lanes * R);
+        const std::uint32_t per_block = (kBlock / 64) * per_wave;
         const std::uint32_t walk_blocks = (n32 + per_block - 1) / per_block;
         hipLaunchKernelGGL(walk, dim3(walk_blocks), dim3(kBlock), 0, ctx->stream, L.walk, L.quadf,
                            L.geo, L.root_rel, L.leaf_rel, L.leaf_of, index, L.c32, seg_of,
@@ -1069,7 +1093,7 @@ void tree_enqueue(orr_ctx* ctx, const orr_tree_args& a, const SegmentView& segs,
                            a.ay, a.az, counts64, L.wave_stats);
         ORR_LAUNCHED("tree_walk");
         hipLaunchKernelGGL(walk_counts, dim3(segs.count), dim3(kBlock), 0, ctx->stream,
-                           L.wave_stats, walk_blocks * (kBlock / 64), 64u * R, seg_start, counts64);
+                           L.wave_stats, walk_blocks * (kBlock / 64), per_wave, seg_start, counts64);
         ORR_LAUNCHED("walk_counts");
     }
 }
@@ -1132,6 +1156,7 @@ void tree_evaluate(orr_ctx* ctx, const orr_tree_args& a, orr_tree_stats* stats) 
     key_add(key, a.leaf_capacity);
     key_add(key, a.quadrupole);
     key_add(key, ctx->walk_targets_per_lane);
+    key_add(key, ctx->walk_lanes);
     run_captured(ctx, ctx->tree_graph, std::move(key),
                  [&] { tree_enqueue(ctx, a, SegmentView{}, nullptr); });
     if (stats != nullptr) tree_read_stats(ctx, nullptr, stats);

@@ -274,17 +274,20 @@ def test_device_resident_tree_stepping(ctx):
 
 
 @pytest.mark.parametrize("quadrupole", [False, True])
-def test_two_targets_per_lane_is_bitwise(ctx, quadrupole):
-    # A wave walking 128 targets (two per lane) visits each target's nodes in the
-    # same DFS order with the same arithmetic as a wave of 64.
+def test_wave_shapes_are_bitwise(ctx, quadrupole):
+    # A wave walking 128 targets (two per lane), 64 or 32 (half its lanes)
+    # visits each target's nodes in the same DFS order with the same arithmetic;
+    # counts and node visits are the same too. Also a batch of two systems
+    # (the segmented walk) under each shape.
     p = oracle.disc_collision(30000, 4)
+    q = oracle.plummer(7000, 9)
     results = []
-    assert ctx.tree_config()["targets_per_lane"] == -1  # by size, the default
+    assert ctx.tree_config() == {"targets_per_lane": -1, "lanes": -1}  # by size, the defaults
     try:
-        for r in (1, 2):
-            ctx.set_tree_config(targets_per_lane=r)
-            ctx.set_tree_config(targets_per_lane=0)  # keeps
-            assert ctx.tree_config()["targets_per_lane"] == r
+        for r, lanes in ((1, 64), (2, 64), (1, 32), (2, 32)):
+            ctx.set_tree_config(targets_per_lane=r, lanes=lanes)
+            ctx.set_tree_config()  # zeros keep
+            assert ctx.tree_config() == {"targets_per_lane": r, "lanes": lanes}
             s = E.HipTreeSolver(E.TreeParameters(0.5, 16, quadrupole=quadrupole), E.Softening(0.01),
                                 context=ctx)
             acc = (np.zeros(p.n), np.zeros(p.n), np.zeros(p.n))
@@ -292,11 +295,33 @@ def test_two_targets_per_lane_is_bitwise(ctx, quadrupole):
             c = s.interaction_count()
             results.append((acc, c.particle_particle, c.particle_cell, s.node_visits()))
     finally:
-        ctx.set_tree_config(targets_per_lane=-1)
-    (a1, pp1, pc1, v1), (a2, pp2, pc2, v2) = results
-    for k in range(3):
-        assert np.array_equal(a1[k], a2[k])
-    assert (pp1, pc1, v1) == (pp2, pc2, v2)
+        ctx.set_tree_config(targets_per_lane=-1, lanes=-1)
+    a1, pp1, pc1, v1 = results[0]
+    for a2, pp2, pc2, v2 in results[1:]:
+        for k in range(3):
+            assert np.array_equal(a1[k], a2[k])
+        assert (pp1, pc1, v1) == (pp2, pc2, v2)
+    # The segmented walk of a two-system batch, full and half lanes.
+    batches = []
+    try:
+        for lanes in (64, 32):
+            ctx.set_tree_config(lanes=lanes)
+            cfg = E.BatchConfig(integrator="velocity-verlet", opening_angle=0.5, leaf_capacity=16,
+                                quadrupole=quadrupole, tiles_per_item=16, direct_below=1000,
+                                use_graph=False)
+            b = E.DeviceBatch(ctx, [E.BatchSystem(p.n, 0.01, 1e-3), E.BatchSystem(q.n, 0.01, 1e-3)], cfg)
+            b.upload(0, *p.arrays())
+            b.upload(1, *q.arrays())
+            b.refresh()
+            batches.append([b.download(k) for k in range(2)] + [b.counts(0), b.counts(1)])
+            b.close()
+    finally:
+        ctx.set_tree_config(lanes=-1)
+    full, half = batches
+    for k in range(2):
+        for key in ("ax", "ay", "az"):
+            assert np.array_equal(full[k][key], half[k][key]), (k, key)
+    assert full[2:] == half[2:]
 
 
 def test_replayed_evaluations_are_bitwise(ctx):
