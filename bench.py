@@ -417,7 +417,8 @@ def bench_plummer_vv(E, W, ctx, cpu) -> dict:
            "gpu": {"ms_per_step": times[len(times) // 2], "runs": runs,
                    "range_ms_per_step": [times[0], times[-1]], "energy_drift": drift,
                    "force_error": force,
-                   "path": "device-resident single-system path (orr_system), eager launches"}}
+                   "path": "device-resident single-system path (orr_system), each step replayed as "
+                           "one captured graph"}}
     if cpu is not None:
         log("[bench] CPU: Plummer N=4096 velocity Verlet, 100 steps per trial")
         out["cpu_baseline"] = cpu.plummer_vv(state, eps, dt, steps)

@@ -30,6 +30,7 @@ struct orr_system {
     std::uint64_t count = 0;
     orr::Buffer state;   // x y z vx vy vz ax ay az m, FP64, count each
     orr::Buffer stages;  // RK4 only: x0 (3), v0 (3), position sums (3), velocity sums (3)
+    orr_ctx::Captured step_graph;  // one whole step (kicks, drifts, field), replayed
     double* array(int k) const { return state.as<double>() + static_cast<std::size_t>(k) * count; }
     double* stage(int k) const { return stages.as<double>() + static_cast<std::size_t>(k) * count; }
 };
@@ -446,6 +447,7 @@ int orr_system_destroy(orr_system* s) {
         if (s == nullptr) return;
         use_device(s->ctx);
         s->ctx->sync();
+        orr::drop_captured(s->step_graph);
         s->state.release();
         s->stages.release();
         delete s;
@@ -498,7 +500,7 @@ int orr_system_step(orr_system* s, const orr_step_config* c, double dt, uint64_t
         check_config(c);
         use_device(s->ctx);
         if (s->count == 0) return;
-        for (uint64_t k = 0; k < steps; ++k) {
+        auto one_step = [&] {
             if (c->integrator == 2) {
                 rk4_step(s, *c, dt);
             } else if (c->integrator == 1) {
@@ -508,7 +510,31 @@ int orr_system_step(orr_system* s, const orr_step_config* c, double dt, uint64_t
             } else {
                 vv_substep(s, *c, dt);
             }
-        }
+        };
+        // A whole step is captured once and replayed (one graph launch a
+        // step instead of its kicks, drifts and evaluation launches), keyed by
+        // everything the enqueued work depends on; with phase timing on it
+        // runs eagerly, timed.
+        std::vector<unsigned char> key;
+        orr::key_add(key, s->state.ptr);
+        orr::key_add(key, s->count);
+        orr::key_add(key, c->integrator);
+        orr::key_add(key, c->solver);
+        orr::key_add(key, c->softening_squared);
+        orr::key_add(key, c->gravitational_constant);
+        orr::key_add(key, c->opening_angle);
+        orr::key_add(key, c->leaf_capacity);
+        orr::key_add(key, c->quadrupole);
+        orr::key_add(key, dt);
+        orr_ctx* ctx = s->ctx;
+        orr::key_add(key, ctx->direct_targets_per_lane);
+        orr::key_add(key, ctx->direct_refine);
+        orr::key_add(key, ctx->direct_block_target);
+        orr::key_add(key, ctx->direct_pair_mode);
+        orr::key_add(key, ctx->direct_equal_masses);
+        orr::key_add(key, ctx->walk_targets_per_lane);
+        orr::key_add(key, ctx->walk_lanes);
+        for (uint64_t k = 0; k < steps; ++k) orr::run_captured(ctx, s->step_graph, key, one_step, false);
         s->ctx->sync();
     });
 }

@@ -63,6 +63,10 @@ void drop_captured(orr_ctx::Captured& g) noexcept {
 
 void run_captured(orr_ctx* ctx, orr_ctx::Captured& g, std::vector<unsigned char> key,
                   const std::function<void()>& enqueue, bool untimed) {
+    if (ctx->capturing) {
+        enqueue();
+        return;
+    }
     if (!ctx->graphs || (ctx->timing && !untimed)) {
         drop_captured(g);
         enqueue();
@@ -86,14 +90,17 @@ void run_captured(orr_ctx* ctx, orr_ctx::Captured& g, std::vector<unsigned char>
     if (g.exec == nullptr) {
         ORR_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         hipGraph_t graph = nullptr;
+        ctx->capturing = true;
         try {
             enqueue();
         } catch (...) {
+            ctx->capturing = false;
             (void)hipStreamEndCapture(ctx->stream, &graph);
             if (graph != nullptr) (void)hipGraphDestroy(graph);
             drop_captured(g);
             throw;
         }
+        ctx->capturing = false;
         ORR_HIP(hipStreamEndCapture(ctx->stream, &graph));
         g.graph = graph;
         const hipError_t made = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);

@@ -169,6 +169,10 @@ struct orr_ctx {
         hipGraphExec_t exec = nullptr;
     };
     Captured tree_graph, direct_graph;
+    // True while a run_captured sequence is being captured: a run_captured
+    // reached from inside it then enqueues its work inline (into the same
+    // capture) instead of launching or capturing a graph of its own.
+    bool capturing = false;
 
     hipEvent_t take_event();
     void begin(int phase, hipEvent_t& start);

@@ -364,3 +364,38 @@ def test_kepler_orbit_keeps_its_elements(ctx):
           f"separation in [{closest:.4f}, {furthest:.4f}]")
     assert a_err <= 0.01 and e_err <= 0.01 and prec <= 0.5
     assert closest >= 0.5 - 0.01 and furthest <= 1.5 + 0.01
+
+
+@pytest.mark.parametrize("integrator,solver,n", [("velocity-verlet", "direct", 4096),
+                                                  ("yoshida4", "barnes-hut", 20000),
+                                                  ("rk4", "direct", 3000),
+                                                  ("velocity-verlet", "direct", 60000)])
+def test_replayed_steps_equal_eager_steps(ctx, integrator, solver, n):
+    # A device-resident step is captured once and replayed as one graph per
+    # step. Replays, with eager steps of a second system (phase timing on, so
+    # uncaptured) and host downloads between them, end in the same bits as the
+    # eager steps after every chunk; a new timestep captures anew. 60,000
+    # particles take the pairs path with its second stream.
+    p = oracle.plummer(n, 5)
+    cfg = E.StepConfig(integrator=integrator, solver=solver, softening=0.01, opening_angle=0.5,
+                       leaf_capacity=16)
+    replayed, eager = E.DeviceSystem(ctx, n), E.DeviceSystem(ctx, n)
+    for s in (replayed, eager):
+        s.upload(*p.arrays())
+        s.refresh(cfg)
+    ctx.reset_timings()
+    try:
+        for dt, chunk in ((1e-3, 1), (1e-3, 3), (1e-3, 4), (5e-4, 2), (5e-4, 3)):
+            replayed.step(cfg, dt, chunk)
+            ctx.set_timing(True)
+            eager.step(cfg, dt, chunk)
+            ctx.set_timing(False)
+            a, b = replayed.download(), eager.download()
+            for key in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+                assert np.array_equal(a[key], b[key]), (dt, chunk, key)
+    finally:
+        ctx.set_timing(False)
+        replayed.close()
+        eager.close()
+    # Captured on the second step of each timestep, replayed after: 2 + 3 + 2 replays.
+    assert ctx.graph_launches() >= 7
