@@ -140,15 +140,17 @@ ORR_API int orr_direct_static_limits(uint64_t* max_count, uint64_t* scratch_byte
  * pairs path take its equal-mass form when every particle has one mass (the
  * mass leaves the pair term and multiplies the folded sums: two multiplies
  * fewer per pair; the same bits when that mass is a power of two), 2 never.
- * targets_per_lane 0, refine -1, block_target 0, pair_mode 0 and
- * equal_masses 0 keep the current value. */
+ * small_order 1 (default) orders a system of at most 4096 particles in one
+ * workgroup (one launch instead of some twenty; the same permutation, so the
+ * same bits), 2 never. targets_per_lane 0, refine -1, block_target 0,
+ * pair_mode 0, equal_masses 0 and small_order 0 keep the current value. */
 typedef struct {
     int32_t targets_per_lane;
     int32_t refine;
     uint32_t block_target;
     int32_t pair_mode;
     int32_t equal_masses;
-    int32_t reserved;
+    int32_t small_order;
 } orr_direct_config;
 ORR_API int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* config);
 ORR_API int orr_ctx_get_direct_config(orr_ctx* ctx, orr_direct_config* config);

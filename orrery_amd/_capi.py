@@ -44,7 +44,7 @@ class OrrDirectArgs(ctypes.Structure):
 class OrrDirectConfig(ctypes.Structure):
     _fields_ = [("targets_per_lane", ctypes.c_int32), ("refine", ctypes.c_int32),
                 ("block_target", ctypes.c_uint32), ("pair_mode", ctypes.c_int32),
-                ("equal_masses", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("equal_masses", ctypes.c_int32), ("small_order", ctypes.c_int32)]
 
 
 class OrrTreeArgs(ctypes.Structure):

@@ -283,6 +283,8 @@ int orr_ctx_set_direct_config(orr_ctx* ctx, const orr_direct_config* c) {
         require(c->refine == 0 || c->refine == 1 || c->refine == -1, "refine must be 0 or 1 (-1 keeps)");
         require(c->pair_mode >= 0 && c->pair_mode <= 3, "pair_mode must be 1, 2 or 3 (0 keeps)");
         require(c->equal_masses >= 0 && c->equal_masses <= 2, "equal_masses must be 1 or 2 (0 keeps)");
+        require(c->small_order >= 0 && c->small_order <= 2, "small_order must be 1 or 2 (0 keeps)");
+        if (c->small_order != 0) ctx->direct_small_order = c->small_order;
         if (c->equal_masses != 0) ctx->direct_equal_masses = c->equal_masses;
         if (c->targets_per_lane != 0) ctx->direct_targets_per_lane = c->targets_per_lane;
         if (c->pair_mode != 0) ctx->direct_pair_mode = c->pair_mode;
@@ -323,7 +325,7 @@ int orr_ctx_get_direct_config(orr_ctx* ctx, orr_direct_config* c) {
         c->block_target = ctx->direct_block_target;
         c->pair_mode = ctx->direct_pair_mode;
         c->equal_masses = ctx->direct_equal_masses;
-        c->reserved = 0;
+        c->small_order = ctx->direct_small_order;
     });
 }
 
@@ -532,6 +534,7 @@ int orr_system_step(orr_system* s, const orr_step_config* c, double dt, uint64_t
         orr::key_add(key, ctx->direct_block_target);
         orr::key_add(key, ctx->direct_pair_mode);
         orr::key_add(key, ctx->direct_equal_masses);
+        orr::key_add(key, ctx->direct_small_order);
         orr::key_add(key, ctx->walk_targets_per_lane);
         orr::key_add(key, ctx->walk_lanes);
         for (uint64_t k = 0; k < steps; ++k) orr::run_captured(ctx, s->step_graph, key, one_step, false);

@@ -1366,9 +1366,17 @@ void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a) {
         key_add(key, a.z);
         key_add(key, a.count);
         // Compact tiles need only the top 32 bits of the 63-bit codes: the four
-        // upper radix passes of eight.
-        run_captured(ctx, ctx->direct_graph, std::move(key),
-                     [&] { morton_enqueue(ctx, a.x, a.y, a.z, a.count, 4); }, true);
+        // upper radix passes of eight, or for a small system the same order
+        // from one workgroup.
+        const bool small = ctx->direct_small_order == 1 && a.count <= kSmallOrder;
+        key_add(key, small);
+        run_captured(ctx, ctx->direct_graph, std::move(key), [&] {
+            if (small) {
+                morton_enqueue_small(ctx, a.x, a.y, a.z, a.count, 4);
+            } else {
+                morton_enqueue(ctx, a.x, a.y, a.z, a.count, 4);
+            }
+        }, true);
         order = ctx->order_index.as<std::uint32_t>();
     }
     direct_enqueue(ctx, a, order);

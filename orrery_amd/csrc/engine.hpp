@@ -132,6 +132,7 @@ struct orr_ctx {
     int direct_pair_mode = 1;
     // 1: the pairs path's equal-mass form where the masses allow it, 2: never.
     int direct_equal_masses = 1;
+    int direct_small_order = 1;  // 1: one-workgroup ordering up to kSmallOrder particles, 2: never
     // Barnes-Hut walk configuration (orr_ctx_set_tree_config); 0 chooses by size.
     int walk_targets_per_lane = 0;
     int walk_lanes = 0;  // active lanes per walk wave: 32, 64, or 0 = by size
@@ -337,6 +338,11 @@ std::size_t ctx_device_bytes(const orr_ctx* ctx);
 // code's top bits only (stable within equal tops), enough for compact tiles.
 void morton_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,
                     std::uint64_t n, int first_digit = 0);
+// morton_enqueue's index (and cube) for n <= kSmallOrder and first_digit >= 4 in
+// one launch: the same permutation; ctx->order_codes is left as it was.
+constexpr std::uint64_t kSmallOrder = 4096;
+void morton_enqueue_small(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                          std::uint64_t n, int first_digit);
 // Copies the cube {origin, size} computed by the last morton_enqueue (syncs).
 void morton_cube(orr_ctx* ctx, double cube[4]);
 

@@ -354,6 +354,102 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __r
     }
 }
 
+// Small systems in one workgroup: extrema, cube, codes and the order by the
+// codes' digits from first_digit up, instead of some twenty short launches.
+// The multi-kernel order is a stable sort by those digits, that is a sort by
+// (code >> 8 first_digit, index), a key unique per element; a bitonic network
+// over the packed keys sorts them to the same permutation. first_digit >= 4
+// keeps the digits within 32 bits beside the 32-bit index.
+constexpr int kSmallThreads = 1024;
+
+__global__ __launch_bounds__(kSmallThreads) void morton_small(const double* __restrict__ x,
+                                                              const double* __restrict__ y,
+                                                              const double* __restrict__ z,
+                                                              std::uint32_t n, std::uint32_t padded,
+                                                              int first_digit,
+                                                              double* __restrict__ cube_out,
+                                                              std::uint32_t* __restrict__ index) {
+    __shared__ unsigned long long keys[kSmallOrder];
+    __shared__ double wave_ext[kSmallThreads / 64][6];
+    __shared__ double cube[4];
+    const std::uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const double inf = __builtin_huge_val();
+    double e[6] = {inf, inf, inf, -inf, -inf, -inf};
+    for (std::uint32_t i = t; i < n; i += kSmallThreads) {
+        const double v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            e[c] = v[c] < e[c] ? v[c] : e[c];
+            e[3 + c] = v[c] > e[3 + c] ? v[c] : e[3 + c];
+        }
+    }
+    // Extrema are exact, so folding them in any order gives the same values.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double a = __shfl_xor(e[c], off, 64);
+            const double b = __shfl_xor(e[3 + c], off, 64);
+            e[c] = a < e[c] ? a : e[c];
+            e[3 + c] = b > e[3 + c] ? b : e[3 + c];
+        }
+    }
+    if (lane == 0) {
+        for (int c = 0; c < 6; ++c) wave_ext[wave][c] = e[c];
+    }
+    __syncthreads();
+    if (t == 0) {
+        double lo[3], hi[3];
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = wave_ext[0][c];
+            hi[c] = wave_ext[0][3 + c];
+            for (std::uint32_t w = 1; w < kSmallThreads / 64; ++w) {
+                lo[c] = wave_ext[w][c] < lo[c] ? wave_ext[w][c] : lo[c];
+                hi[c] = wave_ext[w][3 + c] > hi[c] ? wave_ext[w][3 + c] : hi[c];
+            }
+        }
+        make_cube(lo, hi, cube);
+        for (int c = 0; c < 4; ++c) cube_out[c] = cube[c];
+    }
+    __syncthreads();
+    // Codes as encode computes them, packed (top digits, index); padding last.
+    const double size = cube[3];
+    const double scale = static_cast<double>(kGrid) / size;
+    const int shift = 8 * first_digit;
+    for (std::uint32_t i = t; i < padded; i += kSmallThreads) {
+        unsigned long long key = ~0ull;
+        if (i < n) {
+            std::uint64_t code = 0;
+            if (size > 0) {
+                code = (spread_bits(to_grid(x[i], cube[0], scale)) << 2) |
+                       (spread_bits(to_grid(y[i], cube[1], scale)) << 1) |
+                       spread_bits(to_grid(z[i], cube[2], scale));
+            }
+            key = ((code >> shift) << 32) | i;
+        }
+        keys[i] = key;
+    }
+    __syncthreads();
+    // Bitonic sort, ascending, of the `padded` (a power of two) keys.
+    for (std::uint32_t k = 2; k <= padded; k <<= 1) {
+        for (std::uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (std::uint32_t i = t; i < padded; i += kSmallThreads) {
+                const std::uint32_t partner = i ^ j;
+                if (partner > i) {
+                    const unsigned long long a = keys[i], b = keys[partner];
+                    const bool ascending = (i & k) == 0;
+                    if ((a > b) == ascending) {
+                        keys[i] = b;
+                        keys[partner] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (std::uint32_t i = t; i < n; i += kSmallThreads) index[i] = static_cast<std::uint32_t>(keys[i]);
+}
+
 }  // namespace
 
 OrderSizes order_sizes(std::uint64_t n, std::uint32_t segments, std::uint32_t chunks, bool aos) {
@@ -469,6 +565,22 @@ int morton_enqueue_into(orr_ctx* ctx, const double* x, const double* y, const do
         }
     }
     return cur;
+}
+
+void morton_enqueue_small(orr_ctx* ctx, const double* x, const double* y, const double* z,
+                          std::uint64_t n, int first_digit) {
+    if (n == 0) return;
+    if (n > kSmallOrder) fail(ORR_ERR_INVALID_ARGUMENT, "morton_enqueue_small: more than 4096 particles");
+    if (first_digit < 4) fail(ORR_ERR_INVALID_ARGUMENT, "morton_enqueue_small: first_digit must be at least 4");
+    ctx->order_bbox.reserve(4 * sizeof(double));
+    ctx->order_index.reserve(sizeof(std::uint32_t) * n);
+    std::uint32_t padded = 2;
+    while (padded < n) padded <<= 1;
+    PhaseTimer t(ctx, kSort);
+    hipLaunchKernelGGL(morton_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, x, y, z,
+                       static_cast<std::uint32_t>(n), padded, first_digit, ctx->order_bbox.as<double>(),
+                       ctx->order_index.as<std::uint32_t>());
+    ORR_LAUNCHED("morton_small");
 }
 
 void morton_cube(orr_ctx* ctx, double cube[4]) {

@@ -118,15 +118,15 @@ class Context:
 
     # -- direct kernel configuration (autotuner knob) -----------------------
     def set_direct_config(self, targets_per_lane: int = 0, refine: int = -1, block_target: int = 0,
-                          pair_mode: int = 0, equal_masses: int = 0) -> None:
-        c = C.OrrDirectConfig(targets_per_lane, refine, block_target, pair_mode, equal_masses, 0)
+                          pair_mode: int = 0, equal_masses: int = 0, small_order: int = 0) -> None:
+        c = C.OrrDirectConfig(targets_per_lane, refine, block_target, pair_mode, equal_masses, small_order)
         C.check(self._lib.orr_ctx_set_direct_config(self.handle, ctypes.byref(c)))
 
     def direct_config(self) -> dict:
         c = C.OrrDirectConfig()
         C.check(self._lib.orr_ctx_get_direct_config(self.handle, ctypes.byref(c)))
         return dict(targets_per_lane=c.targets_per_lane, refine=c.refine, block_target=c.block_target,
-                    pair_mode=c.pair_mode, equal_masses=c.equal_masses)
+                    pair_mode=c.pair_mode, equal_masses=c.equal_masses, small_order=c.small_order)
 
     def set_tree_config(self, targets_per_lane: int = 0, lanes: int = 0) -> None:
         c = C.OrrTreeConfig(targets_per_lane, lanes)

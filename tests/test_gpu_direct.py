@@ -202,12 +202,14 @@ def test_device_resident_evaluate_matches_host_staged(ctx):
 
 def test_direct_config_round_trip(ctx):
     assert ctx.direct_config() == {"targets_per_lane": 1, "refine": 0, "block_target": 32768,
-                                   "pair_mode": 1, "equal_masses": 1}
+                                   "pair_mode": 1, "equal_masses": 1, "small_order": 1}
     try:
-        ctx.set_direct_config(2, 1, 4096, 2, 2)
-        ctx.set_direct_config(0, -1, 0, 0, 0)  # keeps every field
+        ctx.set_direct_config(2, 1, 4096, 2, 2, 2)
+        ctx.set_direct_config(0, -1, 0, 0, 0, 0)  # keeps every field
         assert ctx.direct_config() == {"targets_per_lane": 2, "refine": 1, "block_target": 4096,
-                                       "pair_mode": 2, "equal_masses": 2}
+                                       "pair_mode": 2, "equal_masses": 2, "small_order": 2}
+        with pytest.raises(E.solvers.C.EngineError):
+            ctx.set_direct_config(0, -1, 0, 0, 0, 3)
         with pytest.raises(E.solvers.C.EngineError):
             ctx.set_direct_config(3, 0, 0)
         with pytest.raises(E.solvers.C.EngineError):
@@ -215,7 +217,7 @@ def test_direct_config_round_trip(ctx):
         with pytest.raises(E.solvers.C.EngineError):
             ctx.set_direct_config(0, -1, 0, 0, 3)
     finally:
-        ctx.set_direct_config(1, 0, 32768, 1, 1)
+        ctx.set_direct_config(1, 0, 32768, 1, 1, 1)
 
 
 # Pair mode 3 evaluates each far pair of tiles once for both tiles (blocks of
@@ -289,6 +291,42 @@ def test_momentum_residual_of_one_evaluation(ctx, n):
     # pair's term applied to both bodies): measured 4.4e-10 at 1024 and 3.5e-11
     # at 65,536.
     assert residual < 1e-8
+
+
+@pytest.mark.parametrize("case", ["plummer65", "plummer100", "plummer1000", "plummer2048",
+                                  "plummer3000", "plummer4096", "coincident", "line", "duplicates"])
+def test_small_order_is_the_multi_kernel_order(ctx, case):
+    # Up to 4096 particles the Morton order comes from one workgroup (extrema,
+    # cube, codes and a bitonic sort of (top digits, index)) instead of the
+    # multi-kernel radix sort; the permutation is the same, so the forces are
+    # the same bits. Degenerate layouts: every particle at one point (softened),
+    # all on a line, and pairs of coincident particles.
+    eps = 0.01
+    if case.startswith("plummer"):
+        n = int(case[len("plummer"):])
+        p = oracle.plummer(n, n)
+        pos, m = (p.x, p.y, p.z), p.m
+    elif case == "coincident":
+        n = 500
+        pos, m = (np.full(n, 0.3), np.full(n, -0.2), np.full(n, 0.1)), np.full(n, 1.0 / n)
+    elif case == "line":
+        n = 777
+        pos, m = (np.linspace(-1, 1, n), np.zeros(n), np.zeros(n)), np.full(n, 1.0 / n)
+    else:
+        q = oracle.plummer(1500, 77)
+        pos = tuple(np.concatenate([a, a]) for a in (q.x, q.y, q.z))
+        m = np.concatenate([q.m, q.m]) / 2
+    results = []
+    try:
+        for small in (1, 2):
+            ctx.set_direct_config(small_order=small)
+            (ax, ay, az), _ = evaluate(ctx, pos, m, eps=eps)
+            results.append((ax, ay, az))
+    finally:
+        ctx.set_direct_config(small_order=1)
+    for k in range(3):
+        assert np.array_equal(results[0][k], results[1][k]), (case, k)
+    assert np.all(np.isfinite(results[0][0]))
 
 
 def test_replayed_ordering_is_bitwise(ctx):
