@@ -197,12 +197,17 @@ ORR_API int orr_tree_eval(orr_ctx* ctx, const orr_tree_args* args, orr_tree_stat
  * more particles, else 1; the default); lanes 64 or 32 -- how many lanes of a
  * wave hold targets -- or -1 to choose by size (32 for a launch of at most
  * 2048 full waves, two per SIMD, else 64; the default). Results are
- * bit-identical either way; only the speed differs. Zero keeps the current
+ * bit-identical either way; only the speed differs. sort_mode chooses the
+ * Morton sort of every path (the tree's, the batch's, the direct ordering of
+ * more than 4096 particles): 1 (the default) one launch per radix pass after
+ * one launch that counts every pass's digits, 2 histogram, scan and scatter
+ * launches per pass; the same permutation either way. Zero keeps the current
  * value; the getter reports -1 while a choice is by size. */
 typedef struct {
     int32_t targets_per_lane;
     int32_t lanes;
-    int32_t reserved[2];
+    int32_t sort_mode;
+    int32_t reserved;
 } orr_tree_config;
 ORR_API int orr_ctx_set_tree_config(orr_ctx* ctx, const orr_tree_config* config);
 ORR_API int orr_ctx_get_tree_config(orr_ctx* ctx, orr_tree_config* config);

@@ -129,7 +129,8 @@ class OrrServerStats(ctypes.Structure):
 
 
 class OrrTreeConfig(ctypes.Structure):
-    _fields_ = [("targets_per_lane", ctypes.c_int32), ("lanes", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+    _fields_ = [("targets_per_lane", ctypes.c_int32), ("lanes", ctypes.c_int32), ("sort_mode", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 # (name, argtypes) of every entry point the header declares.

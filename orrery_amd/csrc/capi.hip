@@ -302,8 +302,10 @@ int orr_ctx_set_tree_config(orr_ctx* ctx, const orr_tree_config* config) {
         require(l == -1 || l == 0 || l == 32 || l == 64, "lanes must be 32, 64, 0 (keep) or -1 (by size)");
         if (r == -1) ctx->walk_targets_per_lane = 0;
         if (r > 0) ctx->walk_targets_per_lane = r;
+        require(config->sort_mode >= 0 && config->sort_mode <= 2, "sort_mode must be 1, 2 or 0 (keep)");
         if (l == -1) ctx->walk_lanes = 0;
         if (l > 0) ctx->walk_lanes = l;
+        if (config->sort_mode != 0) ctx->sort_mode = config->sort_mode;
     });
 }
 
@@ -314,6 +316,7 @@ int orr_ctx_get_tree_config(orr_ctx* ctx, orr_tree_config* config) {
         // -1 while the choice is by size, as orr_ctx_set_tree_config spells it.
         config->targets_per_lane = ctx->walk_targets_per_lane != 0 ? ctx->walk_targets_per_lane : -1;
         config->lanes = ctx->walk_lanes != 0 ? ctx->walk_lanes : -1;
+        config->sort_mode = ctx->sort_mode;
     });
 }
 

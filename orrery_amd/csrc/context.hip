@@ -79,6 +79,8 @@ void run_captured(orr_ctx* ctx, orr_ctx::Captured& g, std::vector<unsigned char>
         ~Untimed() { ctx->timing = was; }
     } restore{ctx, ctx->timing};
     ctx->timing = false;
+    // The sort form changes the launches, not the arguments: part of the key.
+    key_add(key, ctx->sort_mode);
     if (!(g.seen && g.key == key && g.epoch == g_reallocations.load())) {
         drop_captured(g);
         enqueue();

@@ -133,6 +133,7 @@ struct orr_ctx {
     // 1: the pairs path's equal-mass form where the masses allow it, 2: never.
     int direct_equal_masses = 1;
     int direct_small_order = 1;  // 1: one-workgroup ordering up to kSmallOrder particles, 2: never
+    int sort_mode = 1;  // Morton sort: 1 one-sweep passes, 2 histogram + scan + scatter per pass
     // Barnes-Hut walk configuration (orr_ctx_set_tree_config); 0 chooses by size.
     int walk_targets_per_lane = 0;
     int walk_lanes = 0;  // active lanes per walk wave: 32, 64, or 0 = by size

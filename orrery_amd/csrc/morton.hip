@@ -10,8 +10,11 @@
 //   * a stable LSD radix sort starting from identity order reproduces the
 //     (code, index) lexicographic total order (morton.hpp:155-175).
 //
-// Sort: 8 passes of 8-bit digits. Each pass is histogram -> device-wide exclusive scan ->
-// scatter. The scatter ranks keys stably inside 256-key tiles with wave64
+// Sort: 8 passes of 8-bit digits. By default one launch counts the digits of
+// every pass and each pass is one scatter that finds the blocks before it by
+// decoupled look-back; the multi-kernel form (per pass: histogram -> device-wide
+// exclusive scan -> scatter) gives the same permutation and stays selectable.
+// Both scatters rank keys stably inside 256-key tiles with wave64
 // ballots (8 ballots give each lane the mask of lanes sharing its digit) and an
 // LDS table of per-wave digit counts, so no LDS atomics decide an order.
 #include <hip/hip_runtime.h>
@@ -31,6 +34,7 @@ constexpr int kBlock = 256;
 constexpr int kTilesPerBlock = 8;                   // keys per sort block = 2048
 constexpr int kKeysPerBlock = kBlock * kTilesPerBlock;
 constexpr unsigned kGrid = 1u << 21;
+constexpr int kMaxSortPasses = 10;  // eight code digits and up to two segment digits
 
 // ---- bounding cube -----------------------------------------------------------
 
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void radix_histogram(const std::uint64_t* _
     hist[static_cast<std::uint64_t>(threadIdx.x) * gridDim.x + blockIdx.x] = counts[threadIdx.x];
 }
 
-// Stable scatter of one pass. Each block ranks its 4096 keys tile by tile
+// Stable scatter of one pass. Each block ranks its 2048 keys tile by tile
 // (256 at a time, in order: ballots give each lane the lanes of its wave with
 // the same digit, an LDS table the counts of the waves before it) into LDS in
 // digit order, then writes them out in that order: the keys of one digit leave
@@ -351,6 +355,255 @@ __global__ __launch_bounds__(kBlock) void radix_scatter(const std::uint64_t* __r
         const std::uint32_t pos = out_base[digit_of(key, val, shift, seg_of)] + s;
         keys_out[pos] = key;
         vals_out[pos] = val;
+    }
+}
+
+// ---- one-sweep passes ----------------------------------------------------------
+//
+// The digit totals of a pass do not depend on the order the keys are in, so one
+// launch counts the digits of every pass (per block; a second launch sums them
+// and scans them into each digit's first position), and each pass is then one
+// scatter launch instead of histogram, scan and scatter. A block counts its own
+// keys' digits, publishes the counts, stages its keys in LDS in digit order as
+// radix_scatter does, and learns the counts of the blocks before it by
+// decoupled look-back over one status word per (block, digit): the count in the
+// low 30 bits, above it kAggregate (this block's own count) or kPrefix (the
+// count of all blocks up to and including this one). One word carries flag and
+// count, so nothing else has to be ordered with it; agent-scope atomic loads
+// and stores make it visible across the XCDs' L2 caches. A block waits only on
+// blocks with lower indices, and each XCD starts its workgroups in index order,
+// so the lowest unfinished block is always running or next to start on its XCD
+// (a per-block ticket from one atomic counter, the portable guarantee, cost
+// 1024 contended device atomics a pass); every block publishes its counts
+// before it waits on anything, and a wait that still outlasts kSpinCap polls
+// gives up (a wrong order, never a hang). The positions are those of
+// radix_scatter: the same stable permutation.
+constexpr std::uint32_t kAggregate = 1u << 30, kPrefix = 2u << 30, kCountMask = kAggregate - 1;
+constexpr std::uint32_t kCountBlocks = 256;
+constexpr int kCountThreads = 1024;
+constexpr int kCountUnroll = 8;  // keys a counting thread loads before it counts them
+constexpr std::uint32_t kSpinCap = 1u << 20;
+// Blocks whose words one look-back step loads together. At 2,097,152 keys
+// (1024 blocks) 32 measured best: 64 took the sort from 0.31 to 0.39 ms, and
+// 4096 keys a block (half the blocks) gave no gain at 2M and lost at 262,144.
+constexpr int kLookback = 32;
+
+__global__ __launch_bounds__(kCountThreads) void radix_count_all(const std::uint64_t* __restrict__ keys,
+                                                                 const std::uint32_t* __restrict__ vals,
+                                                                 std::uint64_t n, int first_pass, int passes,
+                                                                 const std::uint32_t* __restrict__ seg_of,
+                                                                 std::uint32_t* __restrict__ partial,
+                                                                 std::uint32_t* __restrict__ clear,
+                                                                 std::uint64_t clear_words) {
+    __shared__ std::uint32_t counts[kMaxSortPasses * 256];
+    for (int w = threadIdx.x; w < kMaxSortPasses * 256; w += kCountThreads) counts[w] = 0;
+    __syncthreads();
+    const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * kCountThreads;
+    for (std::uint64_t i0 = static_cast<std::uint64_t>(blockIdx.x) * kCountThreads + threadIdx.x; i0 < n;
+         i0 += stride * kCountUnroll) {
+        // Loads first, so that they are in flight together.
+        std::uint64_t key[kCountUnroll];
+        std::uint32_t val[kCountUnroll];
+#pragma unroll
+        for (int u = 0; u < kCountUnroll; ++u) {
+            const std::uint64_t i = i0 + u * stride;
+            key[u] = i < n ? keys[i] : 0;
+            val[u] = i < n && seg_of != nullptr ? vals[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kCountUnroll; ++u) {
+            if (i0 + u * stride < n) {
+                for (int p = first_pass; p < passes; ++p) {
+                    const bool by_segment = p >= 8;
+                    const std::uint32_t d = digit_of(key[u], val[u], by_segment ? 8 * (p - 8) : 8 * p,
+                                                     by_segment ? seg_of : nullptr);
+                    atomicAdd(&counts[p * 256 + d], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int w = first_pass * 256 + static_cast<int>(threadIdx.x); w < passes * 256; w += kCountThreads) {
+        const int p = w / 256, d = w % 256;
+        partial[(static_cast<std::size_t>(p) * gridDim.x + blockIdx.x) * 256 + d] = counts[w];
+    }
+    // The status words of this sort's passes start at zero.
+    for (std::uint64_t w = static_cast<std::uint64_t>(blockIdx.x) * kCountThreads + threadIdx.x; w < clear_words;
+         w += stride) {
+        clear[w] = 0;
+    }
+}
+
+// One block per pass: each digit's total over the counting blocks (four
+// threads a digit, each over every fourth block), scanned over the digits into
+// the position of the digit's first key.
+constexpr int kBaseThreads = 1024;
+__global__ __launch_bounds__(kBaseThreads) void radix_bases(const std::uint32_t* __restrict__ partial,
+                                                            std::uint32_t count_blocks, int first_pass,
+                                                            std::uint32_t* __restrict__ bases) {
+    __shared__ std::uint32_t quarter[kBaseThreads / 256][256];
+    __shared__ std::uint32_t wave_sums[4];
+    const int p = first_pass + static_cast<int>(blockIdx.x);
+    const std::uint32_t d = threadIdx.x & 255, q = threadIdx.x >> 8;
+    const std::uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const std::uint32_t* col = partial + static_cast<std::size_t>(p) * count_blocks * 256 + d;
+    std::uint32_t part = 0;
+#pragma unroll 16
+    for (std::uint32_t b = q; b < count_blocks; b += kBaseThreads / 256) part += col[static_cast<std::size_t>(b) * 256];
+    quarter[q][d] = part;
+    __syncthreads();
+    // The first four waves (one thread a digit) scan the totals.
+    std::uint32_t total = 0, incl = 0;
+    if (q == 0) {
+        for (int k = 0; k < kBaseThreads / 256; ++k) total += quarter[k][d];
+        incl = total;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const std::uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= static_cast<std::uint32_t>(off)) incl += v;
+        }
+        if (lane == 63) wave_sums[wave] = incl;
+    }
+    __syncthreads();
+    if (q == 0) {
+        std::uint32_t before = 0;
+        for (std::uint32_t w = 0; w < wave; ++w) before += wave_sums[w];
+        bases[p * 256 + d] = before + incl - total;
+    }
+}
+
+__device__ __forceinline__ std::uint32_t status_load(const std::uint32_t* w) {
+    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(std::uint32_t* w, std::uint32_t v) {
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One pass: count, publish, stage in digit order, look back, write out.
+__global__ __launch_bounds__(kBlock) void radix_onesweep(const std::uint64_t* __restrict__ keys_in,
+                                                         const std::uint32_t* __restrict__ vals_in,
+                                                         std::uint64_t n, int shift,
+                                                         const std::uint32_t* __restrict__ seg_of,
+                                                         const std::uint32_t* __restrict__ bases,
+                                                         std::uint32_t* __restrict__ status,
+                                                         std::uint64_t* __restrict__ keys_out,
+                                                         std::uint32_t* __restrict__ vals_out) {
+    __shared__ std::uint64_t skeys[kKeysPerBlock];
+    __shared__ std::uint32_t svals[kKeysPerBlock];
+    __shared__ std::uint32_t out_base[256];       // global start of d - block-local start of d
+    __shared__ std::uint32_t running[256];        // the block's digit counts, then next local slots
+    __shared__ std::uint32_t wave_counts[4][256];  // digit counts of each wave in this tile
+    __shared__ std::uint32_t wave_sums[4];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const std::uint32_t d0 = threadIdx.x;
+    running[d0] = 0;
+    for (int w = 0; w < 4; ++w) wave_counts[w][d0] = 0;
+    __syncthreads();
+    const std::uint32_t block = blockIdx.x;
+    const std::uint64_t base = static_cast<std::uint64_t>(block) * kKeysPerBlock;
+    const std::uint32_t in_block = static_cast<std::uint32_t>(
+        n - base < static_cast<std::uint64_t>(kKeysPerBlock) ? n - base : kKeysPerBlock);
+    std::uint64_t key[kTilesPerBlock];
+    std::uint32_t val[kTilesPerBlock], dig[kTilesPerBlock];
+#pragma unroll
+    for (int t = 0; t < kTilesPerBlock; ++t) {
+        const std::uint32_t j = static_cast<std::uint32_t>(t) * kBlock + threadIdx.x;
+        const bool valid = j < in_block;
+        key[t] = valid ? keys_in[base + j] : 0;
+        val[t] = valid ? vals_in[base + j] : 0;
+        dig[t] = valid ? digit_of(key[t], val[t], shift, seg_of) : 0u;
+        if (valid) atomicAdd(&running[dig[t]], 1u);
+    }
+    __syncthreads();
+    const std::uint32_t count_d = running[d0];
+    std::uint32_t* const mine = status + static_cast<std::size_t>(block) * 256 + d0;
+    status_store(mine, (block == 0 ? kPrefix : kAggregate) | count_d);
+    // Block-local exclusive scan of the counts (as radix_scatter).
+    std::uint32_t incl = count_d;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const std::uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wave_sums[wave] = incl;
+    __syncthreads();
+    std::uint32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += wave_sums[w];
+    const std::uint32_t local_start = before + incl - count_d;
+    running[d0] = local_start;
+    __syncthreads();
+
+    const std::uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int t = 0; t < kTilesPerBlock; ++t) {
+        const bool valid = static_cast<std::uint32_t>(t) * kBlock + threadIdx.x < in_block;
+        const std::uint32_t d = dig[t];
+        std::uint64_t same = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = ((d >> b) & 1u) != 0;
+            const std::uint64_t m = __ballot(bit);
+            same &= bit ? m : ~m;
+        }
+        const std::uint32_t rank = static_cast<std::uint32_t>(__popcll(same & lanes_below));
+        if (valid && rank == 0) wave_counts[wave][d] = static_cast<std::uint32_t>(__popcll(same));
+        __syncthreads();
+        if (valid) {
+            std::uint32_t slot = running[d] + rank;
+            for (int w = 0; w < wave; ++w) slot += wave_counts[w][d];
+            skeys[slot] = key[t];
+            svals[slot] = val[t];
+        }
+        __syncthreads();
+        const std::uint32_t total = wave_counts[0][threadIdx.x] + wave_counts[1][threadIdx.x] +
+                                    wave_counts[2][threadIdx.x] + wave_counts[3][threadIdx.x];
+        running[threadIdx.x] += total;
+        for (int w = 0; w < 4; ++w) wave_counts[w][threadIdx.x] = 0;
+        __syncthreads();
+    }
+
+    // Look back over the blocks before this one, digit d0 per thread: the words
+    // of kLookback blocks are loaded together (one memory latency a window, not
+    // one a block), then taken nearest first until an inclusive count.
+    std::uint32_t prefix = 0;
+    if (block > 0) {
+        std::int64_t p = static_cast<std::int64_t>(block) - 1;
+        bool found = false;
+        while (!found) {
+            std::uint32_t word[kLookback];
+#pragma unroll
+            for (int k = 0; k < kLookback; ++k) {
+                // Before block 0: an empty inclusive count (block 0 publishes one first anyway).
+                word[k] = p - k >= 0 ? status_load(status + static_cast<std::size_t>(p - k) * 256 + d0) : kPrefix;
+            }
+#pragma unroll
+            for (int k = 0; k < kLookback; ++k) {
+                if (!found) {
+                    std::uint32_t w = word[k];
+                    std::uint32_t spins = 0;
+                    while ((w & (kAggregate | kPrefix)) == 0 && spins < kSpinCap) {
+                        __builtin_amdgcn_s_sleep(1);
+                        w = status_load(status + static_cast<std::size_t>(p - k) * 256 + d0);
+                        ++spins;
+                    }
+                    prefix += w & kCountMask;
+                    found = (w & kPrefix) != 0 || spins >= kSpinCap;
+                }
+            }
+            p -= kLookback;
+        }
+        status_store(mine, kPrefix | (prefix + count_d));
+    }
+    out_base[d0] = bases[d0] + prefix - local_start;
+    __syncthreads();
+#pragma unroll 4
+    for (std::uint32_t s = threadIdx.x; s < in_block; s += kBlock) {
+        const std::uint64_t k = skeys[s];
+        const std::uint32_t v = svals[s];
+        const std::uint32_t pos = out_base[digit_of(k, v, shift, seg_of)] + s;
+        keys_out[pos] = k;
+        vals_out[pos] = v;
     }
 }
 
@@ -452,6 +705,29 @@ __global__ __launch_bounds__(kSmallThreads) void morton_small(const double* __re
 
 }  // namespace
 
+namespace {
+
+// The one-sweep passes' words inside the histogram buffer: one status word
+// per (pass, sort block, digit), the counting blocks' digit counts per pass,
+// and each pass's digit bases.
+struct SweepLayout {
+    std::uint32_t count_blocks;
+    std::size_t status, partial, bases, words;
+};
+
+SweepLayout sweep_layout(std::uint64_t n, std::uint32_t sort_blocks) {
+    SweepLayout L{};
+    L.count_blocks = static_cast<std::uint32_t>(
+        std::min<std::uint64_t>(kCountBlocks, (n + kCountThreads - 1) / kCountThreads));
+    L.status = 0;
+    L.partial = L.status + static_cast<std::size_t>(kMaxSortPasses) * sort_blocks * 256;
+    L.bases = L.partial + static_cast<std::size_t>(kMaxSortPasses) * L.count_blocks * 256;
+    L.words = L.bases + static_cast<std::size_t>(kMaxSortPasses) * 256;
+    return L;
+}
+
+}  // namespace
+
 OrderSizes order_sizes(std::uint64_t n, std::uint32_t segments, std::uint32_t chunks, bool aos) {
     OrderSizes z{};
     z.bbox_blocks = segments > 1
@@ -463,7 +739,8 @@ OrderSizes order_sizes(std::uint64_t n, std::uint32_t segments, std::uint32_t ch
     z.codes = sizeof(std::uint64_t) * n;
     z.index = sizeof(std::uint32_t) * n;
     const std::size_t hist_words = 256 * static_cast<std::size_t>(z.sort_blocks);
-    z.hist = sizeof(std::uint32_t) * (hist_words + scan_scratch_words(hist_words));
+    z.hist = sizeof(std::uint32_t) * std::max(hist_words + scan_scratch_words(hist_words),
+                                              sweep_layout(n, z.sort_blocks).words);
     z.total = z.aos + z.bbox + 2 * z.codes + 2 * z.index + z.hist;
     return z;
 }
@@ -549,6 +826,30 @@ int morton_enqueue_into(orr_ctx* ctx, const double* x, const double* y, const do
         // segment of each element: segment-major, then (code, index).
         int passes = 8;
         if (segmented) passes += segs.count > 256 ? 2 : 1;
+        if (ctx->sort_mode == 1 && n <= kCountMask) {
+            const SweepLayout L = sweep_layout(n, sort_blocks);
+            std::uint32_t* const status = hist + L.status;
+            const std::size_t per_pass = static_cast<std::size_t>(sort_blocks) * 256;
+            hipLaunchKernelGGL(radix_count_all, dim3(L.count_blocks), dim3(kCountThreads), 0, ctx->stream,
+                               keys[0], vals[0], n, first_digit, passes, seg_of, hist + L.partial,
+                               status + first_digit * per_pass,
+                               static_cast<std::uint64_t>(passes - first_digit) * per_pass);
+            ORR_LAUNCHED("radix_count_all");
+            hipLaunchKernelGGL(radix_bases, dim3(passes - first_digit), dim3(kBaseThreads), 0, ctx->stream,
+                               hist + L.partial, L.count_blocks, first_digit, hist + L.bases);
+            ORR_LAUNCHED("radix_bases");
+            for (int pass = first_digit; pass < passes; ++pass) {
+                const bool by_segment = pass >= 8;
+                const int shift = by_segment ? 8 * (pass - 8) : 8 * pass;
+                hipLaunchKernelGGL(radix_onesweep, dim3(sort_blocks), dim3(kBlock), 0, ctx->stream,
+                                   keys[cur], vals[cur], n, shift, by_segment ? seg_of : nullptr,
+                                   hist + L.bases + 256 * pass, status + pass * per_pass, keys[1 - cur],
+                                   vals[1 - cur]);
+                ORR_LAUNCHED("radix_onesweep");
+                cur = 1 - cur;
+            }
+            return cur;
+        }
         for (int pass = first_digit; pass < passes; ++pass) {
             const bool by_segment = pass >= 8;
             const int shift = by_segment ? 8 * (pass - 8) : 8 * pass;

@@ -128,14 +128,14 @@ class Context:
         return dict(targets_per_lane=c.targets_per_lane, refine=c.refine, block_target=c.block_target,
                     pair_mode=c.pair_mode, equal_masses=c.equal_masses, small_order=c.small_order)
 
-    def set_tree_config(self, targets_per_lane: int = 0, lanes: int = 0) -> None:
-        c = C.OrrTreeConfig(targets_per_lane, lanes)
+    def set_tree_config(self, targets_per_lane: int = 0, lanes: int = 0, sort_mode: int = 0) -> None:
+        c = C.OrrTreeConfig(targets_per_lane, lanes, sort_mode)
         C.check(self._lib.orr_ctx_set_tree_config(self.handle, ctypes.byref(c)))
 
     def tree_config(self) -> dict:
         c = C.OrrTreeConfig()
         C.check(self._lib.orr_ctx_get_tree_config(self.handle, ctypes.byref(c)))
-        return dict(targets_per_lane=c.targets_per_lane, lanes=c.lanes)
+        return dict(targets_per_lane=c.targets_per_lane, lanes=c.lanes, sort_mode=c.sort_mode)
 
     # -- machine ceilings (benchmarks/cuda_probes.cu equivalents) -----------
     PROBES = {"fma": 0, "rsq": 1, "read": 2, "copy": 3, "mix": 4}

@@ -256,6 +256,30 @@ def test_system_order_does_not_change_results(ctx):
     b.close()
 
 
+def test_sort_forms_agree_bitwise(ctx):
+    # 300 tree systems: the segmented Morton sort orders by code, then by two
+    # digits of the system number. The one-sweep passes and the multi-kernel
+    # passes give the same permutation, hence the same trees and bits.
+    ns = sizes(300, 16, 2000, 21)
+    config = E.BatchConfig(opening_angle=0.6, leaf_capacity=16, use_graph=False)
+    results = []
+    try:
+        for mode in (1, 2):
+            ctx.set_tree_config(sort_mode=mode)
+            b, _ = make_batch(ctx, ns, config, solver=1)
+            assert b.info()["tree_systems"] == 300
+            b.refresh()
+            results.append(([b.download(k) for k in range(len(ns))], [b.counts(k) for k in range(len(ns))]))
+            b.close()
+    finally:
+        ctx.set_tree_config(sort_mode=1)
+    (a, ca), (b, cb) = results
+    for k in range(len(ns)):
+        for key in ("ax", "ay", "az"):
+            assert np.array_equal(a[k][key], b[k][key]), (k, key)
+    assert ca == cb
+
+
 def test_edge_systems_in_one_batch(ctx):
     # Zero softening (masked self terms), tiny systems forced onto the tree,
     # systems smaller than a leaf, and one direct system alone in its branch.

@@ -282,12 +282,12 @@ def test_wave_shapes_are_bitwise(ctx, quadrupole):
     p = oracle.disc_collision(30000, 4)
     q = oracle.plummer(7000, 9)
     results = []
-    assert ctx.tree_config() == {"targets_per_lane": -1, "lanes": -1}  # by size, the defaults
+    assert ctx.tree_config() == {"targets_per_lane": -1, "lanes": -1, "sort_mode": 1}  # the defaults
     try:
         for r, lanes in ((1, 64), (2, 64), (1, 32), (2, 32)):
             ctx.set_tree_config(targets_per_lane=r, lanes=lanes)
             ctx.set_tree_config()  # zeros keep
-            assert ctx.tree_config() == {"targets_per_lane": r, "lanes": lanes}
+            assert ctx.tree_config() == {"targets_per_lane": r, "lanes": lanes, "sort_mode": 1}
             s = E.HipTreeSolver(E.TreeParameters(0.5, 16, quadrupole=quadrupole), E.Softening(0.01),
                                 context=ctx)
             acc = (np.zeros(p.n), np.zeros(p.n), np.zeros(p.n))
