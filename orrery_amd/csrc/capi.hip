@@ -55,7 +55,9 @@ void orr::system_upload(orr_system* s, int first, int arrays, const double* host
 
 namespace {
 
-void field_eval(orr_system* s, const orr_step_config& c) {
+// Evaluates a(x); with `kick`, also v += kick->s a when the solver can fuse it
+// into its fold (direct), which the return value reports.
+bool field_eval(orr_system* s, const orr_step_config& c, const orr::FusedKick* kick = nullptr) {
     if (c.solver == 1) {
         orr_tree_args t{};
         t.x = s->array(0);
@@ -72,7 +74,9 @@ void field_eval(orr_system* s, const orr_step_config& c) {
         t.leaf_capacity = c.leaf_capacity;
         t.quadrupole = c.quadrupole;
         orr::tree_evaluate(s->ctx, t, nullptr);
-    } else {
+        return false;
+    }
+    {
         orr_direct_args d{};
         d.x = s->array(0);
         d.y = s->array(1);
@@ -85,7 +89,8 @@ void field_eval(orr_system* s, const orr_step_config& c) {
         d.softening_squared = c.softening_squared;
         d.gravitational_constant = c.gravitational_constant;
         d.spatial_order = 1;
-        orr::direct_evaluate(s->ctx, d);
+        orr::direct_evaluate(s->ctx, d, kick != nullptr ? *kick : orr::FusedKick{});
+        return kick != nullptr;
     }
 }
 
@@ -93,13 +98,15 @@ void field_eval(orr_system* s, const orr_step_config& c) {
 void vv_substep(orr_system* s, const orr_step_config& c, double dt) {
     const double half = dt / 2;
     const std::uint64_t n = s->count;
-    orr::kick(s->ctx, n, s->array(3), s->array(4), s->array(5), s->array(6), s->array(7),
-              s->array(8), half);
-    orr::drift(s->ctx, n, s->array(0), s->array(1), s->array(2), s->array(3), s->array(4),
-               s->array(5), dt);
-    field_eval(s, c);
-    orr::kick(s->ctx, n, s->array(3), s->array(4), s->array(5), s->array(6), s->array(7),
-              s->array(8), half);
+    // The first kick and the drift as one launch; the second kick inside the
+    // direct fold where it can be (the same bits as four separate launches).
+    orr::kick_drift(s->ctx, n, s->array(0), s->array(1), s->array(2), s->array(3), s->array(4),
+                    s->array(5), s->array(6), s->array(7), s->array(8), half, dt);
+    const orr::FusedKick k{s->array(3), s->array(4), s->array(5), half};
+    if (!field_eval(s, c, &k)) {
+        orr::kick(s->ctx, n, s->array(3), s->array(4), s->array(5), s->array(6), s->array(7),
+                  s->array(8), half);
+    }
 }
 
 using orr::kInnerWeight;

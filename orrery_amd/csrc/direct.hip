@@ -485,13 +485,27 @@ __global__ __launch_bounds__(kBlock) void direct_tiles(const float4* __restrict_
 }
 
 // Fixed-order FP64 fold of the splits; sum - compensation per split.
+// The fold's output: a, and the fused kick's v += s a when there is one.
+__device__ __forceinline__ void store_field(std::uint64_t dst, double gx, double gy, double gz,
+                                            double* __restrict__ ax, double* __restrict__ ay,
+                                            double* __restrict__ az, const FusedKick& kick) {
+    ax[dst] = gx;
+    ay[dst] = gy;
+    az[dst] = gz;
+    if (kick.vx != nullptr) {
+        kick.vx[dst] = kick_update(kick.vx[dst], kick.s, gx);
+        kick.vy[dst] = kick_update(kick.vy[dst], kick.s, gy);
+        kick.vz[dst] = kick_update(kick.vz[dst], kick.s, gz);
+    }
+}
+
 __global__ __launch_bounds__(256) void direct_reduce(const float* __restrict__ partial,
                                                      std::uint32_t splits, std::uint32_t npad,
                                                      std::uint64_t n,
                                                      const std::uint32_t* __restrict__ order,
                                                      double G, double* __restrict__ ax,
                                                      double* __restrict__ ay,
-                                                     double* __restrict__ az) {
+                                                     double* __restrict__ az, FusedKick kick) {
     const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
@@ -503,9 +517,7 @@ __global__ __launch_bounds__(256) void direct_reduce(const float* __restrict__ p
         a2 += static_cast<double>(p[2 * np + k]) - static_cast<double>(p[5 * np + k]);
     }
     const std::uint64_t dst = order != nullptr ? order[k] : k;
-    ax[dst] = G * a0;
-    ay[dst] = G * a1;
-    az[dst] = G * a2;
+    store_field(dst, G * a0, G * a1, G * a2, ax, ay, az, kick);
 }
 
 // ---- batches: many independent systems in one launch ---------------------
@@ -1053,7 +1065,7 @@ __global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jp
                                                     const std::uint32_t* __restrict__ order,
                                                     double G, double* __restrict__ ax,
                                                     double* __restrict__ ay,
-                                                    double* __restrict__ az) {
+                                                    double* __restrict__ az, FusedKick kick) {
     const std::uint64_t k = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const std::uint32_t t = static_cast<std::uint32_t>(k / kTile), lane = static_cast<std::uint32_t>(k % kTile);
@@ -1086,9 +1098,7 @@ __global__ __launch_bounds__(256) void pairs_reduce(const float* __restrict__ jp
     a1 += static_cast<double>(cpart[np + k]) - static_cast<double>(cpart[4 * np + k]);
     a2 += static_cast<double>(cpart[2 * np + k]) - static_cast<double>(cpart[5 * np + k]);
     const std::uint64_t dst = order != nullptr ? order[k] : k;
-    ax[dst] = G * a0;
-    ay[dst] = G * a1;
-    az[dst] = G * a2;
+    store_field(dst, G * a0, G * a1, G * a2, ax, ay, az, kick);
 }
 
 template<int R, bool kSafe, bool kRefine, bool kMask>
@@ -1256,7 +1266,8 @@ std::size_t mask_buffer_bytes(const DirectShape& d) {
 
 // Every launch of one evaluation on given buffers; phases timed when `timed`.
 void direct_launch(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order,
-                   const DirectShape& d, const DirectBuffers& buf, bool timed) {
+                   const DirectShape& d, const DirectBuffers& buf, bool timed,
+                   const FusedKick& kick = FusedKick{}) {
     const std::uint64_t n = a.count;
     const std::uint32_t tiles = d.tiles, npad = d.npad, words = d.words;
     const PairPlan& plan = d.plan;
@@ -1304,7 +1315,7 @@ void direct_launch(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* 
             const std::uint32_t blocks = static_cast<std::uint32_t>((n + 255) / 256);
             hipLaunchKernelGGL(pairs_reduce, dim3(blocks), dim3(256), 0, ctx->stream, jpart, ipart,
                                cpart, uniform, tiles, plan.blocks, plan.S, n, order,
-                               a.gravitational_constant, a.ax, a.ay, a.az);
+                               a.gravitational_constant, a.ax, a.ay, a.az, kick);
             ORR_LAUNCHED("pairs_reduce");
         }
         return;
@@ -1322,14 +1333,15 @@ void direct_launch(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* 
         PhaseTimer t(ctx, kDirectReduce, timed);
         const std::uint32_t blocks = static_cast<std::uint32_t>((n + 255) / 256);
         hipLaunchKernelGGL(direct_reduce, dim3(blocks), dim3(256), 0, ctx->stream, partials, d.splits,
-                           npad, n, order, a.gravitational_constant, a.ax, a.ay, a.az);
+                           npad, n, order, a.gravitational_constant, a.ax, a.ay, a.az, kick);
         ORR_LAUNCHED("direct_reduce");
     }
 }
 
 }  // namespace
 
-void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order) {
+void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order,
+                    const FusedKick& kick) {
     if (a.count == 0) return;
     const DirectShape d = direct_shape(ctx, a.count);
     ctx->direct_packed.reserve(packed_buffer_bytes(d));
@@ -1341,7 +1353,7 @@ void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t*
     buf.centres = ctx->direct_centres.as<double4>();
     buf.partials = ctx->direct_partials.as<float>();
     buf.mask = d.masked ? ctx->direct_near.as<std::uint32_t>() : nullptr;
-    direct_launch(ctx, a, order, d, buf, true);
+    direct_launch(ctx, a, order, d, buf, true, kick);
 }
 
 namespace {
@@ -1354,7 +1366,7 @@ std::size_t direct_batch_bytes(std::uint64_t tiles, std::uint64_t slots) {
     return packed_bytes(tiles) + centre_bytes(tiles) + slot_bytes(slots);
 }
 
-void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a) {
+void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a, const FusedKick& kick) {
     if (a.count == 0) return;
     const std::uint32_t* order = nullptr;
     if (a.spatial_order != 0 && a.count > 64) {
@@ -1379,7 +1391,7 @@ void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a) {
         }, true);
         order = ctx->order_index.as<std::uint32_t>();
     }
-    direct_enqueue(ctx, a, order);
+    direct_enqueue(ctx, a, order, kick);
 }
 
 void direct_batch_enqueue(orr_ctx* ctx, const double* x, const double* y, const double* z,

@@ -221,8 +221,23 @@ private:
 
 // ---- entry points implemented by the kernel translation units -------------
 
+// A velocity kick v += s a carried out by a direct evaluation's fold right
+// after it writes a: one launch fewer a step, the same bits as kick() (the
+// multiply and the add rounded apart). vx == nullptr: no kick.
+struct FusedKick {
+    double* vx = nullptr;
+    double* vy = nullptr;
+    double* vz = nullptr;
+    double s = 0.0;
+};
+__device__ __forceinline__ double kick_update(double v, double s, double a) {
+#pragma clang fp contract(off)
+    return v + s * a;
+}
+
 // direct.hip: enqueue the direct sum (no synchronisation).
-void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order);
+void direct_enqueue(orr_ctx* ctx, const orr_direct_args& a, const std::uint32_t* order,
+                    const FusedKick& kick = FusedKick{});
 
 // direct.hip, batched: the direct systems of a batch as one tile space.
 // Tables are device arrays; tile/item/slot meaning is given in direct.hip.
@@ -322,7 +337,7 @@ template<typename T> void key_add(std::vector<unsigned char>& key, const T& v) {
 
 // direct.hip: Morton order (when asked for; replayed as a graph where
 // run_captured allows) and the direct sum.
-void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a);
+void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a, const FusedKick& kick = FusedKick{});
 // direct.hip: the same sum with every buffer in the library's static scratch
 // (no allocation, no graph, no phase timing; limits in direct.hip).
 void direct_evaluate_static(orr_ctx* ctx, const orr_direct_args& a);
@@ -399,6 +414,10 @@ void kick(orr_ctx* ctx, std::uint64_t n, double* vx, double* vy, double* vz, con
           const double* ay, const double* az, double scale);
 void drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, const double* vx,
            const double* vy, const double* vz, double scale);
+// kick(v, a, kick_scale) then drift(x, v, drift_scale) in one pass, the same bits.
+void kick_drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, double* vx, double* vy,
+                double* vz, const double* ax, const double* ay, const double* az, double kick_scale,
+                double drift_scale);
 // The arrays of one RK4 step, per component: state, step start, stage sums.
 struct Rk4Arrays {
     double* x[3];

@@ -36,6 +36,29 @@ __global__ __launch_bounds__(256) void kick_kernel(std::uint64_t n, double* __re
     vz[i] = vz[i] + s * az[i];
 }
 
+// A kick, then a drift with the kicked velocity: each element's arithmetic is
+// kick_kernel's twice over, so the result is that of the two launches.
+__global__ __launch_bounds__(256) void kick_drift_kernel(std::uint64_t n, double* __restrict__ x,
+                                                         double* __restrict__ y, double* __restrict__ z,
+                                                         double* __restrict__ vx, double* __restrict__ vy,
+                                                         double* __restrict__ vz,
+                                                         const double* __restrict__ ax,
+                                                         const double* __restrict__ ay,
+                                                         const double* __restrict__ az, double s,
+                                                         double t) {
+    const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ux = vx[i] + s * ax[i];
+    const double uy = vy[i] + s * ay[i];
+    const double uz = vz[i] + s * az[i];
+    vx[i] = ux;
+    vy[i] = uy;
+    vz[i] = uz;
+    x[i] = x[i] + t * ux;
+    y[i] = y[i] + t * uy;
+    z[i] = z[i] + t * uz;
+}
+
 // runge_kutta4.cpp:19-56, one kernel for its four element-wise loops:
 // op 0 begins a step (x0 = x, v0 = v, Sx = v, Sv = a); op 1 moves to a stage
 // (x = x0 + s v, v = v0 + s a, reading v first); op 2 accumulates a stage's
@@ -262,6 +285,16 @@ void kick(orr_ctx* ctx, std::uint64_t n, double* vx, double* vy, double* vz, con
     hipLaunchKernelGGL(kick_kernel, dim3(static_cast<std::uint32_t>((n + 255) / 256)), dim3(256),
                        0, ctx->stream, n, vx, vy, vz, ax, ay, az, s);
     ORR_LAUNCHED("kick");
+}
+
+void kick_drift(orr_ctx* ctx, std::uint64_t n, double* x, double* y, double* z, double* vx, double* vy,
+                double* vz, const double* ax, const double* ay, const double* az, double kick_scale,
+                double drift_scale) {
+    if (n == 0) return;
+    PhaseTimer t(ctx, kIntegrate);
+    hipLaunchKernelGGL(kick_drift_kernel, dim3(static_cast<std::uint32_t>((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, n, x, y, z, vx, vy, vz, ax, ay, az, kick_scale, drift_scale);
+    ORR_LAUNCHED("kick_drift");
 }
 
 void rk4(orr_ctx* ctx, std::uint64_t n, const Rk4Arrays& arrays, int op, double scale) {

@@ -399,3 +399,44 @@ def test_replayed_steps_equal_eager_steps(ctx, integrator, solver, n):
         eager.close()
     # Captured on the second step of each timestep, replayed after: 2 + 3 + 2 replays.
     assert ctx.graph_launches() >= 7
+
+
+@pytest.mark.parametrize("integrator,n", [("velocity-verlet", 4096), ("velocity-verlet", 50000),
+                                          ("yoshida4", 4096)])
+def test_fused_step_equals_host_kicks_around_device_evaluations(ctx, integrator, n):
+    # A device step launches the first half kick and the drift as one kernel and
+    # applies the second half kick inside the direct fold (4096: the one-sided
+    # kernel's fold; 50,000: the pairs path's). The same step rebuilt on the host
+    # -- v += (h/2) a and x += h v in NumPy (a multiply, then an add: never
+    # fused), the field from the engine's own direct evaluation of the drifted
+    # positions -- must give the same bits (velocity_verlet.cpp:59-72,
+    # yoshida4.cpp:24-59).
+    p = oracle.plummer(n, 31)
+    eps, dt = 0.01, 1e-3
+    s = E.DeviceSystem(ctx, n)
+    s.upload(*p.arrays())
+    cfg = E.StepConfig(integrator=integrator, solver="direct", softening=eps)
+    s.refresh(cfg)
+    st = s.download()
+    s.step(cfg, dt, 1)
+    got = s.download()
+    s.close()
+
+    solver = E.HipDirectSolver(E.Softening(eps), context=ctx)
+    cube = 1.2599210498948731647672106
+    outer, inner = 1.0 / (2 - cube), -cube / (2 - cube)
+    subs = [dt] if integrator == "velocity-verlet" else [outer * dt, inner * dt, outer * dt]
+    x = [st["x"], st["y"], st["z"]]
+    v = [st["vx"], st["vy"], st["vz"]]
+    a = [st["ax"], st["ay"], st["az"]]
+    for h in subs:
+        half = h / 2
+        v = [v[c] + half * a[c] for c in range(3)]
+        x = [x[c] + h * v[c] for c in range(3)]
+        a = [np.full(n, np.nan) for _ in range(3)]
+        solver.evaluate(tuple(x), p.m, a)
+        v = [v[c] + half * a[c] for c in range(3)]
+    for c, name in enumerate("xyz"):
+        assert np.array_equal(got[name], x[c]), name
+        assert np.array_equal(got["v" + name], v[c]), "v" + name
+        assert np.array_equal(got["a" + name], a[c]), "a" + name
