@@ -614,12 +614,20 @@ __global__ __launch_bounds__(kBlock) void radix_onesweep(const std::uint64_t* __
 // over the packed keys sorts them to the same permutation. first_digit >= 4
 // keeps the digits within 32 bits beside the 32-bit index.
 constexpr int kSmallThreads = 1024;
+constexpr int kSmallSlots = static_cast<int>(kSmallOrder) / kSmallThreads;  // keys per thread
+
+// Compare-exchange of a key with its partner's: the smaller one where this
+// position keeps the minimum, else the larger.
+__device__ __forceinline__ unsigned long long keep(unsigned long long mine, unsigned long long other,
+                                                   bool keep_min) {
+    const bool other_less = other < mine;
+    return other_less == keep_min ? other : mine;
+}
 
 __global__ __launch_bounds__(kSmallThreads) void morton_small(const double* __restrict__ x,
                                                               const double* __restrict__ y,
                                                               const double* __restrict__ z,
-                                                              std::uint32_t n, std::uint32_t padded,
-                                                              int first_digit,
+                                                              std::uint32_t n, int first_digit,
                                                               double* __restrict__ cube_out,
                                                               std::uint32_t* __restrict__ index) {
     __shared__ unsigned long long keys[kSmallOrder];
@@ -666,10 +674,14 @@ __global__ __launch_bounds__(kSmallThreads) void morton_small(const double* __re
     }
     __syncthreads();
     // Codes as encode computes them, packed (top digits, index); padding last.
+    // Thread t holds positions t + 1024 r.
     const double size = cube[3];
     const double scale = static_cast<double>(kGrid) / size;
     const int shift = 8 * first_digit;
-    for (std::uint32_t i = t; i < padded; i += kSmallThreads) {
+    unsigned long long v[kSmallSlots];
+#pragma unroll
+    for (int r = 0; r < kSmallSlots; ++r) {
+        const std::uint32_t i = t + kSmallThreads * r;
         unsigned long long key = ~0ull;
         if (i < n) {
             std::uint64_t code = 0;
@@ -680,27 +692,66 @@ __global__ __launch_bounds__(kSmallThreads) void morton_small(const double* __re
             }
             key = ((code >> shift) << 32) | i;
         }
-        keys[i] = key;
+        v[r] = key;
     }
-    __syncthreads();
-    // Bitonic sort, ascending, of the `padded` (a power of two) keys.
-    for (std::uint32_t k = 2; k <= padded; k <<= 1) {
-        for (std::uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (std::uint32_t i = t; i < padded; i += kSmallThreads) {
-                const std::uint32_t partner = i ^ j;
-                if (partner > i) {
-                    const unsigned long long a = keys[i], b = keys[partner];
-                    const bool ascending = (i & k) == 0;
-                    if ((a > b) == ascending) {
-                        keys[i] = b;
-                        keys[partner] = a;
-                    }
+    // Bitonic sort, ascending, of all kSmallOrder positions. Partners 1024 or
+    // more apart are in the same thread, partners under 64 apart in the same
+    // wave (shuffles); only the stages in between go through LDS.
+    for (std::uint32_t k = 2; k <= kSmallOrder; k <<= 1) {
+        std::uint32_t j = k >> 1;
+        for (; j >= kSmallThreads; j >>= 1) {
+            const int jr = static_cast<int>(j / kSmallThreads);
+#pragma unroll
+            for (int r = 0; r < kSmallSlots; ++r) {
+                if ((r & jr) != 0) continue;
+                const std::uint32_t i = t + kSmallThreads * r;
+                const bool ascending = (i & k) == 0;
+                const unsigned long long a = v[r], b = v[r | jr];
+                if ((a > b) == ascending) {
+                    v[r] = b;
+                    v[r | jr] = a;
                 }
             }
+        }
+        if (j >= 64) {
+            __syncthreads();  // the last round's reads of keys are done
+#pragma unroll
+            for (int r = 0; r < kSmallSlots; ++r) keys[t + kSmallThreads * r] = v[r];
             __syncthreads();
+            for (; j >= 64; j >>= 1) {
+#pragma unroll
+                for (int r = 0; r < kSmallSlots; ++r) {
+                    const std::uint32_t i = t + kSmallThreads * r;
+                    const std::uint32_t partner = i ^ j;
+                    if (partner > i) {
+                        const unsigned long long a = keys[i], b = keys[partner];
+                        const bool ascending = (i & k) == 0;
+                        if ((a > b) == ascending) {
+                            keys[i] = b;
+                            keys[partner] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int r = 0; r < kSmallSlots; ++r) v[r] = keys[t + kSmallThreads * r];
+        }
+        for (; j > 0; j >>= 1) {
+#pragma unroll
+            for (int r = 0; r < kSmallSlots; ++r) {
+                const std::uint32_t i = t + kSmallThreads * r;
+                const unsigned long long other = __shfl_xor(v[r], static_cast<int>(j), 64);
+                const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+                v[r] = keep(v[r], other, keep_min);
+            }
         }
     }
-    for (std::uint32_t i = t; i < n; i += kSmallThreads) index[i] = static_cast<std::uint32_t>(keys[i]);
+#pragma unroll
+    for (int r = 0; r < kSmallSlots; ++r) {
+        const std::uint32_t i = t + kSmallThreads * r;
+        if (i < n) index[i] = static_cast<std::uint32_t>(v[r]);
+    }
 }
 
 }  // namespace
@@ -875,11 +926,9 @@ void morton_enqueue_small(orr_ctx* ctx, const double* x, const double* y, const 
     if (first_digit < 4) fail(ORR_ERR_INVALID_ARGUMENT, "morton_enqueue_small: first_digit must be at least 4");
     ctx->order_bbox.reserve(4 * sizeof(double));
     ctx->order_index.reserve(sizeof(std::uint32_t) * n);
-    std::uint32_t padded = 2;
-    while (padded < n) padded <<= 1;
     PhaseTimer t(ctx, kSort);
     hipLaunchKernelGGL(morton_small, dim3(1), dim3(kSmallThreads), 0, ctx->stream, x, y, z,
-                       static_cast<std::uint32_t>(n), padded, first_digit, ctx->order_bbox.as<double>(),
+                       static_cast<std::uint32_t>(n), first_digit, ctx->order_bbox.as<double>(),
                        ctx->order_index.as<std::uint32_t>());
     ORR_LAUNCHED("morton_small");
 }
