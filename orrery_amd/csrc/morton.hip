@@ -615,6 +615,8 @@ __global__ __launch_bounds__(kBlock) void radix_onesweep(const std::uint64_t* __
 // keeps the digits within 32 bits beside the 32-bit index.
 constexpr int kSmallThreads = 1024;
 constexpr int kSmallSlots = static_cast<int>(kSmallOrder) / kSmallThreads;  // keys per thread
+// Positions: 6 lane bits, 2 slot bits, 4 wave bits.
+static_assert(kSmallOrder == 4096 && kSmallThreads == 1024 && kSmallSlots == 4, "small-sort layout");
 
 // Compare-exchange of a key with its partner's: the smaller one where this
 // position keeps the minimum, else the larger.
@@ -674,14 +676,17 @@ __global__ __launch_bounds__(kSmallThreads) void morton_small(const double* __re
     }
     __syncthreads();
     // Codes as encode computes them, packed (top digits, index); padding last.
-    // Thread t holds positions t + 1024 r.
+    // Thread t holds the positions whose bits 0-5 are its lane, 6-7 the slot
+    // r and 8-11 its wave: bit b of a position is exchanged in 12 - b stages,
+    // so the low bits, the busiest, stay in the wave or the thread.
+    auto pos = [&](int r) { return lane | (static_cast<std::uint32_t>(r) << 6) | (wave << 8); };
     const double size = cube[3];
     const double scale = static_cast<double>(kGrid) / size;
     const int shift = 8 * first_digit;
     unsigned long long v[kSmallSlots];
 #pragma unroll
     for (int r = 0; r < kSmallSlots; ++r) {
-        const std::uint32_t i = t + kSmallThreads * r;
+        const std::uint32_t i = pos(r);
         unsigned long long key = ~0ull;
         if (i < n) {
             std::uint64_t code = 0;
@@ -694,34 +699,20 @@ __global__ __launch_bounds__(kSmallThreads) void morton_small(const double* __re
         }
         v[r] = key;
     }
-    // Bitonic sort, ascending, of all kSmallOrder positions. Partners 1024 or
-    // more apart are in the same thread, partners under 64 apart in the same
-    // wave (shuffles); only the stages in between go through LDS.
+    // Bitonic sort, ascending, of all kSmallOrder positions: partners 256 or
+    // more apart (another wave) through LDS, 64 or 128 apart in the same
+    // thread, under 64 apart by wave shuffles.
     for (std::uint32_t k = 2; k <= kSmallOrder; k <<= 1) {
         std::uint32_t j = k >> 1;
-        for (; j >= kSmallThreads; j >>= 1) {
-            const int jr = static_cast<int>(j / kSmallThreads);
-#pragma unroll
-            for (int r = 0; r < kSmallSlots; ++r) {
-                if ((r & jr) != 0) continue;
-                const std::uint32_t i = t + kSmallThreads * r;
-                const bool ascending = (i & k) == 0;
-                const unsigned long long a = v[r], b = v[r | jr];
-                if ((a > b) == ascending) {
-                    v[r] = b;
-                    v[r | jr] = a;
-                }
-            }
-        }
-        if (j >= 64) {
+        if (j >= 256) {
             __syncthreads();  // the last round's reads of keys are done
 #pragma unroll
-            for (int r = 0; r < kSmallSlots; ++r) keys[t + kSmallThreads * r] = v[r];
+            for (int r = 0; r < kSmallSlots; ++r) keys[pos(r)] = v[r];
             __syncthreads();
-            for (; j >= 64; j >>= 1) {
+            for (; j >= 256; j >>= 1) {
 #pragma unroll
                 for (int r = 0; r < kSmallSlots; ++r) {
-                    const std::uint32_t i = t + kSmallThreads * r;
+                    const std::uint32_t i = pos(r);
                     const std::uint32_t partner = i ^ j;
                     if (partner > i) {
                         const unsigned long long a = keys[i], b = keys[partner];
@@ -735,12 +726,26 @@ __global__ __launch_bounds__(kSmallThreads) void morton_small(const double* __re
                 __syncthreads();
             }
 #pragma unroll
-            for (int r = 0; r < kSmallSlots; ++r) v[r] = keys[t + kSmallThreads * r];
+            for (int r = 0; r < kSmallSlots; ++r) v[r] = keys[pos(r)];
+        }
+        for (; j >= 64; j >>= 1) {
+            const int jr = static_cast<int>(j >> 6);
+#pragma unroll
+            for (int r = 0; r < kSmallSlots; ++r) {
+                if ((r & jr) != 0) continue;
+                const std::uint32_t i = pos(r);
+                const bool ascending = (i & k) == 0;
+                const unsigned long long a = v[r], b = v[r | jr];
+                if ((a > b) == ascending) {
+                    v[r] = b;
+                    v[r | jr] = a;
+                }
+            }
         }
         for (; j > 0; j >>= 1) {
 #pragma unroll
             for (int r = 0; r < kSmallSlots; ++r) {
-                const std::uint32_t i = t + kSmallThreads * r;
+                const std::uint32_t i = pos(r);
                 const unsigned long long other = __shfl_xor(v[r], static_cast<int>(j), 64);
                 const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
                 v[r] = keep(v[r], other, keep_min);
@@ -749,7 +754,7 @@ __global__ __launch_bounds__(kSmallThreads) void morton_small(const double* __re
     }
 #pragma unroll
     for (int r = 0; r < kSmallSlots; ++r) {
-        const std::uint32_t i = t + kSmallThreads * r;
+        const std::uint32_t i = pos(r);
         if (i < n) index[i] = static_cast<std::uint32_t>(v[r]);
     }
 }
