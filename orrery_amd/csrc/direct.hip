@@ -1370,7 +1370,7 @@ void direct_evaluate(orr_ctx* ctx, const orr_direct_args& a, const FusedKick& ki
     if (a.count == 0) return;
     const std::uint32_t* order = nullptr;
     if (a.spatial_order != 0 && a.count > 64) {
-        // The Morton order is some 35 short launches: replayed as a graph (its
+        // The Morton order is some nine short launches: replayed as a graph (its
         // phases untimed); the three direct launches stay eager and timed.
         std::vector<unsigned char> key;
         key_add(key, a.x);

@@ -608,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void radix_onesweep(const std::uint64_t* __
 }
 
 // Small systems in one workgroup: extrema, cube, codes and the order by the
-// codes' digits from first_digit up, instead of some twenty short launches.
+// codes' digits from first_digit up, instead of nine short launches.
 // The multi-kernel order is a stable sort by those digits, that is a sort by
 // (code >> 8 first_digit, index), a key unique per element; a bitonic network
 // over the packed keys sorts them to the same permutation. first_digit >= 4
