@@ -73,3 +73,14 @@ def test_sort_mode_is_checked(ctx):
     with pytest.raises(E.solvers.C.EngineError):
         ctx.set_tree_config(sort_mode=3)
     assert ctx.tree_config()["sort_mode"] == 1
+
+
+def test_one_sweep_beyond_resident_blocks(ctx):
+    # 8,388,608 keys are 4096 sort blocks, more than the device holds at once,
+    # so the look-back's blocks wait on blocks that started earlier rather than
+    # on blocks all resident together (the 2M case).
+    assert ctx.tree_config()["sort_mode"] == 1
+    rng = np.random.default_rng(17)
+    n = 8_388_608
+    x, y, z = (rng.standard_normal(n) for _ in range(3))
+    check(ctx, x, y, z)
